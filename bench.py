@@ -1,0 +1,221 @@
+#!/usr/bin/env python3
+"""Throughput of the FDTD timestep path on one MI355X.
+
+Workload (default): 256^3 grid, 0.5 mm cells, 8-cell CPML on every face
+(physical mode), 200 seeded dielectric boxes, soft Ez Gaussian (fc = 15 GHz)
+at the centre; energy sampled every 100 steps as the reference's
+Simulation::run does.  A "step" is one full timestep (H update with fused
+CPML, E update with fused CPML, excitation, energy monitoring).
+
+Prints one JSON line:
+  value      MCells/s = cells * steps / wall over the timed steps (inputs and
+             state resident in HBM; the timed region is bracketed by device
+             synchronisation)
+  roofline   the dominant kernel's algorithmic bytes per launch divided by its
+             mean launch time, measured with HIP events on the engine's stream
+             inside the timed region, against 8 TB/s
+  cpu_baseline  the oracle's OpenMP restatement of the reference's
+             ParallelEngine (same arithmetic, AVX-512 when present) timed on
+             the host cores on a bounded sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MCells/s & %HBM peak; sweep time-to-|ΔS11|≤1e-3, learned-stop vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def slab_cells(shape, layers):
+    """Cells carrying CPML psi per direction, summed over directions."""
+    nx, ny, nz = shape
+    tot = 0
+    for d, n in enumerate(shape):
+        other = nx * ny * nz // n
+        L = 2 * layers if 2 * layers < n else 0
+        tot += L * other
+    return tot
+
+
+def kernel_bytes(shape, layers):
+    """Algorithmic HBM bytes of one H (or E) launch: 12 float reads + 3 writes
+    per cell (own 3 components, the other field's 3 components, 6
+    coefficients), plus 2 psi read + 2 psi write per slab cell per direction."""
+    n = shape[0] * shape[1] * shape[2]
+    return 60 * n + 16 * slab_cells(shape, layers)
+
+
+def build_engine(wl, mode):
+    import numpy as np
+
+    from openems_amd import CpmlBoundaries, HipEngine, Operator
+    from openems_amd import workloads as W
+
+    op = Operator.uniform(*wl.shape, wl.delta)
+    W.apply_materials(op, wl)
+    eng = HipEngine(op)
+    exts = []
+    if wl.cpml_layers:
+        exts.append(eng.cpml(CpmlBoundaries(layers=(wl.cpml_layers,) * 6, grading_order=wl.cpml_grading,
+                                            reflection_coeff=wl.cpml_R0, mode=mode)))
+    return op, eng, exts
+
+
+def run_steps(eng, exts, wl, dt, start, n, sample_interval=100):
+    from openems_amd import EngineBatch, ScheduledExcitation, Waveform
+
+    t0, tau = wl.gaussian_t0_tau()
+    smp = Waveform.gaussian(t0, tau).sample(start, n, dt)
+    return eng.run_batch(EngineBatch(num_steps=n, excitations=[ScheduledExcitation(wl.source, wl.source_dir, smp)],
+                                     extensions=exts, sample_interval=sample_interval, track_peak=True))
+
+
+def cpu_baseline(wl, budget_s=12.0, max_steps=200):
+    """Time the oracle (OpenMP, all host threads) on the same workload."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from openems_amd import workloads as W
+
+    sim = O.OracleSim(*wl.shape, wl.delta, 32)
+    W.apply_materials(sim, wl)
+    if wl.cpml_layers:
+        sim.add_cpml((wl.cpml_layers,) * 6, wl.cpml_grading, wl.cpml_R0, 0.0, 1.0, O.MODE_PHYSICAL)
+    sim.yee_steps(1)  # first touch
+    steps = 0
+    t0 = time.perf_counter()
+    while steps < max_steps and time.perf_counter() - t0 < budget_s:
+        sim.yee_steps(2)
+        steps += 2
+    wall = time.perf_counter() - t0
+    threads = O.max_threads()
+    isa = "AVX-512" if O.library_path().endswith("v4.so") else "AVX2"
+    return {
+        "value": wl.cells * steps / wall / 1e6,
+        "unit": "MCells/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} timesteps (H+E with physical CPML) of the {wl.shape[0]}x{wl.shape[1]}x{wl.shape[2]} "
+                  f"workload on {threads} OpenMP threads ({isa}), {wall:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--n", type=int, default=256, help="grid edge of the CPML/boxes workload")
+    ap.add_argument("--mode", choices=["physical", "compat"], default="physical")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--tiles", type=str, default="", help="bx,by,ci,vec override")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from openems_amd import MODE_COMPAT, MODE_PHYSICAL
+    from openems_amd import workloads as W
+
+    wl = W.cpml_boxes(n=args.n, steps=args.steps)
+    mode = MODE_PHYSICAL if args.mode == "physical" else MODE_COMPAT
+    log(f"[bench] building {wl.shape} with {len(wl.boxes)} boxes, CPML {wl.cpml_layers} ({args.mode})")
+    op, eng, exts = build_engine(wl, mode)
+    if args.tiles:
+        bx, by, ci, vec = (int(x) for x in args.tiles.split(","))
+        eng.set_tiles(bx, by, ci, vec)
+    dt = op.dt
+    log(f"[bench] tiles {eng.tiles()}; warmup {args.warmup} steps")
+    if args.warmup:
+        run_steps(eng, exts, wl, dt, 0, args.warmup)
+
+    eng.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = run_steps(eng, exts, wl, dt, args.warmup, args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    eng.set_profiling(False)
+    assert res.timesteps_executed == args.steps
+    log(f"[bench] timed {args.steps} steps in {wall:.3f} s")
+
+    kern = {}
+    for name in ("update_h", "update_e", "point_ops", "energy"):
+        n, ms = eng.kernel_time(name)
+        if n:
+            kern[name] = (n, ms / n)
+    dom = max(("update_h", "update_e"), key=lambda k: kern[k][0] * kern[k][1])
+    avg_ms = kern[dom][1]
+    kb = kernel_bytes(wl.shape, wl.cpml_layers)
+    achieved = kb / (avg_ms * 1e-3) / 1e9
+    for k, (n, ms) in kern.items():
+        log(f"[bench] {k}: {n} launches, mean {ms * 1e3:.1f} us")
+    E, H = eng.read_fields()
+    import numpy as np
+
+    finite = bool(np.isfinite(E).all() and np.isfinite(H).all())
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline (oracle, OpenMP) ...")
+        cpu = cpu_baseline(wl, budget_s=args.cpu_budget)
+        log(f"[bench] CPU baseline {cpu['value']:.1f} MCells/s")
+
+    mcells = wl.cells * args.steps / wall / 1e6
+    step_bytes = 2 * kb
+    out = {
+        "metric": METRIC,
+        "value": round(mcells, 1),
+        "unit": "MCells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"cpml_boxes: {wl.shape[0]}x{wl.shape[1]}x{wl.shape[2]} FP32 Yee, 0.5 mm cells, "
+                        f"{wl.cpml_layers}-cell CPML ({args.mode}) on all faces, {len(wl.boxes)} seeded dielectric "
+                        f"boxes, soft Ez Gaussian fc=15 GHz, energy sampled every 100 steps",
+            "grid": list(wl.shape),
+            "tiles": eng.tiles(),
+        },
+        "hbm_frac_step": round(step_bytes * args.steps / wall / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_launch": kb,
+            "mean_launch_us": round(avg_ms * 1e3, 2),
+        },
+        "fields_finite": finite,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

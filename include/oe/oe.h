@@ -1,0 +1,293 @@
+/*
+ * oe.h -- C ABI of the MI355X (gfx950) FDTD timestep engine.
+ *
+ * Drop-in boundary for the timestep loop of the Rust openEMS port
+ * (umgefahren/openems).  Every entry point names the reference interface it
+ * replaces (paths relative to the reference root).  Plain C types only: host
+ * pointers and sizes in, status codes out.  Field and coefficient arrays use
+ * the reference's layout, one float32 array per component, z fastest:
+ * index = (i * ny + j) * nz + k  (src/arrays/mod.rs:61-64).
+ *
+ * Threading: an engine handle is used by one caller at a time, as the
+ * reference's `&mut self` methods require.  Each engine owns one HIP stream.
+ * Errors never abort: every call returns an oe_status and oe_last_error()
+ * gives the thread-local message (mirrors crate::Error, src/lib.rs:61-88).
+ */
+#ifndef OE_OE_H
+#define OE_OE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ errors */
+/* One code per variant of crate::Error (src/lib.rs:61-88), plus DEVICE for
+ * HIP runtime failures, which the reference's GPU engine panics on
+ * (src/fdtd/gpu_engine.rs:88,108). */
+typedef enum {
+    OE_OK = 0,
+    OE_ERR_CONFIG = 1,
+    OE_ERR_IO = 2,
+    OE_ERR_GRID = 3,
+    OE_ERR_MATERIAL = 4,
+    OE_ERR_NUMERICAL = 5,
+    OE_ERR_XML = 6,
+    OE_ERR_DEVICE = 7
+} oe_status;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* oe_last_error(void);
+
+/* Library version string. */
+const char* oe_version(void);
+
+/* ---------------------------------------------------------------- operator */
+typedef struct oe_operator oe_operator;
+
+/* Grid::uniform + Operator::new (src/geometry/mod.rs:60-66,
+ * src/fdtd/operator.rs:59-77): dt = 0.99 CFL, vacuum Ca=Da=1,
+ * Cb = dt/(eps0 d), Db = -dt/(mu0 d).  delta[3] are the cell sizes in metres. */
+oe_status oe_operator_create(size_t nx, size_t ny, size_t nz, const double delta[3],
+                             oe_operator** out);
+
+/* Grid::cartesian + Operator::new with explicit mesh lines (nx+1 .. lines per
+ * axis).  As in the reference (src/geometry/mod.rs:113-118,
+ * src/fdtd/operator.rs:82-89) only the minimum spacing per axis enters dt
+ * and the coefficients. */
+oe_status oe_operator_create_lines(const double* x, size_t nxl, const double* y, size_t nyl,
+                                   const double* z, size_t nzl, oe_operator** out);
+
+void oe_operator_destroy(oe_operator* op);
+
+/* Operator::timestep (src/fdtd/operator.rs:145). */
+double oe_operator_timestep(const oe_operator* op);
+
+/* Operator::dimensions (src/fdtd/operator.rs:157). */
+void oe_operator_dims(const oe_operator* op, size_t dims[3]);
+
+/* Operator::set_material (src/fdtd/operator.rs:187-244).  region is
+ * [i0, i1, j0, j1, k0, k1) half-open; later calls overwrite earlier ones. */
+oe_status oe_operator_set_material(oe_operator* op, double eps_r, double mu_r, double sigma_e,
+                                   double sigma_m, const size_t region[6]);
+
+/* Operator::set_pec (src/fdtd/operator.rs:249-263): Ca = Cb = 0. */
+oe_status oe_operator_set_pec(oe_operator* op, const size_t region[6]);
+
+/* ------------------------------------------------------------------ engine */
+typedef struct oe_engine oe_engine;
+typedef struct oe_extension oe_extension;
+
+/* EngineImpl::new (src/fdtd/engine_impl.rs:32; clone of the coefficient
+ * arrays at src/fdtd/basic_engine.rs:42-54).  The operator's coefficients
+ * are materialised in HBM; the operator is only borrowed. */
+oe_status oe_engine_create(const oe_operator* op, int device, oe_engine** out);
+
+/* EngineImpl::new from explicit coefficient arrays: coeff[12] are host
+ * pointers to N floats each, ordered Ca_x, Ca_y, Ca_z, Cb_x, Cb_y, Cb_z,
+ * Da_x, Da_y, Da_z, Db_x, Db_y, Db_z (the vv/vi/ii/iv arrays; Db keeps the
+ * reference's negative sign, src/fdtd/operator.rs:129-131). */
+oe_status oe_engine_create_from_arrays(size_t nx, size_t ny, size_t nz, const double delta[3],
+                                       double dt, const float* const coeff[12], int device,
+                                       oe_engine** out);
+
+void oe_engine_destroy(oe_engine* eng);
+
+/* EngineImpl::current_timestep (src/fdtd/engine_impl.rs:53). */
+uint64_t oe_engine_timestep(const oe_engine* eng);
+
+/* EngineImpl::read_fields (src/fdtd/engine_impl.rs:59): copies E then H
+ * (6 arrays of N floats, each may be NULL to skip) from HBM. */
+oe_status oe_engine_read_fields(oe_engine* eng, float* const fields[6]);
+
+/* EngineImpl::write_fields (src/fdtd/engine_impl.rs:65): uploads the given
+ * arrays (NULL entries are left unchanged). */
+oe_status oe_engine_write_fields(oe_engine* eng, const float* const fields[6]);
+
+/* Copy one of the 12 coefficient arrays back (order as in
+ * oe_engine_create_from_arrays). */
+oe_status oe_engine_read_coefficients(oe_engine* eng, int which, float* out);
+
+/* EngineImpl::reset (src/fdtd/engine_impl.rs:70): zero fields and the step
+ * counter (extension state is reset with oe_extension_reset). */
+oe_status oe_engine_reset(oe_engine* eng);
+
+/* Sum of squares of E and of H in float64 (Field3D::energy,
+ * src/arrays/field.rs:219-221; VectorField3D::energy :314-316). */
+oe_status oe_engine_energy(oe_engine* eng, double* e_energy, double* h_energy);
+
+/* -------------------------------------------------------------- batch ABI */
+/* ScheduledExcitation with a Sampled waveform (src/fdtd/batch.rs:71-100):
+ * samples[s] is applied after step s of the batch (soft: E += v, hard:
+ * E = v), exactly as BasicEngine::apply_excitations
+ * (src/fdtd/basic_engine.rs:313-395).  Steps past n_samples apply nothing. */
+typedef struct {
+    uint32_t i, j, k;
+    uint32_t direction; /* 0 = x, 1 = y, 2 = z */
+    uint32_t soft;      /* 1 = soft (add), 0 = hard (set) */
+    const float* samples;
+    uint64_t n_samples;
+} oe_excitation;
+
+/* EnergySample (src/fdtd/batch.rs:197-209). */
+typedef struct {
+    uint64_t timestep;
+    double e_energy;
+    double h_energy;
+    double total_energy;
+} oe_energy_sample;
+
+/* EngineBatch + TerminationConfig + EnergyMonitorConfig
+ * (src/fdtd/batch.rs:17-32, 150-193).  has_* flags encode Option<T>. */
+typedef struct {
+    int has_num_steps;
+    uint64_t num_steps;
+    const oe_excitation* excitations;
+    size_t n_excitations;
+    oe_extension* const* extensions;
+    size_t n_extensions;
+    /* TerminationConfig */
+    int has_max_timesteps;
+    uint64_t max_timesteps;
+    int has_energy_decay_db;
+    double energy_decay_db;
+    uint64_t check_interval; /* reference default 100 */
+    /* EnergyMonitorConfig */
+    uint64_t sample_interval; /* 0 = disabled */
+    int track_peak;
+    /* Extension to the reference (fixes its per-batch peak reset): when set,
+     * the peak used by the decay test persists across batches until
+     * oe_engine_reset. */
+    int persistent_peak;
+    /* Caller-owned output buffer for energy samples. */
+    oe_energy_sample* energy_samples;
+    size_t energy_samples_capacity;
+} oe_batch_desc;
+
+/* TerminationReason (src/fdtd/batch.rs:52-67). */
+typedef enum {
+    OE_TERM_STEPS_COMPLETED = 0,
+    OE_TERM_ENERGY_DECAY = 1,
+    OE_TERM_EXTENSION_STOP = 2
+} oe_termination;
+
+/* BatchResult (src/fdtd/batch.rs:36-48). */
+typedef struct {
+    uint64_t timesteps_executed;
+    int termination_reason; /* oe_termination */
+    double final_decay_db;
+    size_t n_energy_samples; /* may exceed the capacity; only capacity are written */
+    double elapsed_seconds;
+} oe_batch_result;
+
+/* EngineImpl::run_batch (src/fdtd/engine_impl.rs:48-50; loop of
+ * BasicEngine::run_batch, src/fdtd/basic_engine.rs:56-163).  Per step:
+ *   H update (+CPML H correction, fused) -> pre-E hooks (Mur capture, ADE
+ *   capture) -> E update (+CPML E correction, fused) -> post-E hooks (ADE,
+ *   lumped port, Mur) -> excitations -> probes/DFT -> energy sample and
+ *   termination test. */
+oe_status oe_engine_run_batch(oe_engine* eng, const oe_batch_desc* batch, oe_batch_result* result);
+
+/* Kernel timing of the last run_batch, measured with HIP events on the
+ * engine's stream.  names: "update_h", "update_e", ...; returns the number
+ * of launches and total milliseconds for that kernel. */
+oe_status oe_engine_kernel_time(oe_engine* eng, const char* name, uint64_t* launches, double* total_ms);
+
+/* Enable/disable per-launch event timing (off by default). */
+oe_status oe_engine_set_profiling(oe_engine* eng, int on);
+
+/* Tile shape of the Yee kernels: bx lanes along z (in vector units), by rows
+ * along y, ci planes along x per work-group; 0 keeps the automatic choice.
+ * vec = 4 uses 16-byte accesses (requires nz % 4 == 0), 1 scalar, 0 auto. */
+oe_status oe_engine_set_tiles(oe_engine* eng, int bx, int by, int ci, int vec);
+oe_status oe_engine_get_tiles(const oe_engine* eng, int tiles[4]);
+
+/* ------------------------------------------------------------- waveforms */
+/* ExcitationType (src/fdtd/excitation.rs:9-42). */
+typedef enum {
+    OE_EXC_GAUSSIAN = 0,           /* exp(-((t - t0) / tau)^2) */
+    OE_EXC_SINUSOIDAL = 1,         /* sin(2 pi f t) */
+    OE_EXC_GAUSSIAN_MODULATED = 2, /* sin(2 pi f t) exp(-((t - t0) / tau)^2) */
+    OE_EXC_DIRAC = 3,              /* 1 at |t| < 1e-15, else 0 */
+    OE_EXC_STEP = 4,               /* 1 for t >= 0 */
+    OE_EXC_CUSTOM = 5              /* linear interpolation of (times, values) */
+} oe_waveform_type;
+
+typedef struct {
+    int type;         /* oe_waveform_type */
+    double t0, tau;   /* Gaussian variants */
+    double frequency; /* Sinusoidal, GaussianModulated */
+    const double* times;  /* Custom */
+    const double* values; /* Custom */
+    size_t n_points;      /* Custom */
+    double amplitude;
+} oe_waveform;
+
+/* Excitation::evaluate (src/fdtd/excitation.rs:150-200), amplitude included. */
+oe_status oe_waveform_evaluate(const oe_waveform* w, double t, double* value);
+
+/* ScheduledExcitation::from_excitation (src/fdtd/batch.rs:110-129):
+ * out[s] = (float) evaluate((start_timestep + s) * dt) for s < n. */
+oe_status oe_waveform_sample(const oe_waveform* w, uint64_t start_timestep, uint64_t n, double dt, float* out);
+
+/* -------------------------------------------------------------- extensions */
+/* Extensions are device-resident objects bound to one engine and passed to
+ * run_batch by pointer (their state persists across batches).  They replace
+ * the CPU callbacks of the Extension trait (src/extensions/extension_trait.rs:
+ * 32-167) at the fixed hook points listed at oe_engine_run_batch. */
+
+typedef enum { OE_MODE_COMPAT = 0, OE_MODE_PHYSICAL = 1 } oe_mode;
+
+/* CpmlBoundaries + Cpml::new (src/extensions/cpml.rs:27-51, 235-304).
+ * mode COMPAT reproduces the reference exactly (psi recursions updated, no
+ * field correction, :343-421); PHYSICAL applies the Roden-Gedney correction
+ * with the freshly updated psi.  Only one CPML may be attached per engine;
+ * it is fused into the Yee kernels. */
+typedef struct {
+    size_t layers[6]; /* xmin, xmax, ymin, ymax, zmin, zmax */
+    double grading_order;    /* default 3 */
+    double reflection_coeff; /* default 1e-6 */
+    double alpha_max;        /* default 0 */
+    double kappa_max;        /* default 1 */
+    int mode;                /* oe_mode */
+} oe_cpml_config;
+oe_status oe_ext_cpml_create(oe_engine* eng, const oe_cpml_config* cfg, oe_extension** out);
+
+/* MurAbcConfig + MurAbc::new (src/extensions/mur_abc.rs:14-28, 173-200).
+ * phase_velocity <= 0 selects c0. */
+oe_status oe_ext_mur_create(oe_engine* eng, const int enabled[6], double phase_velocity,
+                            oe_extension** out);
+
+/* LorentzMaterial / DrudeMaterial / DebyeMaterial ::new
+ * (src/extensions/dispersive.rs:134-159, 476-498, 570-592).
+ * params: Lorentz {omega_0, gamma, delta_eps}; Drude {omega_p, gamma};
+ * Debye {tau, delta_eps}.  positions: n triples (i, j, k). */
+typedef enum { OE_ADE_LORENTZ = 0, OE_ADE_DRUDE = 1, OE_ADE_DEBYE = 2 } oe_ade_kind;
+oe_status oe_ext_ade_create(oe_engine* eng, int kind, const double params[3],
+                            const uint32_t* positions, size_t n, int mode, oe_extension** out);
+
+/* FieldProbe (src/processing/probes.rs:263-303): records Ex..Hz at a node
+ * after every step.  Data are read with oe_ext_probe_read. */
+oe_status oe_ext_field_probe_create(oe_engine* eng, uint32_t i, uint32_t j, uint32_t k,
+                                    oe_extension** out);
+
+/* Number of samples recorded and copy of [n][6] float64 values. */
+oe_status oe_ext_probe_read(oe_extension* ext, double* out, size_t capacity, size_t* n_samples);
+
+/* CPML psi arrays / ADE state for inspection: field 0 = E, 1 = H; which 0 =
+ * psi_pos, 1 = psi_neg.  Returns the element count in *n. */
+oe_status oe_ext_cpml_read_psi(oe_extension* ext, int face, int field, int which, float* out,
+                               size_t capacity, size_t* n);
+oe_status oe_ext_ade_read_state(oe_extension* ext, float* out, size_t capacity, size_t* n);
+
+/* Extension::pre_batch/post_batch reset counterpart: zero the state. */
+oe_status oe_extension_reset(oe_extension* ext);
+void oe_extension_destroy(oe_extension* ext);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OE_OE_H */

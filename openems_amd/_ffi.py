@@ -1,0 +1,137 @@
+"""ctypes declarations for liboe.so (include/oe/oe.h).
+
+The library is the product: there is no Python or CPU compute path behind
+these bindings.  If liboe.so is missing, or no HIP device is present when an
+engine is created, the calls raise instead of falling back.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboe.so")
+
+OE_OK = 0
+STATUS_NAMES = {0: "OK", 1: "CONFIG", 2: "IO", 3: "GRID", 4: "MATERIAL", 5: "NUMERICAL", 6: "XML", 7: "DEVICE"}
+
+
+class OEError(RuntimeError):
+    """Raised for a non-OK oe_status (mirrors crate::Error, src/lib.rs:61-88)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Excitation(C.Structure):
+    _fields_ = [("i", C.c_uint32), ("j", C.c_uint32), ("k", C.c_uint32), ("direction", C.c_uint32),
+                ("soft", C.c_uint32), ("samples", C.POINTER(C.c_float)), ("n_samples", C.c_uint64)]
+
+
+class EnergySample(C.Structure):
+    _fields_ = [("timestep", C.c_uint64), ("e_energy", C.c_double), ("h_energy", C.c_double),
+                ("total_energy", C.c_double)]
+
+
+class BatchDesc(C.Structure):
+    _fields_ = [
+        ("has_num_steps", C.c_int), ("num_steps", C.c_uint64),
+        ("excitations", C.POINTER(Excitation)), ("n_excitations", C.c_size_t),
+        ("extensions", C.POINTER(C.c_void_p)), ("n_extensions", C.c_size_t),
+        ("has_max_timesteps", C.c_int), ("max_timesteps", C.c_uint64),
+        ("has_energy_decay_db", C.c_int), ("energy_decay_db", C.c_double),
+        ("check_interval", C.c_uint64),
+        ("sample_interval", C.c_uint64), ("track_peak", C.c_int),
+        ("persistent_peak", C.c_int),
+        ("energy_samples", C.POINTER(EnergySample)), ("energy_samples_capacity", C.c_size_t),
+    ]
+
+
+class BatchResultC(C.Structure):
+    _fields_ = [("timesteps_executed", C.c_uint64), ("termination_reason", C.c_int),
+                ("final_decay_db", C.c_double), ("n_energy_samples", C.c_size_t),
+                ("elapsed_seconds", C.c_double)]
+
+
+class Waveform(C.Structure):
+    _fields_ = [("type", C.c_int), ("t0", C.c_double), ("tau", C.c_double), ("frequency", C.c_double),
+                ("times", C.POINTER(C.c_double)), ("values", C.POINTER(C.c_double)), ("n_points", C.c_size_t),
+                ("amplitude", C.c_double)]
+
+
+class CpmlConfig(C.Structure):
+    _fields_ = [("layers", C.c_size_t * 6), ("grading_order", C.c_double), ("reflection_coeff", C.c_double),
+                ("alpha_max", C.c_double), ("kappa_max", C.c_double), ("mode", C.c_int)]
+
+
+# name -> (restype, argtypes); every function declared in include/oe/oe.h
+_vp = C.c_void_p
+_pp = C.POINTER(C.c_void_p)
+_sz = C.c_size_t
+_d = C.c_double
+_i = C.c_int
+_u32 = C.c_uint32
+_u64 = C.c_uint64
+_fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
+_szp = C.POINTER(C.c_size_t)
+
+SIGNATURES = {
+    "oe_last_error": (C.c_char_p, []),
+    "oe_version": (C.c_char_p, []),
+    "oe_operator_create": (_i, [_sz, _sz, _sz, _dp, _pp]),
+    "oe_operator_create_lines": (_i, [_dp, _sz, _dp, _sz, _dp, _sz, _pp]),
+    "oe_operator_destroy": (None, [_vp]),
+    "oe_operator_timestep": (_d, [_vp]),
+    "oe_operator_dims": (None, [_vp, _szp]),
+    "oe_operator_set_material": (_i, [_vp, _d, _d, _d, _d, _szp]),
+    "oe_operator_set_pec": (_i, [_vp, _szp]),
+    "oe_engine_create": (_i, [_vp, _i, _pp]),
+    "oe_engine_create_from_arrays": (_i, [_sz, _sz, _sz, _dp, _d, C.POINTER(_fp), _i, _pp]),
+    "oe_engine_destroy": (None, [_vp]),
+    "oe_engine_timestep": (_u64, [_vp]),
+    "oe_engine_read_fields": (_i, [_vp, C.POINTER(_fp)]),
+    "oe_engine_write_fields": (_i, [_vp, C.POINTER(_fp)]),
+    "oe_engine_read_coefficients": (_i, [_vp, _i, _fp]),
+    "oe_engine_reset": (_i, [_vp]),
+    "oe_engine_energy": (_i, [_vp, _dp, _dp]),
+    "oe_engine_run_batch": (_i, [_vp, C.POINTER(BatchDesc), C.POINTER(BatchResultC)]),
+    "oe_engine_kernel_time": (_i, [_vp, C.c_char_p, C.POINTER(_u64), _dp]),
+    "oe_engine_set_profiling": (_i, [_vp, _i]),
+    "oe_engine_set_tiles": (_i, [_vp, _i, _i, _i, _i]),
+    "oe_engine_get_tiles": (_i, [_vp, C.POINTER(_i)]),
+    "oe_waveform_evaluate": (_i, [C.POINTER(Waveform), _d, _dp]),
+    "oe_waveform_sample": (_i, [C.POINTER(Waveform), _u64, _u64, _d, _fp]),
+    "oe_ext_cpml_create": (_i, [_vp, C.POINTER(CpmlConfig), _pp]),
+    "oe_ext_mur_create": (_i, [_vp, C.POINTER(_i), _d, _pp]),
+    "oe_ext_ade_create": (_i, [_vp, _i, _dp, C.POINTER(_u32), _sz, _i, _pp]),
+    "oe_ext_field_probe_create": (_i, [_vp, _u32, _u32, _u32, _pp]),
+    "oe_ext_probe_read": (_i, [_vp, _dp, _sz, _szp]),
+    "oe_ext_cpml_read_psi": (_i, [_vp, _i, _i, _i, _fp, _sz, _szp]),
+    "oe_ext_ade_read_state": (_i, [_vp, _fp, _sz, _szp]),
+    "oe_extension_reset": (_i, [_vp]),
+    "oe_extension_destroy": (None, [_vp]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load liboe.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m openems_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int):
+    if status != OE_OK:
+        raise OEError(status, lib().oe_last_error().decode(errors="replace"))

@@ -1,0 +1,1043 @@
+// oe_api.hip -- host runtime behind include/oe/oe.h: operator, engine
+// lifecycle, the per-step batch loop and the device-resident extensions.
+//
+// Mirrors Operator (src/fdtd/operator.rs), EngineImpl
+// (src/fdtd/engine_impl.rs:24-71), the batch types (src/fdtd/batch.rs) and
+// BasicEngine::run_batch (src/fdtd/basic_engine.rs:56-163).  No CPU compute
+// fallback exists: every field update runs in a HIP kernel on the engine's
+// stream; a missing or failing device is reported as OE_ERR_DEVICE.
+
+#include "oe/oe.h"
+#include "oe_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+using namespace oe;
+
+// ------------------------------------------------------------------ errors
+namespace {
+thread_local std::string g_err;
+
+oe_status fail(oe_status code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_OK(expr)                                                                            \
+    do {                                                                                        \
+        hipError_t _e = (expr);                                                                 \
+        if (_e != hipSuccess)                                                                   \
+            return fail(OE_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                        __FILE__, __LINE__);                                                    \
+    } while (0)
+
+#define OE_TRY(expr)                      \
+    do {                                  \
+        oe_status _s = (expr);            \
+        if (_s != OE_OK) return _s;       \
+    } while (0)
+}  // namespace
+
+// ---------------------------------------------------------------- operator
+struct MatOp {
+    int pec;
+    double eps_r, mu_r, se, sm;
+    size_t box[6];
+};
+
+struct oe_operator {
+    size_t nx, ny, nz;
+    double d[3];
+    double dt;
+    std::vector<MatOp> ops;
+};
+
+// ------------------------------------------------------------ extensions
+enum ExtKind { EXT_CPML = 1, EXT_MUR = 2, EXT_ADE = 3, EXT_PROBE = 4 };
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    hipError_t alloc(size_t n) {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        bytes = n;
+        return n ? hipMalloc(&p, n) : hipSuccess;
+    }
+    template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+struct oe_extension {
+    int kind = 0;
+    oe_engine* eng = nullptr;
+    // CPML
+    CpmlArgs cpml{};
+    DevBuf cpml_mem;    // psi arrays
+    DevBuf cpml_tab;
+    size_t psi_count[6] = {0, 0, 0, 0, 0, 0};
+    // Mur
+    std::vector<MurDev> mur_faces;
+    DevBuf mur_mem;
+    DevBuf mur_dev;     // device copy of mur_faces
+    // ADE
+    AdeDev ade{};
+    DevBuf ade_mem;
+    // probe
+    uint32_t probe_idx = 0;
+    DevBuf probe_mem;   // data
+    DevBuf probe_cnt;   // uint64 counter
+    uint64_t probe_cap = 0;
+};
+
+// ------------------------------------------------------------------ engine
+struct KernelStat {
+    uint64_t launches = 0;
+    double ms = 0.0;
+};
+
+struct oe_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Grid g{};
+    double d[3] = {0, 0, 0};
+    double dt = 0;
+    float* F = nullptr;   // 6N
+    float* Cf = nullptr;  // 12N
+    uint64_t timestep = 0;
+    Tiles tiles{};
+    int xcd_remap = 1;
+    double peak_persistent = 0.0;
+    oe_extension* cpml_owner = nullptr;
+    // energy
+    double* partials = nullptr;
+    double* energy_dev = nullptr;   // ring of [cap][2]
+    size_t energy_dev_cap = 0;
+    double* energy_host = nullptr;  // pinned
+    // point ops
+    DevBuf exc_samples;
+    DevBuf exc_table;
+    DevBuf probe_table;
+    // profiling
+    int profiling = 0;
+    std::map<std::string, KernelStat> stats;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> event_pool;
+};
+
+namespace {
+
+oe_status set_device(int dev) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+        return fail(OE_ERR_DEVICE, "no HIP device available (%s)", hipGetErrorString(e));
+    if (dev < 0 || dev >= n) return fail(OE_ERR_CONFIG, "device %d out of range (have %d)", dev, n);
+    HIP_OK(hipSetDevice(dev));
+    return OE_OK;
+}
+
+int pow2ceil(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// Automatic tile choice: 256-lane work-groups, as many lanes along k as the
+// row holds (up to one wave of 64 float4 lanes), and i-chunks sized for
+// about 1024 work-groups (4 per CU on 256 CUs).
+Tiles auto_tiles(const Grid& g, int vec_req) {
+    Tiles t{};
+    t.vec = (vec_req == 1) ? 1 : ((g.nz % 4 == 0) ? 4 : 1);
+    if (vec_req == 4 && g.nz % 4 != 0) t.vec = 1;
+    const int nzv = (g.nz + t.vec - 1) / t.vec;
+    t.bx = std::min(64, pow2ceil(std::max(nzv, 1)));
+    t.by = 256 / t.bx;
+    const int n_kb = (nzv + t.bx - 1) / t.bx;
+    const int n_jb = (g.ny + t.by - 1) / t.by;
+    const long plane_blocks = (long)n_kb * n_jb;
+    long n_ib = (1024 + plane_blocks - 1) / plane_blocks;
+    n_ib = std::max(1L, std::min(n_ib, (long)g.nx));
+    int ci = (int)((g.nx + n_ib - 1) / n_ib);
+    ci = std::max(ci, std::min(4, g.nx));
+    t.ci = ci;
+    return t;
+}
+
+YeeArgs yee_args(const oe_engine* e) {
+    YeeArgs a{};
+    a.g = e->g;
+    a.F = e->F;
+    a.Cf = e->Cf;
+    a.t = e->tiles;
+    const int nzv = (e->g.nz + a.t.vec - 1) / a.t.vec;
+    a.n_kb = (nzv + a.t.bx - 1) / a.t.bx;
+    a.n_jb = (e->g.ny + a.t.by - 1) / a.t.by;
+    a.n_ib = (e->g.nx + a.t.ci - 1) / a.t.ci;
+    a.xcd_remap = e->xcd_remap;
+    return a;
+}
+
+hipEvent_t take_event(oe_engine* e) {
+    if (!e->event_pool.empty()) {
+        hipEvent_t ev = e->event_pool.back();
+        e->event_pool.pop_back();
+        return ev;
+    }
+    hipEvent_t ev = nullptr;
+    if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+    return ev;
+}
+
+struct Timed {
+    oe_engine* e;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timed(oe_engine* eng, const char* n) : e(eng), name(n) {
+        if (e->profiling) {
+            a = take_event(e);
+            b = take_event(e);
+            if (a) (void)hipEventRecord(a, e->stream);
+        }
+    }
+    ~Timed() {
+        if (e->profiling && a && b) {
+            (void)hipEventRecord(b, e->stream);
+            e->pending.push_back({name, {a, b}});
+        }
+    }
+};
+
+void collect_timing(oe_engine* e) {
+    for (auto& pr : e->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pr.second.first, pr.second.second) == hipSuccess) {
+            auto& st = e->stats[pr.first];
+            st.launches += 1;
+            st.ms += ms;
+        }
+        e->event_pool.push_back(pr.second.first);
+        e->event_pool.push_back(pr.second.second);
+    }
+    e->pending.clear();
+}
+
+oe_status check_region(const oe_operator* op, const size_t r[6]) {
+    if (r[0] > r[1] || r[2] > r[3] || r[4] > r[5])
+        return fail(OE_ERR_MATERIAL, "region start exceeds end");
+    return OE_OK;
+}
+
+oe_status materialise(oe_engine* e, const oe_operator* op) {
+    const Grid& g = e->g;
+    const size_t N = g.N;
+    const double dt = op->dt;
+    // init_e/h_coefficients (src/fdtd/operator.rs:92-141)
+    for (int d = 0; d < 3; ++d) {
+        HIP_OK(launch_fill(e->Cf + (size_t)d * N, N, 1.0f, e->stream));
+        HIP_OK(launch_fill(e->Cf + (size_t)(3 + d) * N, N, (float)(dt / (EPS0 * op->d[d])), e->stream));
+        HIP_OK(launch_fill(e->Cf + (size_t)(6 + d) * N, N, 1.0f, e->stream));
+        HIP_OK(launch_fill(e->Cf + (size_t)(9 + d) * N, N, (float)(-(dt / (MU0 * op->d[d]))), e->stream));
+    }
+    for (const MatOp& m : op->ops) {
+        int box[6];
+        const size_t lim[6] = {0, (size_t)g.nx, 0, (size_t)g.ny, 0, (size_t)g.nz};
+        for (int q = 0; q < 6; ++q) box[q] = (int)std::min(m.box[q], lim[q | 1]);
+        if (m.pec) {  // src/fdtd/operator.rs:249-263
+            for (int d = 0; d < 6; ++d) HIP_OK(launch_box_fill(e->Cf + (size_t)d * N, g, box, 0.0f, e->stream));
+            continue;
+        }
+        // src/fdtd/operator.rs:195-241 (computed in double, stored as float)
+        const double eps = m.eps_r * EPS0;
+        const double loss_e = m.se * dt / (2.0 * eps);
+        const float ca = (float)((1.0 - loss_e) / (1.0 + loss_e));
+        const double cbf = (dt / eps) / (1.0 + loss_e);
+        const double mu = m.mu_r * MU0;
+        const double loss_m = m.sm * dt / (2.0 * mu);
+        const float da = (float)((1.0 - loss_m) / (1.0 + loss_m));
+        const double dbf = -(dt / mu) / (1.0 + loss_m);
+        for (int d = 0; d < 3; ++d) {
+            HIP_OK(launch_box_fill(e->Cf + (size_t)d * N, g, box, ca, e->stream));
+            HIP_OK(launch_box_fill(e->Cf + (size_t)(3 + d) * N, g, box, (float)(cbf / op->d[d]), e->stream));
+            HIP_OK(launch_box_fill(e->Cf + (size_t)(6 + d) * N, g, box, da, e->stream));
+            HIP_OK(launch_box_fill(e->Cf + (size_t)(9 + d) * N, g, box, (float)(dbf / op->d[d]), e->stream));
+        }
+    }
+    return OE_OK;
+}
+
+oe_status engine_alloc(oe_engine* e, size_t nx, size_t ny, size_t nz, int device) {
+    if (nx == 0 || ny == 0 || nz == 0) return fail(OE_ERR_GRID, "empty grid %zux%zux%zu", nx, ny, nz);
+    if (nx > (1u << 30) || ny > (1u << 30) || nz > (1u << 30) || nx * ny * nz >= (1ull << 32))
+        return fail(OE_ERR_GRID, "grid %zux%zux%zu exceeds 2^32 cells", nx, ny, nz);
+    OE_TRY(set_device(device));
+    e->device = device;
+    e->g.nx = (int)nx; e->g.ny = (int)ny; e->g.nz = (int)nz;
+    e->g.N = nx * ny * nz;
+    HIP_OK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    HIP_OK(hipMalloc(&e->F, 6 * e->g.N * sizeof(float)));
+    HIP_OK(hipMemsetAsync(e->F, 0, 6 * e->g.N * sizeof(float), e->stream));
+    HIP_OK(hipMalloc(&e->Cf, 12 * e->g.N * sizeof(float)));
+    HIP_OK(hipMalloc(&e->partials, 2 * energy_blocks() * sizeof(double)));
+    e->energy_dev_cap = 4096;
+    HIP_OK(hipMalloc(&e->energy_dev, 2 * e->energy_dev_cap * sizeof(double)));
+    HIP_OK(hipHostMalloc(&e->energy_host, 2 * e->energy_dev_cap * sizeof(double), hipHostMallocDefault));
+    e->tiles = auto_tiles(e->g, 0);
+    return OE_OK;
+}
+
+void engine_free(oe_engine* e) {
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (auto& pr : e->pending) {
+        (void)hipEventDestroy(pr.second.first);
+        (void)hipEventDestroy(pr.second.second);
+    }
+    for (auto ev : e->event_pool) (void)hipEventDestroy(ev);
+    if (e->F) (void)hipFree(e->F);
+    if (e->Cf) (void)hipFree(e->Cf);
+    if (e->partials) (void)hipFree(e->partials);
+    if (e->energy_dev) (void)hipFree(e->energy_dev);
+    if (e->energy_host) (void)hipHostFree(e->energy_host);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+const char* oe_last_error(void) { return g_err.c_str(); }
+const char* oe_version(void) { return "openems_amd 0.1 (gfx950)"; }
+
+oe_status oe_operator_create(size_t nx, size_t ny, size_t nz, const double delta[3], oe_operator** out) {
+    if (!out || !delta) return fail(OE_ERR_CONFIG, "null argument");
+    if (nx == 0 || ny == 0 || nz == 0) return fail(OE_ERR_GRID, "empty grid");
+    for (int d = 0; d < 3; ++d)
+        if (!(delta[d] > 0.0) || !std::isfinite(delta[d])) return fail(OE_ERR_GRID, "cell size must be positive");
+    oe_operator* op = new oe_operator();
+    op->nx = nx; op->ny = ny; op->nz = nz;
+    for (int d = 0; d < 3; ++d) op->d[d] = delta[d];
+    // Operator::calculate_timestep (src/fdtd/operator.rs:82-89)
+    const double inv = 1.0 / (delta[0] * delta[0]) + 1.0 / (delta[1] * delta[1]) + 1.0 / (delta[2] * delta[2]);
+    op->dt = 0.99 * (1.0 / (C0 * std::sqrt(inv)));
+    *out = op;
+    return OE_OK;
+}
+
+oe_status oe_operator_create_lines(const double* x, size_t nxl, const double* y, size_t nyl, const double* z,
+                                   size_t nzl, oe_operator** out) {
+    if (!x || !y || !z || nxl < 2 || nyl < 2 || nzl < 2) return fail(OE_ERR_GRID, "need at least 2 mesh lines per axis");
+    const double* L[3] = {x, y, z};
+    size_t n[3] = {nxl, nyl, nzl};
+    double d[3];
+    for (int a = 0; a < 3; ++a) {  // Grid::min_delta (src/geometry/mod.rs:113-118)
+        double m = INFINITY;
+        for (size_t i = 0; i + 1 < n[a]; ++i) m = std::min(m, std::fabs(L[a][i + 1] - L[a][i]));
+        d[a] = m;
+    }
+    return oe_operator_create(nxl - 1, nyl - 1, nzl - 1, d, out);
+}
+
+void oe_operator_destroy(oe_operator* op) { delete op; }
+double oe_operator_timestep(const oe_operator* op) { return op ? op->dt : 0.0; }
+void oe_operator_dims(const oe_operator* op, size_t dims[3]) {
+    if (!op || !dims) return;
+    dims[0] = op->nx; dims[1] = op->ny; dims[2] = op->nz;
+}
+
+oe_status oe_operator_set_material(oe_operator* op, double eps_r, double mu_r, double sigma_e, double sigma_m,
+                                   const size_t region[6]) {
+    if (!op || !region) return fail(OE_ERR_CONFIG, "null argument");
+    OE_TRY(check_region(op, region));
+    if (!(eps_r > 0) || !(mu_r > 0)) return fail(OE_ERR_MATERIAL, "eps_r and mu_r must be positive");
+    MatOp m{};
+    m.pec = 0; m.eps_r = eps_r; m.mu_r = mu_r; m.se = sigma_e; m.sm = sigma_m;
+    std::memcpy(m.box, region, sizeof(m.box));
+    op->ops.push_back(m);
+    return OE_OK;
+}
+
+oe_status oe_operator_set_pec(oe_operator* op, const size_t region[6]) {
+    if (!op || !region) return fail(OE_ERR_CONFIG, "null argument");
+    OE_TRY(check_region(op, region));
+    MatOp m{};
+    m.pec = 1;
+    std::memcpy(m.box, region, sizeof(m.box));
+    op->ops.push_back(m);
+    return OE_OK;
+}
+
+oe_status oe_engine_create(const oe_operator* op, int device, oe_engine** out) {
+    if (!op || !out) return fail(OE_ERR_CONFIG, "null argument");
+    oe_engine* e = new oe_engine();
+    oe_status st = engine_alloc(e, op->nx, op->ny, op->nz, device);
+    if (st == OE_OK) {
+        for (int d = 0; d < 3; ++d) e->d[d] = op->d[d];
+        e->dt = op->dt;
+        st = materialise(e, op);
+    }
+    if (st == OE_OK && hipStreamSynchronize(e->stream) != hipSuccess) st = fail(OE_ERR_DEVICE, "engine setup failed");
+    if (st != OE_OK) { engine_free(e); return st; }
+    *out = e;
+    return OE_OK;
+}
+
+oe_status oe_engine_create_from_arrays(size_t nx, size_t ny, size_t nz, const double delta[3], double dt,
+                                       const float* const coeff[12], int device, oe_engine** out) {
+    if (!coeff || !out || !delta) return fail(OE_ERR_CONFIG, "null argument");
+    for (int w = 0; w < 12; ++w) if (!coeff[w]) return fail(OE_ERR_CONFIG, "coefficient array %d is null", w);
+    oe_engine* e = new oe_engine();
+    oe_status st = engine_alloc(e, nx, ny, nz, device);
+    if (st == OE_OK) {
+        for (int d = 0; d < 3; ++d) e->d[d] = delta[d];
+        e->dt = dt;
+        for (int w = 0; w < 12 && st == OE_OK; ++w)
+            if (hipMemcpy(e->Cf + (size_t)w * e->g.N, coeff[w], e->g.N * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+                st = fail(OE_ERR_DEVICE, "coefficient upload failed");
+    }
+    if (st != OE_OK) { engine_free(e); return st; }
+    *out = e;
+    return OE_OK;
+}
+
+void oe_engine_destroy(oe_engine* eng) { engine_free(eng); }
+
+uint64_t oe_engine_timestep(const oe_engine* eng) { return eng ? eng->timestep : 0; }
+
+oe_status oe_engine_read_fields(oe_engine* e, float* const fields[6]) {
+    if (!e || !fields) return fail(OE_ERR_CONFIG, "null argument");
+    HIP_OK(hipSetDevice(e->device));
+    for (int d = 0; d < 6; ++d)
+        if (fields[d])
+            HIP_OK(hipMemcpyAsync(fields[d], e->F + (size_t)d * e->g.N, e->g.N * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));
+    return OE_OK;
+}
+
+oe_status oe_engine_write_fields(oe_engine* e, const float* const fields[6]) {
+    if (!e || !fields) return fail(OE_ERR_CONFIG, "null argument");
+    HIP_OK(hipSetDevice(e->device));
+    for (int d = 0; d < 6; ++d)
+        if (fields[d])
+            HIP_OK(hipMemcpyAsync(e->F + (size_t)d * e->g.N, fields[d], e->g.N * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));
+    return OE_OK;
+}
+
+oe_status oe_engine_read_coefficients(oe_engine* e, int which, float* out) {
+    if (!e || !out) return fail(OE_ERR_CONFIG, "null argument");
+    if (which < 0 || which >= 12) return fail(OE_ERR_CONFIG, "coefficient index %d out of range", which);
+    HIP_OK(hipSetDevice(e->device));
+    HIP_OK(hipMemcpyAsync(out, e->Cf + (size_t)which * e->g.N, e->g.N * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));
+    return OE_OK;
+}
+
+oe_status oe_engine_reset(oe_engine* e) {
+    if (!e) return fail(OE_ERR_CONFIG, "null argument");
+    HIP_OK(hipSetDevice(e->device));
+    HIP_OK(hipMemsetAsync(e->F, 0, 6 * e->g.N * sizeof(float), e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));
+    e->timestep = 0;
+    e->peak_persistent = 0.0;
+    return OE_OK;
+}
+
+oe_status oe_engine_energy(oe_engine* e, double* ee, double* he) {
+    if (!e) return fail(OE_ERR_CONFIG, "null argument");
+    HIP_OK(hipSetDevice(e->device));
+    HIP_OK(launch_energy(e->F, e->g.N, e->partials, energy_blocks(), e->energy_dev, e->stream));
+    HIP_OK(hipMemcpyAsync(e->energy_host, e->energy_dev, 2 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));
+    if (ee) *ee = e->energy_host[0];
+    if (he) *he = e->energy_host[1];
+    return OE_OK;
+}
+
+oe_status oe_engine_set_profiling(oe_engine* e, int on) {
+    if (!e) return fail(OE_ERR_CONFIG, "null argument");
+    e->profiling = on ? 1 : 0;
+    return OE_OK;
+}
+
+oe_status oe_engine_kernel_time(oe_engine* e, const char* name, uint64_t* launches, double* total_ms) {
+    if (!e || !name) return fail(OE_ERR_CONFIG, "null argument");
+    auto it = e->stats.find(name);
+    if (launches) *launches = it == e->stats.end() ? 0 : it->second.launches;
+    if (total_ms) *total_ms = it == e->stats.end() ? 0.0 : it->second.ms;
+    return OE_OK;
+}
+
+oe_status oe_engine_set_tiles(oe_engine* e, int bx, int by, int ci, int vec) {
+    if (!e) return fail(OE_ERR_CONFIG, "null argument");
+    Tiles t = auto_tiles(e->g, vec);
+    if (vec == 4 && e->g.nz % 4 != 0) return fail(OE_ERR_CONFIG, "vec=4 needs nz %% 4 == 0 (nz=%d)", e->g.nz);
+    if (bx > 0) {
+        if ((bx & (bx - 1)) != 0 || bx > 256) return fail(OE_ERR_CONFIG, "bx must be a power of two <= 256");
+        t.bx = bx;
+        t.by = 256 / bx;
+    }
+    if (by > 0) {
+        if (by * t.bx != 256) return fail(OE_ERR_CONFIG, "bx * by must equal 256");
+        t.by = by;
+    }
+    if (ci > 0) t.ci = std::min(ci, e->g.nx);
+    e->tiles = t;
+    return OE_OK;
+}
+
+oe_status oe_engine_get_tiles(const oe_engine* e, int tiles[4]) {
+    if (!e || !tiles) return fail(OE_ERR_CONFIG, "null argument");
+    tiles[0] = e->tiles.bx; tiles[1] = e->tiles.by; tiles[2] = e->tiles.ci; tiles[3] = e->tiles.vec;
+    return OE_OK;
+}
+
+// ------------------------------------------------------------- waveforms
+oe_status oe_waveform_evaluate(const oe_waveform* w, double t, double* value) {
+    if (!w || !value) return fail(OE_ERR_CONFIG, "null argument");
+    double v = 0.0;
+    switch (w->type) {  // Excitation::evaluate (src/fdtd/excitation.rs:150-200)
+        case OE_EXC_GAUSSIAN: {
+            const double arg = (t - w->t0) / w->tau;
+            v = std::exp(-arg * arg);
+            break;
+        }
+        case OE_EXC_SINUSOIDAL: v = std::sin(2.0 * M_PI * w->frequency * t); break;
+        case OE_EXC_GAUSSIAN_MODULATED: {
+            const double arg = (t - w->t0) / w->tau;
+            v = std::exp(-arg * arg) * std::sin(2.0 * M_PI * w->frequency * t);
+            break;
+        }
+        case OE_EXC_DIRAC: v = std::fabs(t) < 1e-15 ? 1.0 : 0.0; break;
+        case OE_EXC_STEP: v = t >= 0.0 ? 1.0 : 0.0; break;
+        case OE_EXC_CUSTOM: {
+            if (!w->times || !w->values || w->n_points == 0) return fail(OE_ERR_CONFIG, "custom waveform without points");
+            const size_t n = w->n_points;
+            if (t <= w->times[0]) v = w->values[0];
+            else if (t >= w->times[n - 1]) v = w->values[n - 1];
+            else {
+                size_t i = 0;
+                while (i < n - 1 && w->times[i + 1] < t) ++i;
+                const double a = (t - w->times[i]) / (w->times[i + 1] - w->times[i]);
+                v = w->values[i] + a * (w->values[i + 1] - w->values[i]);
+            }
+            break;
+        }
+        default: return fail(OE_ERR_CONFIG, "unknown waveform type %d", w->type);
+    }
+    *value = w->amplitude * v;
+    return OE_OK;
+}
+
+oe_status oe_waveform_sample(const oe_waveform* w, uint64_t start, uint64_t n, double dt, float* out) {
+    if (!w || (n && !out)) return fail(OE_ERR_CONFIG, "null argument");
+    for (uint64_t s = 0; s < n; ++s) {
+        double v;
+        OE_TRY(oe_waveform_evaluate(w, (double)(start + s) * dt, &v));
+        out[s] = (float)v;
+    }
+    return OE_OK;
+}
+
+// ------------------------------------------------------------ extensions
+oe_status oe_ext_cpml_create(oe_engine* e, const oe_cpml_config* cfg, oe_extension** out) {
+    if (!e || !cfg || !out) return fail(OE_ERR_CONFIG, "null argument");
+    if (e->cpml_owner) return fail(OE_ERR_CONFIG, "engine already has a CPML extension");
+    if (cfg->mode != OE_MODE_COMPAT && cfg->mode != OE_MODE_PHYSICAL) return fail(OE_ERR_CONFIG, "bad CPML mode");
+    if (!(cfg->reflection_coeff > 0.0 && cfg->reflection_coeff < 1.0)) return fail(OE_ERR_CONFIG, "reflection_coeff must be in (0,1)");
+    HIP_OK(hipSetDevice(e->device));
+    std::unique_ptr<oe_extension> x(new oe_extension());
+    x->kind = EXT_CPML;
+    x->eng = e;
+    const int n[3] = {e->g.nx, e->g.ny, e->g.nz};
+    const double dmin = std::min(std::min(e->d[0], e->d[1]), e->d[2]);
+    size_t lmax = 0;
+    for (int f = 0; f < 6; ++f) lmax = std::max(lmax, cfg->layers[f]);
+    // CpmlBoundaries::sigma_max (src/extensions/cpml.rs:68-72)
+    const double dd = (double)(lmax ? lmax : 1) * dmin;
+    const double m = cfg->grading_order;
+    const double sigma_max = -(m + 1.0) * std::log(cfg->reflection_coeff) / (2.0 * Z0 * dd);
+    CpmlArgs& c = x->cpml;
+    c.physical = cfg->mode == OE_MODE_PHYSICAL;
+    c.Lmax = (int)std::max<size_t>(lmax, 1);
+    std::vector<float> tab((size_t)6 * CPML_TABLES * c.Lmax, 0.0f);
+    size_t total = 0;
+    size_t offs[6];
+    for (int f = 0; f < 6; ++f) {
+        const int dir = f / 2;
+        const bool is_min = (f % 2) == 0;
+        const size_t L = cfg->layers[f];
+        offs[f] = total;
+        if (!(L > 0 && L < (size_t)n[dir])) continue;  // Cpml::new (src/extensions/cpml.rs:262,283)
+        c.on_mask |= 1 << f;
+        c.L[f] = (int)L;
+        c.g0[f] = is_min ? 0 : n[dir] - (int)L;
+        const double delta = e->d[dir];
+        const double width = (double)L * delta;
+        for (size_t i = 0; i < L; ++i) {  // CpmlCoefficients::init (src/extensions/cpml.rs:98-177)
+            const double de = is_min ? (double)(L - i) * delta : (double)(i + 1) * delta;
+            const double dh = de - 0.5 * delta;
+            const double re = std::min(std::max(de / width, 0.0), 1.0);
+            const double rh = std::min(std::max(dh / width, 0.0), 1.0);
+            const double se = sigma_max * std::pow(re, m), sh = sigma_max * std::pow(rh, m);
+            const double ae_ = cfg->alpha_max * (1.0 - re), ah_ = cfg->alpha_max * (1.0 - rh);
+            const double ke = 1.0 + (cfg->kappa_max - 1.0) * std::pow(re, m);
+            const double kh = 1.0 + (cfg->kappa_max - 1.0) * std::pow(rh, m);
+            const double be = std::exp(-(se / ke / EPS0 + ae_ / EPS0) * e->dt);
+            const double bh = std::exp(-(sh / kh / EPS0 + ah_ / EPS0) * e->dt);
+            const double sne = se / (EPS0 * ke), snh = sh / (EPS0 * kh);
+            const double ane = ae_ / EPS0, anh = ah_ / EPS0;
+            double a_e = std::fabs(sne + ane) > 1e-20 ? sne / (sne + ane) * (1.0 - be) : 0.0;
+            double a_h = std::fabs(snh + anh) > 1e-20 ? snh / (snh + anh) * (1.0 - bh) : 0.0;
+            if (c.physical) { a_e /= ke; a_h /= kh; }
+            float* T = tab.data() + (size_t)f * CPML_TABLES * c.Lmax;
+            T[TAB_BE * c.Lmax + i] = (float)be;
+            T[TAB_BH * c.Lmax + i] = (float)bh;
+            T[TAB_AE * c.Lmax + i] = (float)a_e;
+            T[TAB_AH * c.Lmax + i] = (float)a_h;
+            T[TAB_KE * c.Lmax + i] = (float)(1.0 / ke - 1.0);
+            T[TAB_KH * c.Lmax + i] = (float)(1.0 / kh - 1.0);
+        }
+        size_t cnt = (size_t)L;
+        for (int a = 0; a < 3; ++a) if (a != dir) cnt *= (size_t)n[a];
+        cnt = (cnt + 63) & ~(size_t)63;  // 256-byte aligned sub-arrays
+        x->psi_count[f] = cnt;
+        total += 4 * cnt;
+    }
+    HIP_OK(x->cpml_mem.alloc(std::max<size_t>(total, 64) * sizeof(float)));
+    HIP_OK(hipMemsetAsync(x->cpml_mem.p, 0, x->cpml_mem.bytes, e->stream));
+    for (int f = 0; f < 6; ++f) {
+        if (!(c.on_mask & (1 << f))) continue;
+        float* base = x->cpml_mem.as<float>() + offs[f];
+        const size_t cnt = x->psi_count[f];
+        c.pe_pos[f] = base;
+        c.pe_neg[f] = base + cnt;
+        c.ph_pos[f] = base + 2 * cnt;
+        c.ph_neg[f] = base + 3 * cnt;
+    }
+    HIP_OK(x->cpml_tab.alloc(tab.size() * sizeof(float)));
+    HIP_OK(hipMemcpyAsync(x->cpml_tab.p, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));
+    c.tab = x->cpml_tab.as<float>();
+    e->cpml_owner = x.get();
+    *out = x.release();
+    return OE_OK;
+}
+
+oe_status oe_ext_mur_create(oe_engine* e, const int enabled[6], double phase_velocity, oe_extension** out) {
+    if (!e || !enabled || !out) return fail(OE_ERR_CONFIG, "null argument");
+    HIP_OK(hipSetDevice(e->device));
+    std::unique_ptr<oe_extension> x(new oe_extension());
+    x->kind = EXT_MUR;
+    x->eng = e;
+    const int n[3] = {e->g.nx, e->g.ny, e->g.nz};
+    size_t total = 0;
+    for (int f = 0; f < 6; ++f) {  // MurAbc::build_faces (src/extensions/mur_abc.rs:186-200)
+        if (!enabled[f]) continue;
+        MurDev m{};
+        m.ny = f / 2;
+        const bool is_max = (f % 2) == 1;
+        m.nyp = (m.ny + 1) % 3;
+        m.nypp = (m.ny + 2) % 3;
+        if (n[m.ny] < 2) return fail(OE_ERR_GRID, "Mur face needs at least 2 cells along its normal");
+        if (is_max) { m.line = n[m.ny] - 1; m.shift = n[m.ny] - 2; }
+        else { m.line = 0; m.shift = 1; }
+        m.s0 = n[m.nyp];
+        m.s1 = n[m.nypp];
+        const double c = phase_velocity > 0 ? phase_velocity : C0;
+        const double cdt = c * e->dt;
+        m.coeff = (float)((cdt - e->d[m.ny]) / (cdt + e->d[m.ny]));  // src/extensions/mur_abc.rs:138-142
+        x->mur_faces.push_back(m);
+        total += 2 * (((size_t)m.s0 * m.s1 + 63) & ~(size_t)63);
+    }
+    HIP_OK(x->mur_mem.alloc(std::max<size_t>(total, 64) * sizeof(float)));
+    HIP_OK(hipMemsetAsync(x->mur_mem.p, 0, x->mur_mem.bytes, e->stream));
+    size_t off = 0;
+    for (auto& m : x->mur_faces) {
+        const size_t cnt = ((size_t)m.s0 * m.s1 + 63) & ~(size_t)63;
+        m.v_nyp = x->mur_mem.as<float>() + off;
+        m.v_nypp = x->mur_mem.as<float>() + off + cnt;
+        off += 2 * cnt;
+    }
+    HIP_OK(x->mur_dev.alloc(std::max<size_t>(x->mur_faces.size(), 1) * sizeof(MurDev)));
+    if (!x->mur_faces.empty())
+        HIP_OK(hipMemcpyAsync(x->mur_dev.p, x->mur_faces.data(), x->mur_faces.size() * sizeof(MurDev), hipMemcpyHostToDevice, e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));
+    *out = x.release();
+    return OE_OK;
+}
+
+oe_status oe_ext_ade_create(oe_engine* e, int kind, const double params[3], const uint32_t* positions, size_t n,
+                            int mode, oe_extension** out) {
+    if (!e || !params || !out || (n && !positions)) return fail(OE_ERR_CONFIG, "null argument");
+    if (kind < 0 || kind > 2) return fail(OE_ERR_CONFIG, "unknown ADE kind %d", kind);
+    if (mode != OE_MODE_COMPAT && mode != OE_MODE_PHYSICAL) return fail(OE_ERR_CONFIG, "bad ADE mode");
+    if (n > (size_t)INT32_MAX) return fail(OE_ERR_CONFIG, "too many ADE cells");
+    HIP_OK(hipSetDevice(e->device));
+    std::unique_ptr<oe_extension> x(new oe_extension());
+    x->kind = EXT_ADE;
+    x->eng = e;
+    std::vector<uint32_t> idx(n);
+    for (size_t c = 0; c < n; ++c) {
+        const uint32_t i = positions[3 * c], j = positions[3 * c + 1], k = positions[3 * c + 2];
+        if (i >= (uint32_t)e->g.nx || j >= (uint32_t)e->g.ny || k >= (uint32_t)e->g.nz)
+            return fail(OE_ERR_CONFIG, "ADE position %zu out of the grid", c);
+        idx[c] = (uint32_t)(((size_t)i * e->g.ny + j) * e->g.nz + k);
+    }
+    AdeDev& a = x->ade;
+    a.kind = kind; a.mode = mode; a.n = (int)n;
+    const double dt = e->dt;
+    if (kind == OE_ADE_LORENTZ) {  // src/extensions/dispersive.rs:134-159
+        const double w0 = params[0], g = params[1], de = params[2];
+        const double den = 1.0 + g * dt / 2.0;
+        a.a = (float)((2.0 - w0 * w0 * dt * dt) / den);
+        a.b = (float)((g * dt / 2.0 - 1.0) / den);
+        a.c = (float)(EPS0 * de * w0 * w0 * dt * dt / den);
+        a.pa = a.a; a.pb = a.b; a.pc = a.c;
+    } else if (kind == OE_ADE_DRUDE) {  // :476-498
+        const double wp = params[0], g = params[1];
+        const double den = 1.0 + g * dt / 2.0;
+        a.a = (float)((1.0 - g * dt / 2.0) / den);
+        a.b = (float)(EPS0 * wp * wp * dt / den);
+        a.c = (float)(dt / EPS0);
+        a.pa = (float)((1.0 - g * dt / 2.0) / den);
+        a.pb = (float)((EPS0 * wp * wp * dt / 2.0) / den);
+    } else {  // Debye :570-592
+        const double tau = params[0], de = params[1];
+        const double den = 2.0 * tau + dt;
+        a.a = (float)((2.0 * tau - dt) / den);
+        a.b = (float)(2.0 * EPS0 * de * dt / den);
+        a.c = (float)(dt / EPS0);
+        a.pa = (float)((2.0 * tau - dt) / den);
+        a.pb = (float)(EPS0 * de * dt / den);
+    }
+    a.dt_t = (float)dt;
+    a.dtf = (float)dt;
+    for (int d = 0; d < 3; ++d) a.delta[d] = e->d[d];
+    const size_t cnt = (std::max<size_t>(n, 1) + 63) & ~(size_t)63;
+    HIP_OK(x->ade_mem.alloc(cnt * sizeof(uint32_t) + 9 * cnt * sizeof(float)));
+    HIP_OK(hipMemsetAsync(x->ade_mem.p, 0, x->ade_mem.bytes, e->stream));
+    uint32_t* di = x->ade_mem.as<uint32_t>();
+    float* fb = reinterpret_cast<float*>(di + cnt);
+    a.idx = di;
+    a.J = fb;
+    a.Jold = fb + 3 * cnt;
+    a.Eprev = fb + 6 * cnt;
+    if (n) HIP_OK(hipMemcpyAsync(di, idx.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));
+    *out = x.release();
+    return OE_OK;
+}
+
+oe_status oe_ext_field_probe_create(oe_engine* e, uint32_t i, uint32_t j, uint32_t k, oe_extension** out) {
+    if (!e || !out) return fail(OE_ERR_CONFIG, "null argument");
+    if (i >= (uint32_t)e->g.nx || j >= (uint32_t)e->g.ny || k >= (uint32_t)e->g.nz)
+        return fail(OE_ERR_CONFIG, "probe position out of the grid");
+    HIP_OK(hipSetDevice(e->device));
+    std::unique_ptr<oe_extension> x(new oe_extension());
+    x->kind = EXT_PROBE;
+    x->eng = e;
+    x->probe_idx = (uint32_t)(((size_t)i * e->g.ny + j) * e->g.nz + k);
+    x->probe_cap = 0;
+    HIP_OK(x->probe_cnt.alloc(sizeof(uint64_t)));
+    HIP_OK(hipMemsetAsync(x->probe_cnt.p, 0, sizeof(uint64_t), e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));
+    *out = x.release();
+    return OE_OK;
+}
+
+static oe_status probe_reserve(oe_extension* x, uint64_t need) {
+    if (need <= x->probe_cap) return OE_OK;
+    oe_engine* e = x->eng;
+    uint64_t cap = std::max<uint64_t>(need, std::max<uint64_t>(1024, 2 * x->probe_cap));
+    DevBuf nb;
+    HIP_OK(nb.alloc(cap * 6 * sizeof(double)));
+    uint64_t cnt = 0;
+    HIP_OK(hipMemcpyAsync(&cnt, x->probe_cnt.p, sizeof(uint64_t), hipMemcpyDeviceToHost, e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));
+    if (cnt && x->probe_mem.p)
+        HIP_OK(hipMemcpyAsync(nb.p, x->probe_mem.p, std::min(cnt, x->probe_cap) * 6 * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));
+    std::swap(x->probe_mem.p, nb.p);
+    std::swap(x->probe_mem.bytes, nb.bytes);
+    x->probe_cap = cap;
+    return OE_OK;
+}
+
+oe_status oe_ext_probe_read(oe_extension* x, double* out, size_t capacity, size_t* n_samples) {
+    if (!x || x->kind != EXT_PROBE) return fail(OE_ERR_CONFIG, "not a probe extension");
+    oe_engine* e = x->eng;
+    HIP_OK(hipSetDevice(e->device));
+    uint64_t cnt = 0;
+    HIP_OK(hipMemcpyAsync(&cnt, x->probe_cnt.p, sizeof(uint64_t), hipMemcpyDeviceToHost, e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));
+    cnt = std::min(cnt, x->probe_cap);
+    if (n_samples) *n_samples = cnt;
+    if (out && capacity) {
+        const size_t m = std::min<size_t>(capacity, cnt);
+        if (m) HIP_OK(hipMemcpy(out, x->probe_mem.p, m * 6 * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return OE_OK;
+}
+
+oe_status oe_ext_cpml_read_psi(oe_extension* x, int face, int field, int which, float* out, size_t capacity, size_t* n) {
+    if (!x || x->kind != EXT_CPML) return fail(OE_ERR_CONFIG, "not a CPML extension");
+    if (face < 0 || face > 5 || field < 0 || field > 1 || which < 0 || which > 1) return fail(OE_ERR_CONFIG, "bad psi selector");
+    const CpmlArgs& c = x->cpml;
+    if (!(c.on_mask & (1 << face))) { if (n) *n = 0; return OE_OK; }
+    oe_engine* e = x->eng;
+    size_t cnt = (size_t)c.L[face];
+    const int dims[3] = {e->g.nx, e->g.ny, e->g.nz};
+    for (int a = 0; a < 3; ++a) if (a != face / 2) cnt *= (size_t)dims[a];
+    if (n) *n = cnt;
+    const float* src = field == 0 ? (which == 0 ? c.pe_pos[face] : c.pe_neg[face]) : (which == 0 ? c.ph_pos[face] : c.ph_neg[face]);
+    if (out && capacity) {
+        HIP_OK(hipSetDevice(e->device));
+        HIP_OK(hipMemcpyAsync(out, src, std::min(cnt, capacity) * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+        HIP_OK(hipStreamSynchronize(e->stream));
+    }
+    return OE_OK;
+}
+
+oe_status oe_ext_ade_read_state(oe_extension* x, float* out, size_t capacity, size_t* n) {
+    if (!x || x->kind != EXT_ADE) return fail(OE_ERR_CONFIG, "not an ADE extension");
+    const size_t cnt = (size_t)x->ade.n;
+    if (n) *n = 3 * cnt;
+    if (out && capacity) {
+        oe_engine* e = x->eng;
+        HIP_OK(hipSetDevice(e->device));
+        for (int d = 0; d < 3; ++d) {
+            const size_t m = std::min(cnt, capacity > d * cnt ? capacity - d * cnt : 0);
+            if (m) HIP_OK(hipMemcpyAsync(out + d * cnt, x->ade.J + (size_t)d * cnt, m * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+        }
+        HIP_OK(hipStreamSynchronize(e->stream));
+    }
+    return OE_OK;
+}
+
+oe_status oe_extension_reset(oe_extension* x) {
+    if (!x) return fail(OE_ERR_CONFIG, "null argument");
+    oe_engine* e = x->eng;
+    HIP_OK(hipSetDevice(e->device));
+    if (x->kind == EXT_CPML) HIP_OK(hipMemsetAsync(x->cpml_mem.p, 0, x->cpml_mem.bytes, e->stream));
+    if (x->kind == EXT_MUR) HIP_OK(hipMemsetAsync(x->mur_mem.p, 0, x->mur_mem.bytes, e->stream));
+    if (x->kind == EXT_ADE) {
+        const size_t cnt = (std::max<size_t>(x->ade.n, 1) + 63) & ~(size_t)63;
+        HIP_OK(hipMemsetAsync(x->ade.J, 0, 9 * cnt * sizeof(float), e->stream));
+    }
+    if (x->kind == EXT_PROBE) HIP_OK(hipMemsetAsync(x->probe_cnt.p, 0, sizeof(uint64_t), e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));
+    return OE_OK;
+}
+
+void oe_extension_destroy(oe_extension* x) {
+    if (!x) return;
+    if (x->eng) {
+        (void)hipSetDevice(x->eng->device);
+        (void)hipStreamSynchronize(x->eng->stream);
+        if (x->eng->cpml_owner == x) x->eng->cpml_owner = nullptr;
+    }
+    delete x;
+}
+
+// ------------------------------------------------------------- run_batch
+oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_result* r) {
+    if (!e || !b || !r) return fail(OE_ERR_CONFIG, "null argument");
+    if (b->n_excitations && !b->excitations) return fail(OE_ERR_CONFIG, "excitations is null");
+    if (b->n_extensions && !b->extensions) return fail(OE_ERR_CONFIG, "extensions is null");
+    HIP_OK(hipSetDevice(e->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    std::memset(r, 0, sizeof(*r));
+
+    const uint64_t num_steps = b->has_num_steps ? b->num_steps : (b->has_max_timesteps ? b->max_timesteps : UINT64_MAX);
+    const uint64_t check_interval = b->check_interval ? b->check_interval : 100;
+
+    // Sort extensions by hook.
+    const CpmlArgs* cp = nullptr;
+    std::vector<oe_extension*> murs, ades, probes;
+    for (size_t q = 0; q < b->n_extensions; ++q) {
+        oe_extension* x = b->extensions[q];
+        if (!x) return fail(OE_ERR_CONFIG, "extension %zu is null", q);
+        if (x->eng != e) return fail(OE_ERR_CONFIG, "extension %zu belongs to another engine", q);
+        switch (x->kind) {
+            case EXT_CPML:
+                if (cp) return fail(OE_ERR_CONFIG, "more than one CPML extension in a batch");
+                cp = &x->cpml;
+                break;
+            case EXT_MUR: murs.push_back(x); break;
+            case EXT_ADE: ades.push_back(x); break;
+            case EXT_PROBE: probes.push_back(x); break;
+            default: return fail(OE_ERR_CONFIG, "unknown extension kind");
+        }
+    }
+
+    // Excitation tables (ScheduledExcitation, src/fdtd/batch.rs:71-133).
+    std::vector<PointExc> px(b->n_excitations);
+    size_t total_samples = 0;
+    for (size_t q = 0; q < b->n_excitations; ++q) {
+        const oe_excitation& x = b->excitations[q];
+        if (x.i >= (uint32_t)e->g.nx || x.j >= (uint32_t)e->g.ny || x.k >= (uint32_t)e->g.nz || x.direction > 2)
+            return fail(OE_ERR_CONFIG, "excitation %zu out of the grid", q);
+        if (x.n_samples && !x.samples) return fail(OE_ERR_CONFIG, "excitation %zu has no samples", q);
+        total_samples += x.n_samples;
+    }
+    if (total_samples * sizeof(float) > e->exc_samples.bytes) HIP_OK(e->exc_samples.alloc(total_samples * sizeof(float) + 256));
+    size_t off = 0;
+    for (size_t q = 0; q < b->n_excitations; ++q) {
+        const oe_excitation& x = b->excitations[q];
+        px[q].idx = (uint32_t)(((size_t)x.i * e->g.ny + x.j) * e->g.nz + x.k);
+        px[q].comp = x.direction;
+        px[q].soft = x.soft ? 1u : 0u;
+        px[q].samples = e->exc_samples.as<float>() + off;
+        px[q].n_samples = x.n_samples;
+        if (x.n_samples)
+            HIP_OK(hipMemcpyAsync(e->exc_samples.as<float>() + off, x.samples, x.n_samples * sizeof(float), hipMemcpyHostToDevice, e->stream));
+        off += x.n_samples;
+    }
+    if (px.size() * sizeof(PointExc) > e->exc_table.bytes) HIP_OK(e->exc_table.alloc(px.size() * sizeof(PointExc) + 64));
+    if (!px.empty())
+        HIP_OK(hipMemcpyAsync(e->exc_table.p, px.data(), px.size() * sizeof(PointExc), hipMemcpyHostToDevice, e->stream));
+
+    // Probe tables.
+    std::vector<PointProbe> pp(probes.size());
+    const uint64_t reserve_steps = num_steps == UINT64_MAX ? 0 : num_steps;
+    for (size_t q = 0; q < probes.size(); ++q) {
+        oe_extension* x = probes[q];
+        uint64_t cnt = 0;
+        HIP_OK(hipMemcpyAsync(&cnt, x->probe_cnt.p, sizeof(uint64_t), hipMemcpyDeviceToHost, e->stream));
+        HIP_OK(hipStreamSynchronize(e->stream));
+        OE_TRY(probe_reserve(x, cnt + std::max<uint64_t>(reserve_steps, 1)));
+        pp[q].idx = x->probe_idx;
+        pp[q].out = x->probe_mem.as<double>();
+        pp[q].capacity = x->probe_cap;
+        pp[q].count = x->probe_cnt.as<uint64_t>();
+    }
+    if (pp.size() * sizeof(PointProbe) > e->probe_table.bytes) HIP_OK(e->probe_table.alloc(pp.size() * sizeof(PointProbe) + 64));
+    if (!pp.empty())
+        HIP_OK(hipMemcpyAsync(e->probe_table.p, pp.data(), pp.size() * sizeof(PointProbe), hipMemcpyHostToDevice, e->stream));
+    const bool point_ops = !px.empty() || !pp.empty();
+
+    const YeeArgs ya = yee_args(e);
+    double peak = b->persistent_peak ? e->peak_persistent : 0.0;
+    std::vector<uint64_t> sample_steps;  // timesteps whose energy is pending in energy_dev
+    size_t n_samples_total = 0;
+    double last_total = 0.0;
+    const bool need_decay = b->has_energy_decay_db;
+
+    auto drain = [&](void) -> oe_status {
+        if (sample_steps.empty()) return OE_OK;
+        HIP_OK(hipMemcpyAsync(e->energy_host, e->energy_dev, 2 * sample_steps.size() * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        HIP_OK(hipStreamSynchronize(e->stream));
+        for (size_t q = 0; q < sample_steps.size(); ++q) {
+            const double ee = e->energy_host[2 * q], he = e->energy_host[2 * q + 1];
+            if (b->track_peak) peak = std::max(peak, ee + he);
+            last_total = ee + he;
+            if (b->energy_samples && n_samples_total < b->energy_samples_capacity) {
+                oe_energy_sample& s = b->energy_samples[n_samples_total];
+                s.timestep = sample_steps[q];
+                s.e_energy = ee;
+                s.h_energy = he;
+                s.total_energy = ee + he;
+            }
+            ++n_samples_total;
+        }
+        sample_steps.clear();
+        return OE_OK;
+    };
+
+    uint64_t executed = 0;
+    int reason = OE_TERM_STEPS_COMPLETED;
+    double final_db = 0.0;
+    for (uint64_t st = 0; st < num_steps; ++st) {
+        const uint64_t cur = e->timestep;
+        {
+            Timed tm(e, "update_h");
+            HIP_OK(launch_update_h(ya, cp, e->stream));
+        }
+        for (auto* x : murs) {
+            Timed tm(e, "mur_pre");
+            HIP_OK(launch_mur_pre(e->F, e->g, x->mur_dev.as<MurDev>(), (int)x->mur_faces.size(), e->stream));
+        }
+        for (auto* x : ades)
+            if (x->ade.mode == OE_MODE_PHYSICAL) {
+                Timed tm(e, "ade_pre");
+                HIP_OK(launch_ade_pre(e->F, e->g.N, x->ade, e->stream));
+            }
+        {
+            Timed tm(e, "update_e");
+            HIP_OK(launch_update_e(ya, cp, e->stream));
+        }
+        for (auto* x : ades) {
+            Timed tm(e, "ade_post");
+            HIP_OK(launch_ade_post(e->F, e->Cf, e->g.N, x->ade, e->stream));
+        }
+        for (auto* x : murs) {
+            Timed tm(e, "mur_post");
+            HIP_OK(launch_mur_post(e->F, e->g, x->mur_dev.as<MurDev>(), (int)x->mur_faces.size(), e->stream));
+        }
+        if (point_ops) {
+            Timed tm(e, "point_ops");
+            HIP_OK(launch_point_ops(e->F, e->g.N, e->exc_table.as<PointExc>(), (int)px.size(), st,
+                                    e->probe_table.as<PointProbe>(), (int)pp.size(), e->stream));
+        }
+        e->timestep += 1;
+        ++executed;
+
+        if (b->sample_interval > 0 && cur % b->sample_interval == 0) {
+            if (sample_steps.size() >= e->energy_dev_cap) OE_TRY(drain());
+            Timed tm(e, "energy");
+            HIP_OK(launch_energy(e->F, e->g.N, e->partials, energy_blocks(), e->energy_dev + 2 * sample_steps.size(), e->stream));
+            sample_steps.push_back(cur);
+        }
+        if (cur > 0 && cur % check_interval == 0 && need_decay) {
+            const bool sampled_now = !sample_steps.empty() && sample_steps.back() == cur;
+            OE_TRY(drain());
+            double curE = last_total;
+            if (!sampled_now) {  // compute_energy_sample at the check step
+                double ee, he;
+                OE_TRY(oe_engine_energy(e, &ee, &he));
+                curE = ee + he;
+            }
+            if (peak > 0.0 && curE > 0.0) {  // src/fdtd/basic_engine.rs:128-141
+                const double ddb = 10.0 * std::log10(curE / peak);
+                if (ddb < -std::fabs(b->energy_decay_db)) {
+                    reason = OE_TERM_ENERGY_DECAY;
+                    final_db = ddb;
+                    break;
+                }
+            }
+        }
+        if (e->profiling && e->pending.size() > 4096) {
+            HIP_OK(hipStreamSynchronize(e->stream));
+            collect_timing(e);
+        }
+    }
+    OE_TRY(drain());
+    HIP_OK(hipStreamSynchronize(e->stream));
+    if (e->profiling) collect_timing(e);
+    if (b->persistent_peak) e->peak_persistent = peak;
+
+    r->timesteps_executed = executed;
+    r->termination_reason = reason;
+    r->final_decay_db = final_db;
+    r->n_energy_samples = n_samples_total;
+    r->elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return OE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,295 @@
+// oe_aux.hip -- small gfx950 kernels around the Yee updates: coefficient
+// fills, excitations + probes, the float64 energy reduction, Mur ABC faces
+// and the dispersive (ADE) polarisation-current updates.
+
+#include "oe_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace oe {
+namespace {
+
+__global__ void k_fill(float* __restrict__ p, size_t n, float v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+// Operator::set_material / set_pec region writes (src/fdtd/operator.rs:222-262).
+__global__ void k_box_fill(float* __restrict__ base, int ny, int nz, int i0, int j0, int k0,
+                           int ni, int nj, int nk, float v) {
+    const size_t cnt = (size_t)ni * nj * nk;
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; t < cnt; t += stride) {
+        const int k = (int)(t % nk);
+        const size_t r = t / nk;
+        const int j = (int)(r % nj);
+        const int i = (int)(r / nj);
+        base[((size_t)(i0 + i) * ny + (j0 + j)) * nz + (k0 + k)] = v;
+    }
+}
+
+// Excitations (BasicEngine::apply_excitations, src/fdtd/basic_engine.rs:313-395)
+// run in list order on one lane, as repeated sources on one edge must add up
+// in that order; then each probe lane records its node (FieldProbe::sample,
+// src/processing/probes.rs:289-303).
+__global__ void k_point_ops(float* __restrict__ F, size_t N, const PointExc* __restrict__ exc, int n_exc,
+                            uint64_t step, const PointProbe* __restrict__ probes, int n_probes) {
+    if (threadIdx.x == 0) {
+        for (int e = 0; e < n_exc; ++e) {
+            const PointExc x = exc[e];
+            if (step >= x.n_samples) continue;
+            const float v = x.samples[step];
+            float* f = F + (size_t)x.comp * N + x.idx;
+            if (x.soft) *f = *f + v;
+            else *f = v;
+        }
+    }
+    __syncthreads();
+    for (int pr = threadIdx.x; pr < n_probes; pr += blockDim.x) {
+        const PointProbe q = probes[pr];
+        const uint64_t c = *q.count;
+        if (c < q.capacity) {
+            double* o = q.out + 6 * c;
+#pragma unroll
+            for (int d = 0; d < 6; ++d) o[d] = (double)F[(size_t)d * N + q.idx];
+        }
+        *q.count = c + 1;
+    }
+}
+
+constexpr int EN_THREADS = 256;
+constexpr int EN_BLOCKS = 1024;
+
+// Field3D::energy (src/arrays/field.rs:219-221): sum of v*v in float64.
+__global__ __launch_bounds__(EN_THREADS) void k_energy(const float* __restrict__ F, size_t N,
+                                                       double* __restrict__ partials) {
+    __shared__ double se[EN_THREADS], sh[EN_THREADS];
+    double ae = 0.0, ah = 0.0;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double e = (double)F[(size_t)d * N + i];
+            const double h = (double)F[(size_t)(d + 3) * N + i];
+            ae += e * e;
+            ah += h * h;
+        }
+    }
+    se[threadIdx.x] = ae;
+    sh[threadIdx.x] = ah;
+    __syncthreads();
+    for (int s = EN_THREADS / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            se[threadIdx.x] += se[threadIdx.x + s];
+            sh[threadIdx.x] += sh[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = se[0];
+        partials[2 * blockIdx.x + 1] = sh[0];
+    }
+}
+
+__global__ __launch_bounds__(EN_THREADS) void k_energy_final(const double* __restrict__ partials, int nb,
+                                                             double* __restrict__ out) {
+    __shared__ double se[EN_THREADS], sh[EN_THREADS];
+    double ae = 0.0, ah = 0.0;
+    for (int b = threadIdx.x; b < nb; b += EN_THREADS) {
+        ae += partials[2 * b];
+        ah += partials[2 * b + 1];
+    }
+    se[threadIdx.x] = ae;
+    sh[threadIdx.x] = ah;
+    __syncthreads();
+    for (int s = EN_THREADS / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            se[threadIdx.x] += se[threadIdx.x + s];
+            sh[threadIdx.x] += sh[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = se[0];
+        out[1] = sh[0];
+    }
+}
+
+__device__ __forceinline__ size_t mur_cell(const Grid& g, const MurDev& m, int line, int a, int b) {
+    int pos[3];
+    pos[m.ny] = line;
+    pos[m.nyp] = a;
+    pos[m.nypp] = b;
+    return ((size_t)pos[0] * g.ny + pos[1]) * g.nz + pos[2];
+}
+
+// MurAbc::pre_voltage_update (src/extensions/mur_abc.rs:213-254).
+__global__ void k_mur_pre(const float* __restrict__ F, Grid g, const MurDev* __restrict__ faces) {
+    const MurDev m = faces[blockIdx.y];
+    const int cnt = m.s0 * m.s1;
+    const float* A = F + (size_t)m.nyp * g.N;
+    const float* B = F + (size_t)m.nypp * g.N;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x) {
+        const int a = q / m.s1, b = q % m.s1;
+        const size_t pb = mur_cell(g, m, m.line, a, b), ps = mur_cell(g, m, m.shift, a, b);
+        m.v_nyp[q] = A[ps] - m.coeff * A[pb];
+        m.v_nypp[q] = B[ps] - m.coeff * B[pb];
+    }
+}
+
+// MurAbc::post_voltage_update (src/extensions/mur_abc.rs:257-291).
+__global__ void k_mur_post(const float* __restrict__ F, Grid g, const MurDev* __restrict__ faces) {
+    const MurDev m = faces[blockIdx.y];
+    const int cnt = m.s0 * m.s1;
+    const float* A = F + (size_t)m.nyp * g.N;
+    const float* B = F + (size_t)m.nypp * g.N;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x) {
+        const int a = q / m.s1, b = q % m.s1;
+        const size_t ps = mur_cell(g, m, m.shift, a, b);
+        m.v_nyp[q] += m.coeff * A[ps];
+        m.v_nypp[q] += m.coeff * B[ps];
+    }
+}
+
+// MurAbc::apply_to_voltages (src/extensions/mur_abc.rs:294-324), one face
+// per launch so that faces sharing an edge overwrite in list order.
+__global__ void k_mur_apply(float* __restrict__ F, Grid g, const MurDev* __restrict__ faces, int f) {
+    const MurDev m = faces[f];
+    const int cnt = m.s0 * m.s1;
+    float* A = F + (size_t)m.nyp * g.N;
+    float* B = F + (size_t)m.nypp * g.N;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x) {
+        const int a = q / m.s1, b = q % m.s1;
+        const size_t pb = mur_cell(g, m, m.line, a, b);
+        A[pb] = m.v_nyp[q];
+        B[pb] = m.v_nypp[q];
+    }
+}
+
+__global__ void k_ade_pre(const float* __restrict__ F, size_t N, AdeDev a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n) return;
+    const size_t p = a.idx[c];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) a.Eprev[(size_t)d * a.n + c] = F[(size_t)d * N + p];
+}
+
+// Dispersive post_update_e.  Compat: the reference bodies
+// (src/extensions/dispersive.rs:171-196, 510-535, 604-630).  Physical: the
+// trapezoidal Drude/Debye and second-order Lorentz schemes, with
+// K = Cb * delta = dt / eps_cell / (1 + loss) taken from the cell.
+__global__ void k_ade_post(float* __restrict__ F, const float* __restrict__ Cf, size_t N, AdeDev a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n) return;
+    const size_t p = a.idx[c];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        float* fe = F + (size_t)d * N + p;
+        const float e = *fe;
+        const size_t s = (size_t)d * a.n + c;
+        if (a.mode == 0) {
+            const float jo = a.J[s];
+            if (a.kind == 0) {
+                const float jn = a.a * jo + a.c * e;
+                a.J[s] = jn;
+                *fe = e - a.b * jn;
+            } else if (a.kind == 1) {
+                const float jn = a.a * jo + a.b * e;
+                a.J[s] = jn;
+                *fe = e - a.c * jn;
+            } else {
+                const float pn = a.a * jo + a.b * e;
+                a.J[s] = pn;
+                const float dpdt = (pn - jo) / a.dt_t;
+                *fe = e - a.c * dpdt;
+            }
+        } else {
+            const float K = (float)((double)Cf[(size_t)(3 + d) * N + p] * (double)a.delta[d]);
+            const float en = a.Eprev[s];
+            const float jn = a.J[s];
+            if (a.kind == 1) {
+                const float kj = a.pa, bj = a.pb;
+                const float enew = (e - (K / 2.f) * ((1.f + kj) * jn + bj * en)) / (1.f + K * bj / 2.f);
+                a.J[s] = kj * jn + bj * (enew + en);
+                *fe = enew;
+            } else if (a.kind == 0) {
+                const float pold = a.Jold[s];
+                const float pnew = a.pa * jn + a.pb * pold + a.pc * en;
+                a.Jold[s] = jn;
+                a.J[s] = pnew;
+                *fe = e - K * ((pnew - jn) / a.dtf);
+            } else {
+                const float aa = a.pa, bb = a.pb, dtt = a.dtf;
+                const float enew = (e - (K / dtt) * ((aa - 1.f) * jn + bb * en)) / (1.f + K * bb / dtt);
+                a.J[s] = aa * jn + bb * (enew + en);
+                *fe = enew;
+            }
+        }
+    }
+}
+
+int grid_for(size_t n, int threads, int cap) {
+    size_t b = (n + threads - 1) / threads;
+    if (b < 1) b = 1;
+    if (b > (size_t)cap) b = cap;
+    return (int)b;
+}
+
+}  // namespace
+
+hipError_t launch_fill(float* p, size_t n, float v, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_box_fill(float* base, const Grid& g, const int box[6], float v, hipStream_t s) {
+    const int ni = box[1] - box[0], nj = box[3] - box[2], nk = box[5] - box[4];
+    if (ni <= 0 || nj <= 0 || nk <= 0) return hipSuccess;
+    const size_t cnt = (size_t)ni * nj * nk;
+    hipLaunchKernelGGL(k_box_fill, dim3(grid_for(cnt, 256, 8192)), dim3(256), 0, s, base, g.ny, g.nz, box[0],
+                       box[2], box[4], ni, nj, nk, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_point_ops(float* F, size_t N, const PointExc* exc, int n_exc, uint64_t step,
+                            const PointProbe* probes, int n_probes, hipStream_t s) {
+    hipLaunchKernelGGL(k_point_ops, dim3(1), dim3(64), 0, s, F, N, exc, n_exc, step, probes, n_probes);
+    return hipGetLastError();
+}
+
+int energy_blocks() { return EN_BLOCKS; }
+
+hipError_t launch_energy(const float* F, size_t N, double* partials, int blocks, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_energy, dim3(blocks), dim3(EN_THREADS), 0, s, F, N, partials);
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(EN_THREADS), 0, s, partials, blocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mur_pre(float* F, const Grid& g, const MurDev* faces, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mur_pre, dim3(256, n), dim3(256), 0, s, F, g, faces);
+    return hipGetLastError();
+}
+
+hipError_t launch_mur_post(float* F, const Grid& g, const MurDev* faces, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mur_post, dim3(256, n), dim3(256), 0, s, F, g, faces);
+    for (int f = 0; f < n; ++f) hipLaunchKernelGGL(k_mur_apply, dim3(256), dim3(256), 0, s, F, g, faces, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_ade_pre(const float* F, size_t N, const AdeDev& a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ade_pre, dim3((a.n + 255) / 256), dim3(256), 0, s, F, N, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_ade_post(float* F, const float* Cf, size_t N, const AdeDev& a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ade_post, dim3((a.n + 255) / 256), dim3(256), 0, s, F, Cf, N, a);
+    return hipGetLastError();
+}
+
+}  // namespace oe

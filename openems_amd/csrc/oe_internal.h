@@ -1,0 +1,121 @@
+// oe_internal.h -- types shared by the host runtime (oe_api.hip) and the
+// gfx950 kernels (oe_yee.hip, oe_aux.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace oe {
+
+constexpr double C0 = 299792458.0;
+constexpr double EPS0 = 8.854187817e-12;
+constexpr double MU0 = 1.256637062e-6;
+constexpr double Z0 = 376.730313668;
+
+// Field block: one allocation of 6 arrays of N floats, E(x,y,z) then H(x,y,z).
+// Coefficient block: 12 arrays of N floats, Ca(x,y,z) Cb(x,y,z) Da(x,y,z) Db(x,y,z).
+struct Grid {
+    int nx, ny, nz;
+    size_t N;
+};
+
+// Yee kernel launch shape.  A work-group covers bx*V consecutive k of `by`
+// rows (j) and marches along i over `ci` planes, keeping the i+1 (H update)
+// or i-1 (E update) plane of the marched components in registers.
+struct Tiles {
+    int vec;  // 4: float4 accesses (nz % 4 == 0), 1: scalar
+    int bx;   // lanes along k (power of two)
+    int by;   // rows along j; bx * by == 256
+    int ci;   // planes along i per work-group
+};
+
+// CPML state as the fused Yee kernels see it.  Face f = 2*dir + (max side).
+// psi layout for face f: dims (nx,ny,nz) with axis `dir` replaced by L[f],
+// z fastest.  tab holds, per face, 6 tables of Lmax floats:
+// b_e, b_h, a_e, a_h, (1/kappa_e - 1), (1/kappa_h - 1).
+constexpr int CPML_TABLES = 6;
+enum { TAB_BE = 0, TAB_BH = 1, TAB_AE = 2, TAB_AH = 3, TAB_KE = 4, TAB_KH = 5 };
+
+struct CpmlArgs {
+    int on_mask;   // bit f set when face f is active
+    int physical;  // 1 = apply the field correction
+    int Lmax;
+    int L[6];
+    int g0[6];     // first global index of the slab along its axis
+    float* pe_pos[6];
+    float* pe_neg[6];
+    float* ph_pos[6];
+    float* ph_neg[6];
+    const float* tab;  // [6][CPML_TABLES][Lmax]
+};
+
+struct YeeArgs {
+    Grid g;
+    float* F;        // fields
+    const float* Cf; // coefficients
+    Tiles t;
+    int n_kb, n_jb, n_ib;  // blocks along k, j, i
+    int xcd_remap;         // 1: make consecutive tiles share an XCD (L2)
+};
+
+// Kernel launchers (oe_yee.hip).
+hipError_t launch_update_h(const YeeArgs& a, const CpmlArgs* c, hipStream_t s);
+hipError_t launch_update_e(const YeeArgs& a, const CpmlArgs* c, hipStream_t s);
+
+// Auxiliary kernels (oe_aux.hip).
+hipError_t launch_fill(float* p, size_t n, float v, hipStream_t s);
+hipError_t launch_box_fill(float* base, const Grid& g, const int box[6], float v, hipStream_t s);
+
+struct PointExc {
+    uint32_t idx;   // linear cell index
+    uint32_t comp;  // 0..2 (E component)
+    uint32_t soft;
+    uint32_t pad;
+    const float* samples;
+    uint64_t n_samples;
+};
+struct PointProbe {
+    uint32_t idx;
+    uint32_t pad;
+    double* out;    // [capacity][6]
+    uint64_t capacity;
+    uint64_t* count;  // device counter
+};
+hipError_t launch_point_ops(float* F, size_t N, const PointExc* exc, int n_exc, uint64_t step,
+                            const PointProbe* probes, int n_probes, hipStream_t s);
+
+// Sum of squares of E (out[0]) and H (out[1]); partials has >= 2*blocks doubles.
+hipError_t launch_energy(const float* F, size_t N, double* partials, int blocks, double* out,
+                         hipStream_t s);
+int energy_blocks();
+
+// Mur ABC face (src/extensions/mur_abc.rs:59-151).
+struct MurDev {
+    int ny, nyp, nypp;
+    int line, shift;
+    int s0, s1;
+    float coeff;
+    float* v_nyp;
+    float* v_nypp;
+};
+hipError_t launch_mur_pre(float* F, const Grid& g, const MurDev* faces, int n, hipStream_t s);
+hipError_t launch_mur_post(float* F, const Grid& g, const MurDev* faces, int n, hipStream_t s);
+
+// ADE (src/extensions/dispersive.rs).  One thread per listed cell.
+struct AdeDev {
+    int kind, mode;
+    int n;
+    const uint32_t* idx;  // linear indices
+    float* J;             // [3][n]
+    float* Jold;          // [3][n] (Lorentz physical: P^{n-1})
+    float* Eprev;         // [3][n] (physical: E^n captured before the E update)
+    float a, b, c, dt_t;  // compat coefficients (float, as the reference)
+    float pa, pb, pc;     // physical coefficients
+    double delta[3];      // cell sizes: K = Cb * delta per cell and component
+    float dtf;
+};
+hipError_t launch_ade_pre(const float* F, size_t N, const AdeDev& a, hipStream_t s);
+hipError_t launch_ade_post(float* F, const float* Cf, size_t N, const AdeDev& a, hipStream_t s);
+
+}  // namespace oe

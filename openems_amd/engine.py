@@ -1,0 +1,362 @@
+"""Python mirror of the reference's engine-level API over liboe.so.
+
+Names follow the Rust reference so tests read like its own harness
+(src/fdtd/engine_testing.rs:401-475):
+
+    op = Operator.uniform(nx, ny, nz, delta)          # Grid::uniform + Operator::new
+    op.set_material(eps_r, mu_r, sigma_e, sigma_m, region)
+    eng = HipEngine(op)                                # EngineImpl::new
+    cpml = eng.cpml(CpmlBoundaries(...))               # device-resident extension
+    res = eng.run_batch(EngineBatch(num_steps=..., excitations=[...], extensions=[cpml]))
+    E, H = eng.read_fields()
+
+All compute happens in liboe.so on the GPU; this module only marshals
+arguments.  Excitation waveforms are sampled by the library's C++ host code
+(oe_waveform_sample, the counterpart of ScheduledExcitation::from_excitation).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import check, lib
+
+MODE_COMPAT = 0
+MODE_PHYSICAL = 1
+ADE_LORENTZ, ADE_DRUDE, ADE_DEBYE = 0, 1, 2
+
+EXC_GAUSSIAN, EXC_SINUSOIDAL, EXC_GAUSSIAN_MODULATED, EXC_DIRAC, EXC_STEP, EXC_CUSTOM = range(6)
+
+
+# ------------------------------------------------------------------ operator
+class Operator:
+    """Operator (src/fdtd/operator.rs:40-271): grid, dt and material boxes."""
+
+    def __init__(self, handle: int, shape: Tuple[int, int, int], delta: Tuple[float, float, float]):
+        self._h = C.c_void_p(handle)
+        self.shape = shape
+        self.delta = delta
+
+    @classmethod
+    def uniform(cls, nx: int, ny: int, nz: int, delta) -> "Operator":
+        if np.isscalar(delta):
+            delta = (float(delta),) * 3
+        d = (C.c_double * 3)(*delta)
+        h = C.c_void_p()
+        check(lib().oe_operator_create(nx, ny, nz, d, C.byref(h)))
+        return cls(h.value, (nx, ny, nz), tuple(map(float, delta)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().oe_operator_destroy(self._h)
+            self._h = C.c_void_p()
+
+    @property
+    def dt(self) -> float:
+        return lib().oe_operator_timestep(self._h)
+
+    def set_material(self, eps_r, mu_r, sigma_e, sigma_m, region):
+        r = (C.c_size_t * 6)(*map(int, region))
+        check(lib().oe_operator_set_material(self._h, eps_r, mu_r, sigma_e, sigma_m, r))
+
+    def set_pec(self, region):
+        r = (C.c_size_t * 6)(*map(int, region))
+        check(lib().oe_operator_set_pec(self._h, r))
+
+
+# -------------------------------------------------------------- waveforms
+@dataclass
+class Waveform:
+    """ExcitationType + amplitude (src/fdtd/excitation.rs:9-57)."""
+
+    type: int
+    t0: float = 0.0
+    tau: float = 0.0
+    frequency: float = 0.0
+    times: Optional[Sequence[float]] = None
+    values: Optional[Sequence[float]] = None
+    amplitude: float = 1.0
+
+    @staticmethod
+    def gaussian(t0: float, tau: float, amplitude: float = 1.0) -> "Waveform":
+        return Waveform(EXC_GAUSSIAN, t0=t0, tau=tau, amplitude=amplitude)
+
+    @staticmethod
+    def openems_gaussian(fc: float, amplitude: float = 1.0) -> "Waveform":
+        """openEMS Gaussian with f0 = 0 and cutoff fc, written as the reference's
+        Gaussian{t0, tau}: tau = 3/(2 pi fc), t0 = 9/(2 pi fc)
+        (matlab/examples/other/gauss_excitation_test.m:14-21)."""
+        return Waveform(EXC_GAUSSIAN, t0=9.0 / (2 * math.pi * fc), tau=3.0 / (2 * math.pi * fc), amplitude=amplitude)
+
+    @staticmethod
+    def reference_gaussian(center_freq: float, bandwidth: float, amplitude: float = 1.0) -> "Waveform":
+        """Excitation::gaussian (src/fdtd/excitation.rs:67-85)."""
+        f_max = center_freq * (1.0 + 0.5 * bandwidth)
+        tau = 1.0 / (math.pi * f_max)
+        return Waveform(EXC_GAUSSIAN, t0=4.0 * tau, tau=tau, amplitude=amplitude)
+
+    def _c(self):
+        w = _ffi.Waveform()
+        w.type = self.type
+        w.t0, w.tau, w.frequency, w.amplitude = self.t0, self.tau, self.frequency, self.amplitude
+        keep = []
+        if self.type == EXC_CUSTOM:
+            t = np.ascontiguousarray(self.times, dtype=np.float64)
+            v = np.ascontiguousarray(self.values, dtype=np.float64)
+            keep = [t, v]
+            w.times = t.ctypes.data_as(C.POINTER(C.c_double))
+            w.values = v.ctypes.data_as(C.POINTER(C.c_double))
+            w.n_points = len(t)
+        return w, keep
+
+    def evaluate(self, t: float) -> float:
+        w, keep = self._c()
+        out = C.c_double()
+        check(lib().oe_waveform_evaluate(C.byref(w), float(t), C.byref(out)))
+        return out.value
+
+    def sample(self, start_timestep: int, n: int, dt: float) -> np.ndarray:
+        w, keep = self._c()
+        out = np.empty(n, np.float32)
+        check(lib().oe_waveform_sample(C.byref(w), start_timestep, n, dt, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+
+@dataclass
+class ScheduledExcitation:
+    """ScheduledExcitation with a Sampled waveform (src/fdtd/batch.rs:71-100)."""
+
+    position: Tuple[int, int, int]
+    direction: int
+    samples: np.ndarray
+    soft_source: bool = True
+
+
+# ------------------------------------------------------------- extensions
+@dataclass
+class CpmlBoundaries:
+    """CpmlBoundaries (src/extensions/cpml.rs:27-51) + the mode switch."""
+
+    layers: Sequence[int] = (8, 8, 8, 8, 8, 8)
+    grading_order: float = 3.0
+    reflection_coeff: float = 1e-6
+    alpha_max: float = 0.0
+    kappa_max: float = 1.0
+    mode: int = MODE_COMPAT
+
+
+class Extension:
+    def __init__(self, eng: "HipEngine", handle: int, kind: str):
+        self._eng = eng
+        self._h = C.c_void_p(handle)
+        self.kind = kind
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().oe_extension_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def reset(self):
+        check(lib().oe_extension_reset(self._h))
+
+    # CPML
+    def psi(self, face: int, field: str, which: str) -> Optional[np.ndarray]:
+        f = 0 if field == "E" else 1
+        w = 0 if which == "pos" else 1
+        n = C.c_size_t()
+        check(lib().oe_ext_cpml_read_psi(self._h, face, f, w, None, 0, C.byref(n)))
+        if n.value == 0:
+            return None
+        out = np.empty(n.value, np.float32)
+        check(lib().oe_ext_cpml_read_psi(self._h, face, f, w, out.ctypes.data_as(C.POINTER(C.c_float)), n.value, C.byref(n)))
+        return out
+
+    # ADE
+    def state(self) -> np.ndarray:
+        n = C.c_size_t()
+        check(lib().oe_ext_ade_read_state(self._h, None, 0, C.byref(n)))
+        out = np.empty(n.value, np.float32)
+        check(lib().oe_ext_ade_read_state(self._h, out.ctypes.data_as(C.POINTER(C.c_float)), n.value, C.byref(n)))
+        return out.reshape(3, -1)
+
+    # probes
+    def data(self) -> np.ndarray:
+        n = C.c_size_t()
+        check(lib().oe_ext_probe_read(self._h, None, 0, C.byref(n)))
+        out = np.empty((n.value, 6), np.float64)
+        if n.value:
+            check(lib().oe_ext_probe_read(self._h, out.ctypes.data_as(C.POINTER(C.c_double)), n.value, C.byref(n)))
+        return out
+
+
+@dataclass
+class EngineBatch:
+    """EngineBatch + TerminationConfig + EnergyMonitorConfig (src/fdtd/batch.rs)."""
+
+    num_steps: Optional[int] = None
+    excitations: List[ScheduledExcitation] = field(default_factory=list)
+    extensions: List[Extension] = field(default_factory=list)
+    max_timesteps: Optional[int] = None
+    energy_decay_db: Optional[float] = None
+    check_interval: int = 100
+    sample_interval: int = 0
+    track_peak: bool = False
+    persistent_peak: bool = False
+    energy_capacity: int = 100000
+
+
+@dataclass
+class BatchResult:
+    timesteps_executed: int
+    termination_reason: int
+    final_decay_db: float
+    energy_samples: np.ndarray  # [n, 3] timestep, e, h
+    elapsed_seconds: float
+
+
+# ----------------------------------------------------------------- engine
+class HipEngine:
+    """EngineImpl on one MI355X (src/fdtd/engine_impl.rs:24-71)."""
+
+    def __init__(self, op: Operator, device: int = 0):
+        h = C.c_void_p()
+        check(lib().oe_engine_create(op._h, device, C.byref(h)))
+        self._h = h
+        self.shape = op.shape
+        self.N = int(np.prod(op.shape))
+        self.dt = op.dt
+        self.delta = op.delta
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().oe_engine_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def timestep(self) -> int:
+        return lib().oe_engine_timestep(self._h)
+
+    def read_fields(self) -> Tuple[np.ndarray, np.ndarray]:
+        out = np.empty((6,) + tuple(self.shape), np.float32)
+        ptrs = (C.POINTER(C.c_float) * 6)(*[out[d].ctypes.data_as(C.POINTER(C.c_float)) for d in range(6)])
+        check(lib().oe_engine_read_fields(self._h, ptrs))
+        return out[:3], out[3:]
+
+    def write_fields(self, E: Optional[np.ndarray] = None, H: Optional[np.ndarray] = None):
+        arrs = [None] * 6
+        keep = []
+        for base, F in ((0, E), (3, H)):
+            if F is None:
+                continue
+            for d in range(3):
+                a = np.ascontiguousarray(F[d], dtype=np.float32)
+                keep.append(a)
+                arrs[base + d] = a.ctypes.data_as(C.POINTER(C.c_float))
+        ptrs = (C.POINTER(C.c_float) * 6)(*[p if p is not None else C.POINTER(C.c_float)() for p in arrs])
+        check(lib().oe_engine_write_fields(self._h, ptrs))
+
+    def coefficients(self) -> np.ndarray:
+        out = np.empty((12,) + tuple(self.shape), np.float32)
+        for w in range(12):
+            check(lib().oe_engine_read_coefficients(self._h, w, out[w].ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def reset(self):
+        check(lib().oe_engine_reset(self._h))
+
+    def energy(self) -> Tuple[float, float]:
+        e, h = C.c_double(), C.c_double()
+        check(lib().oe_engine_energy(self._h, C.byref(e), C.byref(h)))
+        return e.value, h.value
+
+    def set_profiling(self, on: bool):
+        check(lib().oe_engine_set_profiling(self._h, int(on)))
+
+    def kernel_time(self, name: str) -> Tuple[int, float]:
+        n, ms = C.c_uint64(), C.c_double()
+        check(lib().oe_engine_kernel_time(self._h, name.encode(), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def set_tiles(self, bx=0, by=0, ci=0, vec=0):
+        check(lib().oe_engine_set_tiles(self._h, bx, by, ci, vec))
+
+    def tiles(self):
+        t = (C.c_int * 4)()
+        check(lib().oe_engine_get_tiles(self._h, t))
+        return dict(bx=t[0], by=t[1], ci=t[2], vec=t[3])
+
+    # extensions
+    def cpml(self, cfg: CpmlBoundaries) -> Extension:
+        c = _ffi.CpmlConfig()
+        for q in range(6):
+            c.layers[q] = int(cfg.layers[q])
+        c.grading_order, c.reflection_coeff = cfg.grading_order, cfg.reflection_coeff
+        c.alpha_max, c.kappa_max, c.mode = cfg.alpha_max, cfg.kappa_max, cfg.mode
+        h = C.c_void_p()
+        check(lib().oe_ext_cpml_create(self._h, C.byref(c), C.byref(h)))
+        return Extension(self, h.value, "cpml")
+
+    def mur(self, enabled: Sequence[bool] = (True,) * 6, phase_velocity: float = 0.0) -> Extension:
+        en = (C.c_int * 6)(*[int(bool(x)) for x in enabled])
+        h = C.c_void_p()
+        check(lib().oe_ext_mur_create(self._h, en, phase_velocity, C.byref(h)))
+        return Extension(self, h.value, "mur")
+
+    def ade(self, kind: int, params: Sequence[float], positions, mode: int = MODE_COMPAT) -> Extension:
+        p = np.ascontiguousarray(np.asarray(positions, dtype=np.uint32).reshape(-1, 3))
+        prm = (C.c_double * 3)(*(list(params) + [0.0] * (3 - len(params))))
+        h = C.c_void_p()
+        check(lib().oe_ext_ade_create(self._h, kind, prm, p.ctypes.data_as(C.POINTER(C.c_uint32)), p.shape[0], mode,
+                                      C.byref(h)))
+        return Extension(self, h.value, "ade")
+
+    def field_probe(self, pos) -> Extension:
+        h = C.c_void_p()
+        check(lib().oe_ext_field_probe_create(self._h, *map(int, pos), C.byref(h)))
+        return Extension(self, h.value, "probe")
+
+    def run_batch(self, batch: EngineBatch) -> BatchResult:
+        n_exc = len(batch.excitations)
+        ex = (_ffi.Excitation * max(n_exc, 1))()
+        keep = []
+        for q, s in enumerate(batch.excitations):
+            a = np.ascontiguousarray(s.samples, dtype=np.float32)
+            keep.append(a)
+            i, j, k = s.position
+            ex[q] = _ffi.Excitation(i, j, k, s.direction, int(bool(s.soft_source)),
+                                    a.ctypes.data_as(C.POINTER(C.c_float)), a.size)
+        n_ext = len(batch.extensions)
+        exts = (C.c_void_p * max(n_ext, 1))(*[x._h.value for x in batch.extensions])
+        cap = batch.energy_capacity
+        samples = (_ffi.EnergySample * max(cap, 1))()
+        d = _ffi.BatchDesc()
+        d.has_num_steps = int(batch.num_steps is not None)
+        d.num_steps = int(batch.num_steps or 0)
+        d.excitations = ex
+        d.n_excitations = n_exc
+        d.extensions = exts
+        d.n_extensions = n_ext
+        d.has_max_timesteps = int(batch.max_timesteps is not None)
+        d.max_timesteps = int(batch.max_timesteps or 0)
+        d.has_energy_decay_db = int(batch.energy_decay_db is not None)
+        d.energy_decay_db = float(batch.energy_decay_db or 0.0)
+        d.check_interval = batch.check_interval
+        d.sample_interval = batch.sample_interval
+        d.track_peak = int(batch.track_peak)
+        d.persistent_peak = int(batch.persistent_peak)
+        d.energy_samples = samples
+        d.energy_samples_capacity = cap
+        r = _ffi.BatchResultC()
+        check(lib().oe_engine_run_batch(self._h, C.byref(d), C.byref(r)))
+        n = min(r.n_energy_samples, cap)
+        es = np.array([(samples[q].timestep, samples[q].e_energy, samples[q].h_energy) for q in range(n)],
+                      dtype=np.float64).reshape(-1, 3)
+        return BatchResult(r.timesteps_executed, r.termination_reason, r.final_decay_db, es, r.elapsed_seconds)
