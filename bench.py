@@ -81,7 +81,7 @@ def run_steps(eng, exts, wl, dt, start, n, sample_interval=100):
                                      extensions=exts, sample_interval=sample_interval, track_peak=True))
 
 
-def cpu_baseline(wl, budget_s=12.0, max_steps=200):
+def cpu_baseline(wl, budget_s=12.0, max_steps=4000):
     """Time the oracle (OpenMP, all host threads) on the same workload."""
     import numpy as np
 

@@ -93,6 +93,9 @@ oe_status oe_engine_create_from_arrays(size_t nx, size_t ny, size_t nz, const do
                                        double dt, const float* const coeff[12], int device,
                                        oe_engine** out);
 
+/* Frees the engine.  Extensions created on it stay valid handles but are
+ * detached: every later call on them except oe_extension_destroy returns
+ * OE_ERR_CONFIG. */
 void oe_engine_destroy(oe_engine* eng);
 
 /* EngineImpl::current_timestep (src/fdtd/engine_impl.rs:53). */

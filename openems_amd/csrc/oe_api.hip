@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -122,6 +123,7 @@ struct oe_engine {
     int xcd_remap = 1;
     double peak_persistent = 0.0;
     oe_extension* cpml_owner = nullptr;
+    std::vector<oe_extension*> exts;  // live extensions bound to this engine
     // energy
     double* partials = nullptr;
     double* energy_dev = nullptr;   // ring of [cap][2]
@@ -158,7 +160,8 @@ int pow2ceil(int v) {
 
 // Automatic tile choice: 256-lane work-groups, as many lanes along k as the
 // row holds (up to one wave of 64 float4 lanes), and i-chunks sized for
-// about 1024 work-groups (4 per CU on 256 CUs).
+// about 16384 work-groups (64 per CU on 256 CUs), at least 4 planes deep
+// (measured best: 4-8 planes at 256^3 and 512^3).
 Tiles auto_tiles(const Grid& g, int vec_req) {
     Tiles t{};
     t.vec = (vec_req == 1) ? 1 : ((g.nz % 4 == 0) ? 4 : 1);
@@ -169,7 +172,7 @@ Tiles auto_tiles(const Grid& g, int vec_req) {
     const int n_kb = (nzv + t.bx - 1) / t.bx;
     const int n_jb = (g.ny + t.by - 1) / t.by;
     const long plane_blocks = (long)n_kb * n_jb;
-    long n_ib = (1024 + plane_blocks - 1) / plane_blocks;
+    long n_ib = (16384 + plane_blocks - 1) / plane_blocks;
     n_ib = std::max(1L, std::min(n_ib, (long)g.nx));
     int ci = (int)((g.nx + n_ib - 1) / n_ib);
     ci = std::max(ci, std::min(4, g.nx));
@@ -177,18 +180,63 @@ Tiles auto_tiles(const Grid& g, int vec_req) {
     return t;
 }
 
-YeeArgs yee_args(const oe_engine* e) {
+struct Launch {
+    YeeArgs a;
+    int axes;
+};
+
+Box make_box(const Tiles& t, int i0, int i1, int j0, int j1) {
+    Box b{};
+    b.i0 = i0; b.i1 = i1; b.j0 = j0; b.j1 = j1;
+    b.n_jb = (j1 - j0 + t.by - 1) / t.by;
+    b.n_ib = (i1 - i0 + t.ci - 1) / t.ci;
+    return b;
+}
+
+YeeArgs launch_args(const oe_engine* e, const Tiles& t, const std::vector<Box>& boxes) {
     YeeArgs a{};
     a.g = e->g;
     a.F = e->F;
     a.Cf = e->Cf;
-    a.t = e->tiles;
-    const int nzv = (e->g.nz + a.t.vec - 1) / a.t.vec;
-    a.n_kb = (nzv + a.t.bx - 1) / a.t.bx;
-    a.n_jb = (e->g.ny + a.t.by - 1) / a.t.by;
-    a.n_ib = (e->g.nx + a.t.ci - 1) / a.t.ci;
+    a.t = t;
+    const int nzv = (e->g.nz + t.vec - 1) / t.vec;
+    a.n_kb = (nzv + t.bx - 1) / t.bx;
+    a.nbox = (int)boxes.size();
+    a.first[0] = 0;
+    for (int b = 0; b < a.nbox; ++b) {
+        a.box[b] = boxes[b];
+        a.first[b + 1] = a.first[b] + a.n_kb * boxes[b].n_jb * boxes[b].n_ib;
+    }
     a.xcd_remap = e->xcd_remap;
     return a;
+}
+
+// Launch plan of one half-step.  Without CPML: one launch of the plain kernel
+// over the grid.  With CPML: the x and y face slabs (whole planes / rows) go
+// to the all-faces kernel in one launch of up to four boxes, and the interior
+// planes and rows -- whose only slab cells are the z faces at the ends of each
+// row -- to the lighter z-faces kernel.  The boxes tile the grid exactly once.
+std::vector<Launch> plan_launches(const oe_engine* e, const Tiles& t, const CpmlArgs* cp) {
+    std::vector<Launch> out;
+    const Grid& g = e->g;
+    if (!cp) {
+        out.push_back({launch_args(e, t, {make_box(t, 0, g.nx, 0, g.ny)}), 0});
+        return out;
+    }
+    const int m = cp->on_mask;
+    const int ix0 = (m & 1) ? cp->L[0] : 0, ix1 = (m & 2) ? cp->g0[1] : g.nx;
+    const int jy0 = (m & 4) ? cp->L[2] : 0, jy1 = (m & 8) ? cp->g0[3] : g.ny;
+    std::vector<Box> slabs;
+    if (ix0 > 0) slabs.push_back(make_box(t, 0, ix0, 0, g.ny));
+    if (ix1 < g.nx) slabs.push_back(make_box(t, ix1, g.nx, 0, g.ny));
+    if (ix1 > ix0) {
+        if (jy0 > 0) slabs.push_back(make_box(t, ix0, ix1, 0, jy0));
+        if (jy1 < g.ny) slabs.push_back(make_box(t, ix0, ix1, jy1, g.ny));
+    }
+    if (!slabs.empty()) out.push_back({launch_args(e, t, slabs), AX_X | AX_Y | AX_Z});
+    if (ix1 > ix0 && jy1 > jy0)
+        out.push_back({launch_args(e, t, {make_box(t, ix0, ix1, jy0, jy1)}), (m & 48) ? AX_Z : 0});
+    return out;
 }
 
 hipEvent_t take_event(oe_engine* e) {
@@ -296,12 +344,17 @@ oe_status engine_alloc(oe_engine* e, size_t nx, size_t ny, size_t nz, int device
     HIP_OK(hipMalloc(&e->energy_dev, 2 * e->energy_dev_cap * sizeof(double)));
     HIP_OK(hipHostMalloc(&e->energy_host, 2 * e->energy_dev_cap * sizeof(double), hipHostMallocDefault));
     e->tiles = auto_tiles(e->g, 0);
+    if (const char* v = getenv("OE_XCD_REMAP")) e->xcd_remap = atoi(v) != 0;
     return OE_OK;
 }
 
 void engine_free(oe_engine* e) {
     if (!e) return;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    // Extensions may outlive their engine: detach them so that later calls on
+    // them report an error instead of touching the freed engine.
+    for (oe_extension* x : e->exts) x->eng = nullptr;
+    e->exts.clear();
     for (auto& pr : e->pending) {
         (void)hipEventDestroy(pr.second.first);
         (void)hipEventDestroy(pr.second.second);
@@ -574,7 +627,13 @@ oe_status oe_ext_cpml_create(oe_engine* e, const oe_cpml_config* cfg, oe_extensi
     const double sigma_max = -(m + 1.0) * std::log(cfg->reflection_coeff) / (2.0 * Z0 * dd);
     CpmlArgs& c = x->cpml;
     c.physical = cfg->mode == OE_MODE_PHYSICAL;
-    c.Lmax = (int)std::max<size_t>(lmax, 1);
+    c.Lmax = (int)((std::max<size_t>(lmax, 1) + 3) & ~(size_t)3);  // 16-byte table rows
+    for (int dir = 0; dir < 3; ++dir) {
+        const size_t a0 = cfg->layers[2 * dir], a1 = cfg->layers[2 * dir + 1];
+        const bool on0 = a0 > 0 && a0 < (size_t)n[dir], on1 = a1 > 0 && a1 < (size_t)n[dir];
+        if (on0 && on1 && a0 + a1 > (size_t)n[dir])
+            return fail(OE_ERR_CONFIG, "CPML faces of axis %d overlap (%zu + %zu layers > %d cells)", dir, a0, a1, n[dir]);
+    }
     std::vector<float> tab((size_t)6 * CPML_TABLES * c.Lmax, 0.0f);
     size_t total = 0;
     size_t offs[6];
@@ -619,8 +678,14 @@ oe_status oe_ext_cpml_create(oe_engine* e, const oe_cpml_config* cfg, oe_extensi
         x->psi_count[f] = cnt;
         total += 4 * cnt;
     }
-    HIP_OK(x->cpml_mem.alloc(std::max<size_t>(total, 64) * sizeof(float)));
+    const size_t dummy_off = total;  // 64 zeroed floats read by cells outside every slab
+    total += 64;
+    HIP_OK(x->cpml_mem.alloc(total * sizeof(float)));
     HIP_OK(hipMemsetAsync(x->cpml_mem.p, 0, x->cpml_mem.bytes, e->stream));
+    c.dummy = x->cpml_mem.as<float>() + dummy_off;
+    c.zvec = (e->g.nz % 4 == 0) ? 1 : 0;
+    for (int f = 4; f < 6; ++f)
+        if ((c.on_mask & (1 << f)) && c.L[f] % 4 != 0) c.zvec = 0;
     for (int f = 0; f < 6; ++f) {
         if (!(c.on_mask & (1 << f))) continue;
         float* base = x->cpml_mem.as<float>() + offs[f];
@@ -635,6 +700,7 @@ oe_status oe_ext_cpml_create(oe_engine* e, const oe_cpml_config* cfg, oe_extensi
     HIP_OK(hipStreamSynchronize(e->stream));
     c.tab = x->cpml_tab.as<float>();
     e->cpml_owner = x.get();
+    e->exts.push_back(x.get());
     *out = x.release();
     return OE_OK;
 }
@@ -678,6 +744,7 @@ oe_status oe_ext_mur_create(oe_engine* e, const int enabled[6], double phase_vel
     if (!x->mur_faces.empty())
         HIP_OK(hipMemcpyAsync(x->mur_dev.p, x->mur_faces.data(), x->mur_faces.size() * sizeof(MurDev), hipMemcpyHostToDevice, e->stream));
     HIP_OK(hipStreamSynchronize(e->stream));
+    e->exts.push_back(x.get());
     *out = x.release();
     return OE_OK;
 }
@@ -740,6 +807,7 @@ oe_status oe_ext_ade_create(oe_engine* e, int kind, const double params[3], cons
     a.Eprev = fb + 6 * cnt;
     if (n) HIP_OK(hipMemcpyAsync(di, idx.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, e->stream));
     HIP_OK(hipStreamSynchronize(e->stream));
+    e->exts.push_back(x.get());
     *out = x.release();
     return OE_OK;
 }
@@ -757,6 +825,7 @@ oe_status oe_ext_field_probe_create(oe_engine* e, uint32_t i, uint32_t j, uint32
     HIP_OK(x->probe_cnt.alloc(sizeof(uint64_t)));
     HIP_OK(hipMemsetAsync(x->probe_cnt.p, 0, sizeof(uint64_t), e->stream));
     HIP_OK(hipStreamSynchronize(e->stream));
+    e->exts.push_back(x.get());
     *out = x.release();
     return OE_OK;
 }
@@ -781,6 +850,7 @@ static oe_status probe_reserve(oe_extension* x, uint64_t need) {
 
 oe_status oe_ext_probe_read(oe_extension* x, double* out, size_t capacity, size_t* n_samples) {
     if (!x || x->kind != EXT_PROBE) return fail(OE_ERR_CONFIG, "not a probe extension");
+    if (!x->eng) return fail(OE_ERR_CONFIG, "the extension's engine was destroyed");
     oe_engine* e = x->eng;
     HIP_OK(hipSetDevice(e->device));
     uint64_t cnt = 0;
@@ -800,6 +870,7 @@ oe_status oe_ext_cpml_read_psi(oe_extension* x, int face, int field, int which, 
     if (face < 0 || face > 5 || field < 0 || field > 1 || which < 0 || which > 1) return fail(OE_ERR_CONFIG, "bad psi selector");
     const CpmlArgs& c = x->cpml;
     if (!(c.on_mask & (1 << face))) { if (n) *n = 0; return OE_OK; }
+    if (!x->eng) return fail(OE_ERR_CONFIG, "the extension's engine was destroyed");
     oe_engine* e = x->eng;
     size_t cnt = (size_t)c.L[face];
     const int dims[3] = {e->g.nx, e->g.ny, e->g.nz};
@@ -816,6 +887,7 @@ oe_status oe_ext_cpml_read_psi(oe_extension* x, int face, int field, int which, 
 
 oe_status oe_ext_ade_read_state(oe_extension* x, float* out, size_t capacity, size_t* n) {
     if (!x || x->kind != EXT_ADE) return fail(OE_ERR_CONFIG, "not an ADE extension");
+    if (!x->eng) return fail(OE_ERR_CONFIG, "the extension's engine was destroyed");
     const size_t cnt = (size_t)x->ade.n;
     if (n) *n = 3 * cnt;
     if (out && capacity) {
@@ -832,6 +904,7 @@ oe_status oe_ext_ade_read_state(oe_extension* x, float* out, size_t capacity, si
 
 oe_status oe_extension_reset(oe_extension* x) {
     if (!x) return fail(OE_ERR_CONFIG, "null argument");
+    if (!x->eng) return fail(OE_ERR_CONFIG, "the extension's engine was destroyed");
     oe_engine* e = x->eng;
     HIP_OK(hipSetDevice(e->device));
     if (x->kind == EXT_CPML) HIP_OK(hipMemsetAsync(x->cpml_mem.p, 0, x->cpml_mem.bytes, e->stream));
@@ -848,9 +921,11 @@ oe_status oe_extension_reset(oe_extension* x) {
 void oe_extension_destroy(oe_extension* x) {
     if (!x) return;
     if (x->eng) {
-        (void)hipSetDevice(x->eng->device);
-        (void)hipStreamSynchronize(x->eng->stream);
-        if (x->eng->cpml_owner == x) x->eng->cpml_owner = nullptr;
+        oe_engine* e = x->eng;
+        (void)hipSetDevice(e->device);
+        (void)hipStreamSynchronize(e->stream);
+        if (e->cpml_owner == x) e->cpml_owner = nullptr;
+        e->exts.erase(std::remove(e->exts.begin(), e->exts.end(), x), e->exts.end());
     }
     delete x;
 }
@@ -873,6 +948,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
     for (size_t q = 0; q < b->n_extensions; ++q) {
         oe_extension* x = b->extensions[q];
         if (!x) return fail(OE_ERR_CONFIG, "extension %zu is null", q);
+        if (!x->eng) return fail(OE_ERR_CONFIG, "extension %zu: its engine was destroyed", q);
         if (x->eng != e) return fail(OE_ERR_CONFIG, "extension %zu belongs to another engine", q);
         switch (x->kind) {
             case EXT_CPML:
@@ -932,7 +1008,9 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
         HIP_OK(hipMemcpyAsync(e->probe_table.p, pp.data(), pp.size() * sizeof(PointProbe), hipMemcpyHostToDevice, e->stream));
     const bool point_ops = !px.empty() || !pp.empty();
 
-    const YeeArgs ya = yee_args(e);
+    Tiles tl = e->tiles;
+    if (cp && !cp->zvec && tl.vec == 4) tl = auto_tiles(e->g, 1);  // z slabs not 4-aligned: scalar lanes
+    const std::vector<Launch> plan = plan_launches(e, tl, cp);
     double peak = b->persistent_peak ? e->peak_persistent : 0.0;
     std::vector<uint64_t> sample_steps;  // timesteps whose energy is pending in energy_dev
     size_t n_samples_total = 0;
@@ -967,7 +1045,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
         const uint64_t cur = e->timestep;
         {
             Timed tm(e, "update_h");
-            HIP_OK(launch_update_h(ya, cp, e->stream));
+            for (const Launch& L : plan) HIP_OK(launch_update_h(L.a, cp, L.axes, e->stream));
         }
         for (auto* x : murs) {
             Timed tm(e, "mur_pre");
@@ -980,7 +1058,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             }
         {
             Timed tm(e, "update_e");
-            HIP_OK(launch_update_e(ya, cp, e->stream));
+            for (const Launch& L : plan) HIP_OK(launch_update_e(L.a, cp, L.axes, e->stream));
         }
         for (auto* x : ades) {
             Timed tm(e, "ade_post");

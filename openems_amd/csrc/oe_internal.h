@@ -40,7 +40,8 @@ enum { TAB_BE = 0, TAB_BH = 1, TAB_AE = 2, TAB_AH = 3, TAB_KE = 4, TAB_KH = 5 };
 struct CpmlArgs {
     int on_mask;   // bit f set when face f is active
     int physical;  // 1 = apply the field correction
-    int Lmax;
+    int Lmax;      // table stride, a multiple of 4
+    int zvec;      // z faces can be handled 4 cells at a time (L % 4 == 0, nz % 4 == 0)
     int L[6];
     int g0[6];     // first global index of the slab along its axis
     float* pe_pos[6];
@@ -48,6 +49,15 @@ struct CpmlArgs {
     float* ph_pos[6];
     float* ph_neg[6];
     const float* tab;  // [6][CPML_TABLES][Lmax]
+    const float* dummy;  // 16 zeroed floats: load target for cells outside every slab
+};
+
+// A launch covers up to 4 boxes [i0,i1) x [j0,j1) x [0,nz); work-group ids
+// are split over the boxes by prefix sums (first[b] .. first[b+1]).
+constexpr int MAX_BOXES = 4;
+struct Box {
+    int i0, i1, j0, j1;
+    int n_jb, n_ib;  // j-blocks, i-chunks of this box
 };
 
 struct YeeArgs {
@@ -55,13 +65,19 @@ struct YeeArgs {
     float* F;        // fields
     const float* Cf; // coefficients
     Tiles t;
-    int n_kb, n_jb, n_ib;  // blocks along k, j, i
+    int n_kb;              // blocks along k (every box spans all k)
+    int nbox;
+    Box box[MAX_BOXES];
+    int first[MAX_BOXES + 1];
     int xcd_remap;         // 1: make consecutive tiles share an XCD (L2)
 };
 
+// CPML code compiled into a launch: bit 0 x faces, bit 1 y faces, bit 2 z faces.
+enum { AX_X = 1, AX_Y = 2, AX_Z = 4 };
+
 // Kernel launchers (oe_yee.hip).
-hipError_t launch_update_h(const YeeArgs& a, const CpmlArgs* c, hipStream_t s);
-hipError_t launch_update_e(const YeeArgs& a, const CpmlArgs* c, hipStream_t s);
+hipError_t launch_update_h(const YeeArgs& a, const CpmlArgs* c, int axes, hipStream_t s);
+hipError_t launch_update_e(const YeeArgs& a, const CpmlArgs* c, int axes, hipStream_t s);
 
 // Auxiliary kernels (oe_aux.hip).
 hipError_t launch_fill(float* p, size_t n, float v, hipStream_t s);

@@ -20,6 +20,12 @@
 // per half-step; the j +- 1 and k +- 1 neighbours are re-reads that hit L1/L2.
 // Consecutive tiles are remapped onto one XCD so that the j-halo rows shared
 // by neighbouring work-groups stay in that XCD's L2.
+//
+// CPML: each cell is in at most one face slab per axis (faces of one axis do
+// not overlap), so the face is selected arithmetically and the psi loads are
+// issued unconditionally together with the bulk loads -- cells outside every
+// slab read a zeroed dummy line -- which keeps the psi latency hidden behind
+// the bulk loads instead of serialising it after them.
 
 #include "oe_internal.h"
 
@@ -63,10 +69,10 @@ __device__ __forceinline__ Vec<V> zerov() {
 }
 
 struct Tile {
-    int k0, j, i0, i1;
+    int k0, j, i0, i1, jend;
 };
 
-// Block id -> (k-block, j-block, i-chunk).  Hardware hands consecutive
+// Block id -> (box, k-block, j-block, i-chunk).  Hardware hands consecutive
 // work-group ids to the 8 XCDs round-robin; the remap gives XCD x the
 // contiguous logical range [x*per, (x+1)*per) so that j-neighbouring tiles
 // run on the same XCD at the same time.
@@ -77,30 +83,112 @@ __device__ __forceinline__ Tile decode(const YeeArgs& a) {
         const int per = gridDim.x >> 3;
         if (bid < (per << 3)) bid = (bid & 7) * per + (bid >> 3);
     }
+    int b = 0;
+#pragma unroll
+    for (int q = 1; q < MAX_BOXES; ++q)
+        if (q < a.nbox && bid >= a.first[q]) b = q;
+    const Box& B = a.box[b];
+    bid -= a.first[b];
     const int kb = bid % a.n_kb;
     const int t = bid / a.n_kb;
-    const int jb = t % a.n_jb;
-    const int ib = t / a.n_jb;
+    const int jb = t % B.n_jb;
+    const int ib = t / B.n_jb;
     const int lane = threadIdx.x;
     const int kx = lane % a.t.bx;
     const int jy = lane / a.t.bx;
     Tile r;
     r.k0 = (kb * a.t.bx + kx) * V;
-    r.j = jb * a.t.by + jy;
-    r.i0 = ib * a.t.ci;
-    r.i1 = min(r.i0 + a.t.ci, a.g.nx);
+    r.j = B.j0 + jb * a.t.by + jy;
+    r.i0 = B.i0 + ib * a.t.ci;
+    r.i1 = min(r.i0 + a.t.ci, B.i1);
+    r.jend = B.j1;
     return r;
 }
 
-__device__ __forceinline__ float tab(const CpmlArgs& c, int f, int t, int l) {
-    return c.tab[(f * CPML_TABLES + t) * c.Lmax + l];
+// Face of one axis holding global index g, or -1 (faces 2d, 2d+1).
+__device__ __forceinline__ int face_of(const CpmlArgs& c, int d, int g, int& l) {
+    const int f0 = 2 * d, f1 = 2 * d + 1;
+    if ((c.on_mask >> f0 & 1) && g < c.L[f0]) { l = g; return f0; }
+    if ((c.on_mask >> f1 & 1) && g >= c.g0[f1]) { l = g - c.g0[f1]; return f1; }
+    l = 0;
+    return -1;
 }
 
-template <int V, bool CP>
+// psi and table operands of one axis for the V cells of a lane.
+template <int V>
+struct Psi {
+    int f;
+    const float* p;  // psi_pos address (or dummy)
+    const float* n;  // psi_neg address (or dummy)
+    Vec<V> pp, pn;   // loaded values
+    Vec<V> b, a, km; // per-cell table values
+};
+
+// Loads psi of face f (2d or 2d+1, or -1 for the dummy line) and the three
+// tables; tables of x and y faces are uniform over the V cells, those of z
+// faces step with k.  Faces are chosen with selects (no dynamic indexing of
+// the argument arrays, which would spill them to scratch).
+template <int V, bool ZAXIS>
+__device__ __forceinline__ void psi_load(Psi<V>& s, const CpmlArgs& c, int d, int f, int l, size_t q,
+                                         float* const* pos, float* const* neg, int tb, int ta, int tk) {
+    s.f = f;
+    const bool on = f >= 0;
+    const bool lo = f == 2 * d;
+    const int fs = on ? f : 2 * d;
+    float* P = lo ? pos[2 * d] : pos[2 * d + 1];
+    float* Q = lo ? neg[2 * d] : neg[2 * d + 1];
+    s.p = on ? P + q : c.dummy;
+    s.n = on ? Q + q : c.dummy;
+    s.pp = ldv<V>(s.p);
+    s.pn = ldv<V>(s.n);
+    const float* T = c.tab + (size_t)fs * CPML_TABLES * c.Lmax;
+    const int lt = on ? l : 0;
+    if constexpr (ZAXIS) {
+        s.b = ldv<V>(T + tb * c.Lmax + lt);
+        s.a = ldv<V>(T + ta * c.Lmax + lt);
+        s.km = ldv<V>(T + tk * c.Lmax + lt);
+    } else {
+        const float b = T[tb * c.Lmax + lt], a = T[ta * c.Lmax + lt], km = T[tk * c.Lmax + lt];
+#pragma unroll
+        for (int e = 0; e < V; ++e) { s.b.v[e] = b; s.a.v[e] = a; s.km.v[e] = km; }
+    }
+}
+
+// psi <- b psi + a D for the pos and neg recursions, then (physical) the
+// correction of the Q-term component c1 (+psi) and P-term component c2 (-psi):
+//   c1 += C1 (psi_pos - km D_pos),  c2 += C2 (km D_neg - psi_neg)
+template <int V, bool PHYS>
+__device__ __forceinline__ void psi_apply(Psi<V>& s, const float* dpos, const float* dneg, Vec<V>& c1,
+                                          const Vec<V>& C1, Vec<V>& c2, const Vec<V>& C2, const bool* upd) {
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        s.pp.v[e] = s.b.v[e] * s.pp.v[e] + s.a.v[e] * dpos[e];
+        s.pn.v[e] = s.b.v[e] * s.pn.v[e] + s.a.v[e] * dneg[e];
+        if constexpr (PHYS) {
+            if (s.f >= 0 && upd[e]) {  // off-slab cells computed psi from the dummy line: no correction
+                c1.v[e] = c1.v[e] + C1.v[e] * (s.pp.v[e] - s.km.v[e] * dpos[e]);
+                c2.v[e] = c2.v[e] + C2.v[e] * (s.km.v[e] * dneg[e] - s.pn.v[e]);
+            }
+        }
+    }
+}
+
+template <int V>
+__device__ __forceinline__ void psi_store(const Psi<V>& s) {
+    if (s.f >= 0) {
+        stv<V>(const_cast<float*>(s.p), s.pp);
+        stv<V>(const_cast<float*>(s.n), s.pn);
+    }
+}
+
+// AX: CPML axes compiled in (AX_X | AX_Y | AX_Z); PHYS: apply the field
+// correction (else psi only, as the reference).
+template <int V, int AX, bool PHYS>
 __global__ __launch_bounds__(256) void k_update_h(YeeArgs a, CpmlArgs c) {
+    constexpr bool CP = AX != 0;
     const Tile t = decode<V>(a);
     const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
-    if (t.k0 >= nz || t.j >= ny) return;
+    if (t.k0 >= nz || t.j >= t.jend) return;
     const size_t N = a.g.N;
     const size_t plane = (size_t)ny * nz;
     const float* __restrict__ Ex = a.F;
@@ -119,9 +207,16 @@ __global__ __launch_bounds__(256) void k_update_h(YeeArgs a, CpmlArgs c) {
     const int j = t.j, k0 = t.k0;
     const bool hj = j + 1 < ny;
     const bool hk_last = k0 + V < nz;
+    int ly = 0, lz = 0;
+    int fy = -1, fz = -1;
+    if constexpr ((AX & AX_Y) != 0) fy = face_of(c, 1, j, ly);
+    if constexpr ((AX & AX_Z) != 0) fz = face_of(c, 2, k0, lz);
     size_t p = (size_t)t.i0 * plane + (size_t)j * nz + k0;
     Vec<V> ey_c = ldv<V>(Ey + p);
     Vec<V> ez_c = ldv<V>(Ez + p);
+    bool upd[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) upd[e] = true;
 
     for (int i = t.i0; i < t.i1; ++i, p += plane) {
         const bool hi = i + 1 < nx;
@@ -145,7 +240,21 @@ __global__ __launch_bounds__(256) void k_update_h(YeeArgs a, CpmlArgs c) {
         const Vec<V> a0 = ldv<V>(da0 + p), a1 = ldv<V>(da1 + p), a2 = ldv<V>(da2 + p);
         const Vec<V> b0 = ldv<V>(db0 + p), b1 = ldv<V>(db1 + p), b2 = ldv<V>(db2 + p);
 
-        // Differences kept for the CPML pass.
+        Psi<V> px, py, pz;
+        if constexpr ((AX & AX_X) != 0) {
+            int lx;
+            const int fx = face_of(c, 0, i, lx);
+            psi_load<V, false>(px, c, 0, fx, lx, ((size_t)lx * ny + j) * nz + k0, c.ph_pos, c.ph_neg, TAB_BH, TAB_AH, TAB_KH);
+        }
+        if constexpr ((AX & AX_Y) != 0) {
+            const int Ly = fy == 2 ? c.L[2] : (fy == 3 ? c.L[3] : 0);
+            psi_load<V, false>(py, c, 1, fy, ly, ((size_t)i * Ly + ly) * nz + k0, c.ph_pos, c.ph_neg, TAB_BH, TAB_AH, TAB_KH);
+        }
+        if constexpr ((AX & AX_Z) != 0) {
+            const int Lz = fz == 4 ? c.L[4] : (fz == 5 ? c.L[5] : 0);
+            psi_load<V, true>(pz, c, 2, fz, lz, ((size_t)i * ny + j) * Lz + lz, c.ph_pos, c.ph_neg, TAB_BH, TAB_AH, TAB_KH);
+        }
+
         float dEz_x[V], dEy_x[V], dEx_y[V], dEz_y[V], dEy_z[V], dEx_z[V];
 #pragma unroll
         for (int e = 0; e < V; ++e) {
@@ -169,72 +278,15 @@ __global__ __launch_bounds__(256) void k_update_h(YeeArgs a, CpmlArgs c) {
             dEy_z[e] = dey_dz; dEx_z[e] = dex_dz;
         }
 
-        if constexpr (CP) {
-            const bool phys = c.physical != 0;
-            // x faces (psi_pos <- dEz/dx for Hy, psi_neg <- dEy/dx for Hz)
-#pragma unroll
-            for (int f = 0; f < 2; ++f) {
-                if (!(c.on_mask & (1 << f))) continue;
-                const int l = i - c.g0[f];
-                if (l < 0 || l >= c.L[f]) continue;
-                const size_t q = ((size_t)l * ny + j) * nz + k0;
-                const float bb = tab(c, f, TAB_BH, l), aa = tab(c, f, TAB_AH, l), km = tab(c, f, TAB_KH, l);
-                Vec<V> pp = ldv<V>(c.ph_pos[f] + q), pn = ldv<V>(c.ph_neg[f] + q);
-#pragma unroll
-                for (int e = 0; e < V; ++e) {
-                    pp.v[e] = bb * pp.v[e] + aa * dEz_x[e];
-                    pn.v[e] = bb * pn.v[e] + aa * dEy_x[e];
-                    if (phys) {
-                        hy.v[e] = hy.v[e] + b1.v[e] * (pp.v[e] - km * dEz_x[e]);
-                        hz.v[e] = hz.v[e] + b2.v[e] * (km * dEy_x[e] - pn.v[e]);
-                    }
-                }
-                stv<V>(c.ph_pos[f] + q, pp);
-                stv<V>(c.ph_neg[f] + q, pn);
-            }
-            // y faces (psi_pos <- dEx/dy for Hz, psi_neg <- dEz/dy for Hx)
-#pragma unroll
-            for (int f = 2; f < 4; ++f) {
-                if (!(c.on_mask & (1 << f))) continue;
-                const int l = j - c.g0[f];
-                if (l < 0 || l >= c.L[f]) continue;
-                const size_t q = ((size_t)i * c.L[f] + l) * nz + k0;
-                const float bb = tab(c, f, TAB_BH, l), aa = tab(c, f, TAB_AH, l), km = tab(c, f, TAB_KH, l);
-                Vec<V> pp = ldv<V>(c.ph_pos[f] + q), pn = ldv<V>(c.ph_neg[f] + q);
-#pragma unroll
-                for (int e = 0; e < V; ++e) {
-                    pp.v[e] = bb * pp.v[e] + aa * dEx_y[e];
-                    pn.v[e] = bb * pn.v[e] + aa * dEz_y[e];
-                    if (phys) {
-                        hz.v[e] = hz.v[e] + b2.v[e] * (pp.v[e] - km * dEx_y[e]);
-                        hx.v[e] = hx.v[e] + b0.v[e] * (km * dEz_y[e] - pn.v[e]);
-                    }
-                }
-                stv<V>(c.ph_pos[f] + q, pp);
-                stv<V>(c.ph_neg[f] + q, pn);
-            }
-            // z faces (psi_pos <- dEy/dz for Hx, psi_neg <- dEx/dz for Hy)
-#pragma unroll
-            for (int f = 4; f < 6; ++f) {
-                if (!(c.on_mask & (1 << f))) continue;
-                const int L = c.L[f];
-#pragma unroll
-                for (int e = 0; e < V; ++e) {
-                    const int l = k0 + e - c.g0[f];
-                    if (l < 0 || l >= L) continue;
-                    const size_t q = ((size_t)i * ny + j) * L + l;
-                    const float bb = tab(c, f, TAB_BH, l), aa = tab(c, f, TAB_AH, l), km = tab(c, f, TAB_KH, l);
-                    const float pp = bb * c.ph_pos[f][q] + aa * dEy_z[e];
-                    const float pn = bb * c.ph_neg[f][q] + aa * dEx_z[e];
-                    c.ph_pos[f][q] = pp;
-                    c.ph_neg[f][q] = pn;
-                    if (phys) {
-                        hx.v[e] = hx.v[e] + b0.v[e] * (pp - km * dEy_z[e]);
-                        hy.v[e] = hy.v[e] + b1.v[e] * (km * dEx_z[e] - pn);
-                    }
-                }
-            }
-        }
+        // x: psi_pos <- dEz/dx (Hy, Q-term), psi_neg <- dEy/dx (Hz, P-term)
+        if constexpr ((AX & AX_X) != 0) psi_apply<V, PHYS>(px, dEz_x, dEy_x, hy, b1, hz, b2, upd);
+        // y: psi_pos <- dEx/dy (Hz), psi_neg <- dEz/dy (Hx)
+        if constexpr ((AX & AX_Y) != 0) psi_apply<V, PHYS>(py, dEx_y, dEz_y, hz, b2, hx, b0, upd);
+        // z: psi_pos <- dEy/dz (Hx), psi_neg <- dEx/dz (Hy)
+        if constexpr ((AX & AX_Z) != 0) psi_apply<V, PHYS>(pz, dEy_z, dEx_z, hx, b0, hy, b1, upd);
+        if constexpr ((AX & AX_X) != 0) psi_store<V>(px);
+        if constexpr ((AX & AX_Y) != 0) psi_store<V>(py);
+        if constexpr ((AX & AX_Z) != 0) psi_store<V>(pz);
 
         stv<V>(Hx + p, hx);
         stv<V>(Hy + p, hy);
@@ -244,11 +296,12 @@ __global__ __launch_bounds__(256) void k_update_h(YeeArgs a, CpmlArgs c) {
     }
 }
 
-template <int V, bool CP>
+template <int V, int AX, bool PHYS>
 __global__ __launch_bounds__(256) void k_update_e(YeeArgs a, CpmlArgs c) {
+    constexpr bool CP = AX != 0;
     const Tile t = decode<V>(a);
     const int ny = a.g.ny, nz = a.g.nz;
-    if (t.k0 >= nz || t.j >= ny) return;
+    if (t.k0 >= nz || t.j >= t.jend) return;
     const size_t N = a.g.N;
     const size_t plane = (size_t)ny * nz;
     float* __restrict__ Ex = a.F;
@@ -267,6 +320,10 @@ __global__ __launch_bounds__(256) void k_update_e(YeeArgs a, CpmlArgs c) {
     const int j = t.j, k0 = t.k0;
     const bool gj = j > 0;
     const bool gk_first = k0 > 0;
+    int ly = 0, lz = 0;
+    int fy = -1, fz = -1;
+    if constexpr ((AX & AX_Y) != 0) fy = face_of(c, 1, j, ly);
+    if constexpr ((AX & AX_Z) != 0) fz = face_of(c, 2, k0, lz);
     size_t p = (size_t)t.i0 * plane + (size_t)j * nz + k0;
     Vec<V> hy_p = zerov<V>(), hz_p = zerov<V>();
     if (t.i0 > 0) {
@@ -276,6 +333,7 @@ __global__ __launch_bounds__(256) void k_update_e(YeeArgs a, CpmlArgs c) {
 
     for (int i = t.i0; i < t.i1; ++i, p += plane) {
         const bool gi = i > 0;
+        const bool row = gi && gj;
         const Vec<V> hx_c = ldv<V>(Hx + p), hy_c = ldv<V>(Hy + p), hz_c = ldv<V>(Hz + p);
         Vec<V> hz_j = hz_c, hx_j = hx_c;
         if (gj) {
@@ -287,10 +345,34 @@ __global__ __launch_bounds__(256) void k_update_e(YeeArgs a, CpmlArgs c) {
             hy_kf = Hy[p - 1];
             hx_kf = Hx[p - 1];
         }
+        Vec<V> ex, ey, ez, c0, c1, c2, s0, s1, s2;
+        if (row) {
+            ex = ldv<V>(Ex + p); ey = ldv<V>(Ey + p); ez = ldv<V>(Ez + p);
+            s0 = ldv<V>(ca0 + p); s1 = ldv<V>(ca1 + p); s2 = ldv<V>(ca2 + p);
+            c0 = ldv<V>(cb0 + p); c1 = ldv<V>(cb1 + p); c2 = ldv<V>(cb2 + p);
+        }
+
+        Psi<V> px, py, pz;
+        if constexpr ((AX & AX_X) != 0) {
+            int lx;
+            const int fx = face_of(c, 0, i, lx);
+            psi_load<V, false>(px, c, 0, fx, lx, ((size_t)lx * ny + j) * nz + k0, c.pe_pos, c.pe_neg, TAB_BE, TAB_AE, TAB_KE);
+        }
+        if constexpr ((AX & AX_Y) != 0) {
+            const int Ly = fy == 2 ? c.L[2] : (fy == 3 ? c.L[3] : 0);
+            psi_load<V, false>(py, c, 1, fy, ly, ((size_t)i * Ly + ly) * nz + k0, c.pe_pos, c.pe_neg, TAB_BE, TAB_AE, TAB_KE);
+        }
+        if constexpr ((AX & AX_Z) != 0) {
+            const int Lz = fz == 4 ? c.L[4] : (fz == 5 ? c.L[5] : 0);
+            psi_load<V, true>(pz, c, 2, fz, lz, ((size_t)i * ny + j) * Lz + lz, c.pe_pos, c.pe_neg, TAB_BE, TAB_AE, TAB_KE);
+        }
+
         float dHz_y[V], dHy_z[V], dHx_z[V], dHz_x[V], dHy_x[V], dHx_y[V];
+        bool upd[V];
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             const bool gk = (e > 0) || gk_first;
+            upd[e] = row && gk;
             const float hyk = (e > 0) ? hy_c.v[(e > 0) ? e - 1 : e] : hy_kf;
             const float hxk = (e > 0) ? hx_c.v[(e > 0) ? e - 1 : e] : hx_kf;
             dHz_y[e] = gj ? hz_c.v[e] - hz_j.v[e] : 0.f;
@@ -301,15 +383,10 @@ __global__ __launch_bounds__(256) void k_update_e(YeeArgs a, CpmlArgs c) {
             dHx_y[e] = gj ? hx_c.v[e] - hx_j.v[e] : 0.f;
         }
 
-        const bool row = gi && gj;
-        Vec<V> ex, ey, ez, c0, c1, c2;
         if (row) {
-            ex = ldv<V>(Ex + p); ey = ldv<V>(Ey + p); ez = ldv<V>(Ez + p);
-            const Vec<V> s0 = ldv<V>(ca0 + p), s1 = ldv<V>(ca1 + p), s2 = ldv<V>(ca2 + p);
-            c0 = ldv<V>(cb0 + p); c1 = ldv<V>(cb1 + p); c2 = ldv<V>(cb2 + p);
 #pragma unroll
             for (int e = 0; e < V; ++e) {
-                if (!((e > 0) || gk_first)) continue;  // k == 0: not written
+                if (!upd[e]) continue;  // k == 0: not written
                 const float cx = dHz_y[e] - dHy_z[e];
                 ex.v[e] = s0.v[e] * ex.v[e] + c0.v[e] * cx;
                 const float cy = dHx_z[e] - dHz_x[e];
@@ -319,72 +396,15 @@ __global__ __launch_bounds__(256) void k_update_e(YeeArgs a, CpmlArgs c) {
             }
         }
 
-        if constexpr (CP) {
-            const bool phys = (c.physical != 0) && row;
-            // x faces (psi_pos <- dHz/dx for Ey, psi_neg <- dHy/dx for Ez)
-#pragma unroll
-            for (int f = 0; f < 2; ++f) {
-                if (!(c.on_mask & (1 << f))) continue;
-                const int l = i - c.g0[f];
-                if (l < 0 || l >= c.L[f]) continue;
-                const size_t q = ((size_t)l * ny + j) * nz + k0;
-                const float bb = tab(c, f, TAB_BE, l), aa = tab(c, f, TAB_AE, l), km = tab(c, f, TAB_KE, l);
-                Vec<V> pp = ldv<V>(c.pe_pos[f] + q), pn = ldv<V>(c.pe_neg[f] + q);
-#pragma unroll
-                for (int e = 0; e < V; ++e) {
-                    pp.v[e] = bb * pp.v[e] + aa * dHz_x[e];
-                    pn.v[e] = bb * pn.v[e] + aa * dHy_x[e];
-                    if (phys && ((e > 0) || gk_first)) {
-                        ey.v[e] = ey.v[e] + c1.v[e] * (pp.v[e] - km * dHz_x[e]);
-                        ez.v[e] = ez.v[e] + c2.v[e] * (km * dHy_x[e] - pn.v[e]);
-                    }
-                }
-                stv<V>(c.pe_pos[f] + q, pp);
-                stv<V>(c.pe_neg[f] + q, pn);
-            }
-            // y faces (psi_pos <- dHx/dy for Ez, psi_neg <- dHz/dy for Ex)
-#pragma unroll
-            for (int f = 2; f < 4; ++f) {
-                if (!(c.on_mask & (1 << f))) continue;
-                const int l = j - c.g0[f];
-                if (l < 0 || l >= c.L[f]) continue;
-                const size_t q = ((size_t)i * c.L[f] + l) * nz + k0;
-                const float bb = tab(c, f, TAB_BE, l), aa = tab(c, f, TAB_AE, l), km = tab(c, f, TAB_KE, l);
-                Vec<V> pp = ldv<V>(c.pe_pos[f] + q), pn = ldv<V>(c.pe_neg[f] + q);
-#pragma unroll
-                for (int e = 0; e < V; ++e) {
-                    pp.v[e] = bb * pp.v[e] + aa * dHx_y[e];
-                    pn.v[e] = bb * pn.v[e] + aa * dHz_y[e];
-                    if (phys && ((e > 0) || gk_first)) {
-                        ez.v[e] = ez.v[e] + c2.v[e] * (pp.v[e] - km * dHx_y[e]);
-                        ex.v[e] = ex.v[e] + c0.v[e] * (km * dHz_y[e] - pn.v[e]);
-                    }
-                }
-                stv<V>(c.pe_pos[f] + q, pp);
-                stv<V>(c.pe_neg[f] + q, pn);
-            }
-            // z faces (psi_pos <- dHy/dz for Ex, psi_neg <- dHx/dz for Ey)
-#pragma unroll
-            for (int f = 4; f < 6; ++f) {
-                if (!(c.on_mask & (1 << f))) continue;
-                const int L = c.L[f];
-#pragma unroll
-                for (int e = 0; e < V; ++e) {
-                    const int l = k0 + e - c.g0[f];
-                    if (l < 0 || l >= L) continue;
-                    const size_t q = ((size_t)i * ny + j) * L + l;
-                    const float bb = tab(c, f, TAB_BE, l), aa = tab(c, f, TAB_AE, l), km = tab(c, f, TAB_KE, l);
-                    const float pp = bb * c.pe_pos[f][q] + aa * dHy_z[e];
-                    const float pn = bb * c.pe_neg[f][q] + aa * dHx_z[e];
-                    c.pe_pos[f][q] = pp;
-                    c.pe_neg[f][q] = pn;
-                    if (phys && ((e > 0) || gk_first)) {
-                        ex.v[e] = ex.v[e] + c0.v[e] * (pp - km * dHy_z[e]);
-                        ey.v[e] = ey.v[e] + c1.v[e] * (km * dHx_z[e] - pn);
-                    }
-                }
-            }
-        }
+        // x: psi_pos <- dHz/dx (Ey, Q-term), psi_neg <- dHy/dx (Ez, P-term)
+        if constexpr ((AX & AX_X) != 0) psi_apply<V, PHYS>(px, dHz_x, dHy_x, ey, c1, ez, c2, upd);
+        // y: psi_pos <- dHx/dy (Ez), psi_neg <- dHz/dy (Ex)
+        if constexpr ((AX & AX_Y) != 0) psi_apply<V, PHYS>(py, dHx_y, dHz_y, ez, c2, ex, c0, upd);
+        // z: psi_pos <- dHy/dz (Ex), psi_neg <- dHx/dz (Ey)
+        if constexpr ((AX & AX_Z) != 0) psi_apply<V, PHYS>(pz, dHy_z, dHx_z, ex, c0, ey, c1, upd);
+        if constexpr ((AX & AX_X) != 0) psi_store<V>(px);
+        if constexpr ((AX & AX_Y) != 0) psi_store<V>(py);
+        if constexpr ((AX & AX_Z) != 0) psi_store<V>(pz);
 
         if (row) {
             stv<V>(Ex + p, ex);
@@ -396,32 +416,41 @@ __global__ __launch_bounds__(256) void k_update_e(YeeArgs a, CpmlArgs c) {
     }
 }
 
-template <int V, bool CP>
+template <int V, int AX, bool PHYS>
 hipError_t launch_h_t(const YeeArgs& a, const CpmlArgs& c, hipStream_t s) {
-    const unsigned nb = (unsigned)a.n_kb * a.n_jb * a.n_ib;
-    hipLaunchKernelGGL((k_update_h<V, CP>), dim3(nb), dim3(256), 0, s, a, c);
+    hipLaunchKernelGGL((k_update_h<V, AX, PHYS>), dim3(a.first[a.nbox]), dim3(256), 0, s, a, c);
     return hipGetLastError();
 }
 
-template <int V, bool CP>
+template <int V, int AX, bool PHYS>
 hipError_t launch_e_t(const YeeArgs& a, const CpmlArgs& c, hipStream_t s) {
-    const unsigned nb = (unsigned)a.n_kb * a.n_jb * a.n_ib;
-    hipLaunchKernelGGL((k_update_e<V, CP>), dim3(nb), dim3(256), 0, s, a, c);
+    hipLaunchKernelGGL((k_update_e<V, AX, PHYS>), dim3(a.first[a.nbox]), dim3(256), 0, s, a, c);
     return hipGetLastError();
+}
+
+// Instantiated variants: plain (no CPML), z faces only (interior of a CPML
+// grid) and all faces (the x/y slab boxes), each in compat and physical mode.
+template <bool H, int V>
+hipError_t dispatch(const YeeArgs& a, const CpmlArgs* c, int axes, hipStream_t s) {
+    const CpmlArgs none{};
+    if (!c || axes == 0) return H ? launch_h_t<V, 0, false>(a, none, s) : launch_e_t<V, 0, false>(a, none, s);
+    if (axes == AX_Z) {
+        if (c->physical) return H ? launch_h_t<V, AX_Z, true>(a, *c, s) : launch_e_t<V, AX_Z, true>(a, *c, s);
+        return H ? launch_h_t<V, AX_Z, false>(a, *c, s) : launch_e_t<V, AX_Z, false>(a, *c, s);
+    }
+    constexpr int ALL = AX_X | AX_Y | AX_Z;
+    if (c->physical) return H ? launch_h_t<V, ALL, true>(a, *c, s) : launch_e_t<V, ALL, true>(a, *c, s);
+    return H ? launch_h_t<V, ALL, false>(a, *c, s) : launch_e_t<V, ALL, false>(a, *c, s);
 }
 
 }  // namespace
 
-hipError_t launch_update_h(const YeeArgs& a, const CpmlArgs* c, hipStream_t s) {
-    CpmlArgs none{};
-    if (a.t.vec == 4) return c ? launch_h_t<4, true>(a, *c, s) : launch_h_t<4, false>(a, none, s);
-    return c ? launch_h_t<1, true>(a, *c, s) : launch_h_t<1, false>(a, none, s);
+hipError_t launch_update_h(const YeeArgs& a, const CpmlArgs* c, int axes, hipStream_t s) {
+    return a.t.vec == 4 ? dispatch<true, 4>(a, c, axes, s) : dispatch<true, 1>(a, c, axes, s);
 }
 
-hipError_t launch_update_e(const YeeArgs& a, const CpmlArgs* c, hipStream_t s) {
-    CpmlArgs none{};
-    if (a.t.vec == 4) return c ? launch_e_t<4, true>(a, *c, s) : launch_e_t<4, false>(a, none, s);
-    return c ? launch_e_t<1, true>(a, *c, s) : launch_e_t<1, false>(a, none, s);
+hipError_t launch_update_e(const YeeArgs& a, const CpmlArgs* c, int axes, hipStream_t s) {
+    return a.t.vec == 4 ? dispatch<false, 4>(a, c, axes, s) : dispatch<false, 1>(a, c, axes, s);
 }
 
 }  // namespace oe

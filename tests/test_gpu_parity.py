@@ -245,3 +245,22 @@ def test_coefficients_with_materials_match_oracle():
     sim.set_pec((30, 34, 30, 34, 0, 64))
     eng = HipEngine(op)
     assert np.array_equal(eng.coefficients(), sim.coefficients())
+
+
+def test_extension_outliving_engine_is_detached():
+    """Regression: destroying the engine before its extensions used to leave
+    them pointing at freed memory; they are now detached and report an error."""
+    from openems_amd import CpmlBoundaries, OEError
+
+    op = Operator.uniform(16, 16, 16, 1e-3)
+    eng = HipEngine(op)
+    cp = eng.cpml(CpmlBoundaries(layers=(4,) * 6))
+    pr = eng.field_probe((8, 8, 8))
+    eng.run_batch(EngineBatch(num_steps=5, extensions=[cp, pr]))
+    assert pr.data().shape == (5, 6)
+    eng.close()
+    with pytest.raises(OEError):
+        pr.data()
+    with pytest.raises(OEError):
+        cp.reset()
+    del cp, pr  # oe_extension_destroy on detached handles
