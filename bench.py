@@ -49,11 +49,67 @@ def slab_cells(shape, layers):
 
 
 def kernel_bytes(shape, layers):
-    """Algorithmic HBM bytes of one H (or E) launch: 12 float reads + 3 writes
-    per cell (own 3 components, the other field's 3 components, 6
-    coefficients), plus 2 psi read + 2 psi write per slab cell per direction."""
+    """Algorithmic HBM bytes of one H (or E) half-step over the whole grid: 12
+    float reads + 3 writes per cell (own 3 components, the other field's 3
+    components, 6 coefficients), plus 2 psi reads + 2 psi writes per slab cell
+    per direction."""
     n = shape[0] * shape[1] * shape[2]
     return 60 * n + 16 * slab_cells(shape, layers)
+
+
+def launch_bytes(name, shape, layers):
+    """Algorithmic bytes of one launch of the plan: 'grid' (no CPML), or
+    'interior' (x/y slabs excluded; z-face psi at the row ends) and 'slabs'
+    (the x/y face slabs with all faces) when a CPML is attached."""
+    nx, ny, nz = shape
+    part = name.split(":")[1]
+    if part == "grid" or not layers:
+        return kernel_bytes(shape, 0)
+    ix, jy = nx - 2 * layers, ny - 2 * layers
+    interior = 60 * ix * jy * nz + 16 * ix * jy * 2 * layers
+    if part == "interior":
+        return interior
+    return kernel_bytes(shape, layers) - interior
+
+
+def summarize_rocprof(paths):
+    """Markdown summary of rocprofv3 result databases (kernel statistics and
+    per-dispatch counters).  Only the standard library is used."""
+    import sqlite3
+
+    out = []
+    for path in paths:
+        con = sqlite3.connect(path)
+        views = {r[0] for r in con.execute("select name from sqlite_master")}
+        out.append(f"### `{path}`\n")
+        if "top_kernels" in views:
+            rows = con.execute("select name, total_calls, total_duration, average, percentage from top_kernels").fetchall()
+            if rows:
+                out.append("| kernel | calls | total us | mean us | % |\n|---|---|---|---|---|")
+                for r in rows[:12]:
+                    out.append(f"| `{r[0][:90]}` | {r[1]} | {r[2]:.1f} | {r[3]:.2f} | {r[4]:.2f} |")
+                out.append("")
+        if "counters_collection" in views:
+            rows = con.execute("select kernel_name, counter_name, count(*), avg(value), min(value), max(value) "
+                               "from counters_collection group by kernel_name, counter_name "
+                               "order by avg(value) desc").fetchall()
+            if rows:
+                out.append("| kernel | counter | dispatches | mean KiB | min | max |\n|---|---|---|---|---|---|")
+                for r in rows[:10]:
+                    out.append(f"| `{r[0][:90]}` | {r[1]} | {r[2]} | {r[3]:.0f} | {r[4]:.0f} | {r[5]:.0f} |")
+                out.append("")
+    return "\n".join(out)
+
+
+def measured_traffic(kernel_key):
+    """HBM bytes per launch from the committed counter summary, if present."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get(kernel_key)
+    except (OSError, ValueError):
+        return None
 
 
 def build_engine(wl, mode):
@@ -119,9 +175,30 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="grid edge of the CPML/boxes workload")
     ap.add_argument("--mode", choices=["physical", "compat"], default="physical")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--prof-period", type=int, default=8,
+                    help="HIP events around every launch of every n-th timed step (1 = all)")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--tiles", type=str, default="", help="bx,by,ci,vec override")
+    ap.add_argument("--summarize-rocprof", nargs="+", metavar="DB",
+                    help="print a markdown summary of rocprofv3 result databases and exit")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 10 device copies of 1 GiB (torch copy_), to scale HBM byte counters")
     args = ap.parse_args()
+
+    if args.summarize_rocprof:
+        print(summarize_rocprof(args.summarize_rocprof))
+        return
+
+    if args.copy_calibration:
+        import torch
+
+        a = torch.ones(1 << 28, device="cuda", dtype=torch.float32)
+        b = torch.empty_like(a)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("[bench] copy calibration: 10 copies of 1 GiB (read 1 GiB + write 1 GiB each)")
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -145,27 +222,31 @@ def main():
     if args.warmup:
         run_steps(eng, exts, wl, dt, 0, args.warmup)
 
-    eng.set_profiling(True)
+    eng.set_profiling(args.prof_period)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     res = run_steps(eng, exts, wl, dt, args.warmup, args.steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    eng.set_profiling(False)
+    eng.set_profiling(0)
     assert res.timesteps_executed == args.steps
     log(f"[bench] timed {args.steps} steps in {wall:.3f} s")
 
     kern = {}
-    for name in ("update_h", "update_e", "point_ops", "energy"):
+    for name in ("update_h:grid", "update_e:grid", "update_h:interior", "update_e:interior", "update_h:slabs",
+                 "update_e:slabs", "point_ops", "energy"):
         n, ms = eng.kernel_time(name)
         if n:
             kern[name] = (n, ms / n)
-    dom = max(("update_h", "update_e"), key=lambda k: kern[k][0] * kern[k][1])
+    yee = [k for k in kern if k.startswith("update_")]
+    dom = max(yee, key=lambda k: kern[k][0] * kern[k][1])
     avg_ms = kern[dom][1]
-    kb = kernel_bytes(wl.shape, wl.cpml_layers)
+    kb = launch_bytes(dom, wl.shape, wl.cpml_layers)
     achieved = kb / (avg_ms * 1e-3) / 1e9
     for k, (n, ms) in kern.items():
         log(f"[bench] {k}: {n} launches, mean {ms * 1e3:.1f} us")
+    dom_key = f"{dom}@{wl.name}{wl.shape[0]}"
+    traffic = measured_traffic(dom_key)
     E, H = eng.read_fields()
     import numpy as np
 
@@ -178,7 +259,7 @@ def main():
         log(f"[bench] CPU baseline {cpu['value']:.1f} MCells/s")
 
     mcells = wl.cells * args.steps / wall / 1e6
-    step_bytes = 2 * kb
+    step_bytes = 2 * kernel_bytes(wl.shape, wl.cpml_layers)
     out = {
         "metric": METRIC,
         "value": round(mcells, 1),
@@ -202,12 +283,13 @@ def main():
         "hbm_frac_step": round(step_bytes * args.steps / wall / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {
             "bound": "hbm",
-            "kernel": dom,
+            "kernel": {"update_h:interior": "k_update_h<4,AX_Z,phys> (interior box, z-face CPML)",
+                       "update_e:interior": "k_update_e<4,AX_Z,phys> (interior box, z-face CPML)"}.get(dom, dom),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "bytes_per_launch": kb,
             "mean_launch_us": round(avg_ms * 1e3, 2),
         },

@@ -193,13 +193,17 @@ typedef struct {
  *   termination test. */
 oe_status oe_engine_run_batch(oe_engine* eng, const oe_batch_desc* batch, oe_batch_result* result);
 
-/* Kernel timing of the last run_batch, measured with HIP events on the
- * engine's stream.  names: "update_h", "update_e", ...; returns the number
- * of launches and total milliseconds for that kernel. */
+/* Kernel timing accumulated while profiling is on, measured with HIP events
+ * on the engine's stream around every launch.  Names: "update_h:grid" and
+ * "update_e:grid" (no CPML), or "update_h:interior" / "update_h:slabs" and
+ * the "update_e:" pair (CPML: interior box with z-face CPML, x/y face slabs
+ * with all faces), "mur_pre", "mur_post", "ade_pre", "ade_post",
+ * "point_ops", "energy".  Returns the launch count and total milliseconds. */
 oe_status oe_engine_kernel_time(oe_engine* eng, const char* name, uint64_t* launches, double* total_ms);
 
-/* Enable/disable per-launch event timing (off by default). */
-oe_status oe_engine_set_profiling(oe_engine* eng, int on);
+/* Per-launch event timing: 0 off (default); n > 0 records HIP events around
+ * every launch of every n-th step of a batch (n = 1: every step). */
+oe_status oe_engine_set_profiling(oe_engine* eng, int period);
 
 /* Tile shape of the Yee kernels: bx lanes along z (in vector units), by rows
  * along y, ci planes along x per work-group; 0 keeps the automatic choice.

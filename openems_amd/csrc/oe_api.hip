@@ -133,8 +133,9 @@ struct oe_engine {
     DevBuf exc_samples;
     DevBuf exc_table;
     DevBuf probe_table;
-    // profiling
+    // profiling: 0 off, n > 0 times every launch of every n-th step
     int profiling = 0;
+    bool prof_step = false;
     std::map<std::string, KernelStat> stats;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::vector<hipEvent_t> event_pool;
@@ -255,14 +256,14 @@ struct Timed {
     const char* name;
     hipEvent_t a = nullptr, b = nullptr;
     Timed(oe_engine* eng, const char* n) : e(eng), name(n) {
-        if (e->profiling) {
+        if (e->profiling && e->prof_step) {
             a = take_event(e);
             b = take_event(e);
             if (a) (void)hipEventRecord(a, e->stream);
         }
     }
     ~Timed() {
-        if (e->profiling && a && b) {
+        if (a && b) {
             (void)hipEventRecord(b, e->stream);
             e->pending.push_back({name, {a, b}});
         }
@@ -524,7 +525,7 @@ oe_status oe_engine_energy(oe_engine* e, double* ee, double* he) {
 
 oe_status oe_engine_set_profiling(oe_engine* e, int on) {
     if (!e) return fail(OE_ERR_CONFIG, "null argument");
-    e->profiling = on ? 1 : 0;
+    e->profiling = on > 0 ? on : 0;
     return OE_OK;
 }
 
@@ -1043,9 +1044,10 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
     double final_db = 0.0;
     for (uint64_t st = 0; st < num_steps; ++st) {
         const uint64_t cur = e->timestep;
-        {
-            Timed tm(e, "update_h");
-            for (const Launch& L : plan) HIP_OK(launch_update_h(L.a, cp, L.axes, e->stream));
+        e->prof_step = e->profiling > 0 && (st % (uint64_t)e->profiling) == 0;
+        for (const Launch& L : plan) {
+            Timed tm(e, L.axes == (AX_X | AX_Y | AX_Z) ? "update_h:slabs" : (cp ? "update_h:interior" : "update_h:grid"));
+            HIP_OK(launch_update_h(L.a, cp, L.axes, e->stream));
         }
         for (auto* x : murs) {
             Timed tm(e, "mur_pre");
@@ -1056,9 +1058,9 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
                 Timed tm(e, "ade_pre");
                 HIP_OK(launch_ade_pre(e->F, e->g.N, x->ade, e->stream));
             }
-        {
-            Timed tm(e, "update_e");
-            for (const Launch& L : plan) HIP_OK(launch_update_e(L.a, cp, L.axes, e->stream));
+        for (const Launch& L : plan) {
+            Timed tm(e, L.axes == (AX_X | AX_Y | AX_Z) ? "update_e:slabs" : (cp ? "update_e:interior" : "update_e:grid"));
+            HIP_OK(launch_update_e(L.a, cp, L.axes, e->stream));
         }
         for (auto* x : ades) {
             Timed tm(e, "ade_post");

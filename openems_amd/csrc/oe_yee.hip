@@ -52,6 +52,53 @@ __device__ __forceinline__ Vec<V> ldv(const float* __restrict__ p) {
 }
 
 template <int V>
+__device__ __forceinline__ void stv(float* __restrict__ p, const Vec<V>& a);
+
+// Streaming (nontemporal) accesses for data without reuse inside a launch:
+// the coefficient arrays and the component being updated (load and store).
+// On gfx950 the nt hint keeps these streams from displacing the neighbour
+// lines (the other field's j/k halos) in L2/MALL.  Measured on the 256^3
+// CPML workload: 4.7 -> 5.7 TB/s for the H update, 4.8 -> 5.8 TB/s for E.
+#ifndef OE_NT_COEF
+#define OE_NT_COEF 1
+#endif
+#ifndef OE_NT_OWN
+#define OE_NT_OWN 1
+#endif
+#ifndef OE_NT_STORE
+#define OE_NT_STORE 1
+#endif
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <int V, bool NT>
+__device__ __forceinline__ Vec<V> ldn(const float* __restrict__ p) {
+    if constexpr (!NT) {
+        return ldv<V>(p);
+    } else {
+        Vec<V> r;
+        if constexpr (V == 4) {
+            const f4v t = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p));
+            r.v[0] = t.x; r.v[1] = t.y; r.v[2] = t.z; r.v[3] = t.w;
+        } else {
+            r.v[0] = __builtin_nontemporal_load(p);
+        }
+        return r;
+    }
+}
+
+template <int V, bool NT>
+__device__ __forceinline__ void stn(float* __restrict__ p, const Vec<V>& a) {
+    if constexpr (!NT) {
+        stv<V>(p, a);
+    } else if constexpr (V == 4) {
+        f4v t = {a.v[0], a.v[1], a.v[2], a.v[3]};
+        __builtin_nontemporal_store(t, reinterpret_cast<f4v*>(p));
+    } else {
+        __builtin_nontemporal_store(a.v[0], p);
+    }
+}
+
+template <int V>
 __device__ __forceinline__ void stv(float* __restrict__ p, const Vec<V>& a) {
     if constexpr (V == 4) {
         *reinterpret_cast<float4*>(p) = make_float4(a.v[0], a.v[1], a.v[2], a.v[3]);
@@ -236,9 +283,9 @@ __global__ __launch_bounds__(256) void k_update_h(YeeArgs a, CpmlArgs c) {
             ey_kl = Ey[p + V];
             ex_kl = Ex[p + V];
         }
-        Vec<V> hx = ldv<V>(Hx + p), hy = ldv<V>(Hy + p), hz = ldv<V>(Hz + p);
-        const Vec<V> a0 = ldv<V>(da0 + p), a1 = ldv<V>(da1 + p), a2 = ldv<V>(da2 + p);
-        const Vec<V> b0 = ldv<V>(db0 + p), b1 = ldv<V>(db1 + p), b2 = ldv<V>(db2 + p);
+        Vec<V> hx = ldn<V, OE_NT_OWN>(Hx + p), hy = ldn<V, OE_NT_OWN>(Hy + p), hz = ldn<V, OE_NT_OWN>(Hz + p);
+        const Vec<V> a0 = ldn<V, OE_NT_COEF>(da0 + p), a1 = ldn<V, OE_NT_COEF>(da1 + p), a2 = ldn<V, OE_NT_COEF>(da2 + p);
+        const Vec<V> b0 = ldn<V, OE_NT_COEF>(db0 + p), b1 = ldn<V, OE_NT_COEF>(db1 + p), b2 = ldn<V, OE_NT_COEF>(db2 + p);
 
         Psi<V> px, py, pz;
         if constexpr ((AX & AX_X) != 0) {
@@ -288,9 +335,9 @@ __global__ __launch_bounds__(256) void k_update_h(YeeArgs a, CpmlArgs c) {
         if constexpr ((AX & AX_Y) != 0) psi_store<V>(py);
         if constexpr ((AX & AX_Z) != 0) psi_store<V>(pz);
 
-        stv<V>(Hx + p, hx);
-        stv<V>(Hy + p, hy);
-        stv<V>(Hz + p, hz);
+        stn<V, OE_NT_STORE>(Hx + p, hx);
+        stn<V, OE_NT_STORE>(Hy + p, hy);
+        stn<V, OE_NT_STORE>(Hz + p, hz);
         ey_c = ey_n;
         ez_c = ez_n;
     }
@@ -347,9 +394,9 @@ __global__ __launch_bounds__(256) void k_update_e(YeeArgs a, CpmlArgs c) {
         }
         Vec<V> ex, ey, ez, c0, c1, c2, s0, s1, s2;
         if (row) {
-            ex = ldv<V>(Ex + p); ey = ldv<V>(Ey + p); ez = ldv<V>(Ez + p);
-            s0 = ldv<V>(ca0 + p); s1 = ldv<V>(ca1 + p); s2 = ldv<V>(ca2 + p);
-            c0 = ldv<V>(cb0 + p); c1 = ldv<V>(cb1 + p); c2 = ldv<V>(cb2 + p);
+            ex = ldn<V, OE_NT_OWN>(Ex + p); ey = ldn<V, OE_NT_OWN>(Ey + p); ez = ldn<V, OE_NT_OWN>(Ez + p);
+            s0 = ldn<V, OE_NT_COEF>(ca0 + p); s1 = ldn<V, OE_NT_COEF>(ca1 + p); s2 = ldn<V, OE_NT_COEF>(ca2 + p);
+            c0 = ldn<V, OE_NT_COEF>(cb0 + p); c1 = ldn<V, OE_NT_COEF>(cb1 + p); c2 = ldn<V, OE_NT_COEF>(cb2 + p);
         }
 
         Psi<V> px, py, pz;
@@ -407,9 +454,9 @@ __global__ __launch_bounds__(256) void k_update_e(YeeArgs a, CpmlArgs c) {
         if constexpr ((AX & AX_Z) != 0) psi_store<V>(pz);
 
         if (row) {
-            stv<V>(Ex + p, ex);
-            stv<V>(Ey + p, ey);
-            stv<V>(Ez + p, ez);
+            stn<V, OE_NT_STORE>(Ex + p, ex);
+            stn<V, OE_NT_STORE>(Ey + p, ey);
+            stn<V, OE_NT_STORE>(Ez + p, ez);
         }
         hy_p = hy_c;
         hz_p = hz_c;

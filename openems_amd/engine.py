@@ -277,8 +277,9 @@ class HipEngine:
         check(lib().oe_engine_energy(self._h, C.byref(e), C.byref(h)))
         return e.value, h.value
 
-    def set_profiling(self, on: bool):
-        check(lib().oe_engine_set_profiling(self._h, int(on)))
+    def set_profiling(self, period: int):
+        """0 off; n > 0 times every launch of every n-th step."""
+        check(lib().oe_engine_set_profiling(self._h, int(period)))
 
     def kernel_time(self, name: str) -> Tuple[int, float]:
         n, ms = C.c_uint64(), C.c_double()
