@@ -76,6 +76,15 @@ oe_status oe_operator_set_material(oe_operator* op, double eps_r, double mu_r, d
 /* Operator::set_pec (src/fdtd/operator.rs:249-263): Ca = Cb = 0. */
 oe_status oe_operator_set_pec(oe_operator* op, const size_t region[6]);
 
+/* Lumped resistor R (ohm) on the `dir` edges of region, as an openEMS
+ * LumpedElement (python/openEMS/ports.py:177-181; replaces the heuristic
+ * LumpedRlc of src/extensions/lumped_rlc.rs:165-313).  Edges along dir are in
+ * series, edge lines across it in parallel; each edge gets the conductivity
+ * d_dir / (R_edge * A), R_edge = R * n_parallel / n_series, added to the
+ * material under it (the last set_material covering the cell) before Ca/Cb
+ * of component dir are formed.  PEC cells stay PEC. */
+oe_status oe_operator_add_lumped_element(oe_operator* op, int dir, const size_t region[6], double R);
+
 /* ------------------------------------------------------------------ engine */
 typedef struct oe_engine oe_engine;
 typedef struct oe_extension oe_extension;
@@ -282,6 +291,52 @@ oe_status oe_ext_field_probe_create(oe_engine* eng, uint32_t i, uint32_t j, uint
 
 /* Number of samples recorded and copy of [n][6] float64 values. */
 oe_status oe_ext_probe_read(oe_extension* ext, double* out, size_t capacity, size_t* n_samples);
+
+/* Running DFT of field signals (FrequencyDomainFieldDump::accumulate,
+ * src/processing/fields_fd.rs:149-217, with the openEMS pulse convention of
+ * python/openEMS/utilities.py:21-35).  Signal m is the float64 sum
+ * sum_t weight[t] * F[component[t]][cell[t]] over its terms
+ * offsets[m] .. offsets[m+1]; components 0..2 are Ex..Ez, 3..5 Hx..Hz, cell
+ * is the linear index (i*ny + j)*nz + k.  A signal's terms must all be E or
+ * all H; E signals are sampled at t = (n+1) dt and H signals at
+ * (n+1/2) dt after step n.  After every step each signal is accumulated as
+ * X(f) += x exp(-j 2 pi f t) (reads return 2 dt X).  precision FP64 keeps
+ * complex float64 sums; FP32_KAHAN keeps float32 sums with Kahan
+ * compensation.  record_steps > 0 also keeps the raw series. */
+typedef enum { OE_DFT_FP64 = 0, OE_DFT_FP32_KAHAN = 1 } oe_dft_precision;
+typedef struct {
+    size_t n_signals;
+    const uint64_t* offsets;   /* n_signals + 1 */
+    const uint32_t* component; /* per term */
+    const uint64_t* cell;      /* per term */
+    const double* weight;      /* per term */
+    const double* freqs;       /* Hz */
+    size_t n_freqs;
+    int precision;             /* oe_dft_precision */
+    uint64_t record_steps;
+} oe_dft_config;
+oe_status oe_ext_dft_create(oe_engine* eng, const oe_dft_config* cfg, oe_extension** out);
+
+/* 2 dt X(f) for every (frequency, signal): re/im are [n_freqs][n_signals]. */
+oe_status oe_ext_dft_read(oe_extension* ext, double* re, double* im, uint64_t* n_accumulated);
+
+/* Recorded raw series [n][n_signals] (n <= record_steps). */
+oe_status oe_ext_dft_read_series(oe_extension* ext, double* out, size_t capacity_steps, size_t* n_steps);
+
+/* Lumped port probes (python/openEMS/ports.py:155-209): a DFT extension with
+ * two signals on the port of region/dir (sign = +1 when the port points
+ * along +dir): U = -sign * sum(E_dir * d_dir) along the centre line of the
+ * cross-section, and I = sign * loop integral of H around the dir-edges of
+ * the cross-section at the middle index along dir.  The port's resistor is
+ * oe_operator_add_lumped_element; its excitation is a soft source of
+ * -sign * amplitude * signal on every dir-edge of region. */
+oe_status oe_ext_lumped_port_create(oe_engine* eng, int dir, const size_t region[6], int sign,
+                                    const double* freqs, size_t n_freqs, int precision,
+                                    uint64_t record_steps, oe_extension** out);
+
+/* Port::CalcPort (python/openEMS/ports.py:115-152) on a lumped port's DFT:
+ * uf_inc = (U + I z_ref)/2, uf_ref = U - uf_inc, S11 = uf_ref / uf_inc. */
+oe_status oe_ext_port_s11(oe_extension* port, double z_ref, double* s11_re, double* s11_im);
 
 /* CPML psi arrays / ADE state for inspection: field 0 = E, 1 = H; which 0 =
  * psi_pos, 1 = psi_neg.  Returns the element count in *n. */

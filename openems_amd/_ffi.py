@@ -60,6 +60,12 @@ class Waveform(C.Structure):
                 ("amplitude", C.c_double)]
 
 
+class DftConfig(C.Structure):
+    _fields_ = [("n_signals", C.c_size_t), ("offsets", C.POINTER(C.c_uint64)), ("component", C.POINTER(C.c_uint32)),
+                ("cell", C.POINTER(C.c_uint64)), ("weight", C.POINTER(C.c_double)), ("freqs", C.POINTER(C.c_double)),
+                ("n_freqs", C.c_size_t), ("precision", C.c_int), ("record_steps", C.c_uint64)]
+
+
 class CpmlConfig(C.Structure):
     _fields_ = [("layers", C.c_size_t * 6), ("grading_order", C.c_double), ("reflection_coeff", C.c_double),
                 ("alpha_max", C.c_double), ("kappa_max", C.c_double), ("mode", C.c_int)]
@@ -87,6 +93,7 @@ SIGNATURES = {
     "oe_operator_dims": (None, [_vp, _szp]),
     "oe_operator_set_material": (_i, [_vp, _d, _d, _d, _d, _szp]),
     "oe_operator_set_pec": (_i, [_vp, _szp]),
+    "oe_operator_add_lumped_element": (_i, [_vp, _i, _szp, _d]),
     "oe_engine_create": (_i, [_vp, _i, _pp]),
     "oe_engine_create_from_arrays": (_i, [_sz, _sz, _sz, _dp, _d, C.POINTER(_fp), _i, _pp]),
     "oe_engine_destroy": (None, [_vp]),
@@ -108,6 +115,11 @@ SIGNATURES = {
     "oe_ext_ade_create": (_i, [_vp, _i, _dp, C.POINTER(_u32), _sz, _i, _pp]),
     "oe_ext_field_probe_create": (_i, [_vp, _u32, _u32, _u32, _pp]),
     "oe_ext_probe_read": (_i, [_vp, _dp, _sz, _szp]),
+    "oe_ext_dft_create": (_i, [_vp, C.POINTER(DftConfig), _pp]),
+    "oe_ext_dft_read": (_i, [_vp, _dp, _dp, C.POINTER(_u64)]),
+    "oe_ext_dft_read_series": (_i, [_vp, _dp, _sz, _szp]),
+    "oe_ext_lumped_port_create": (_i, [_vp, _i, _szp, _i, _dp, _sz, _i, _u64, _pp]),
+    "oe_ext_port_s11": (_i, [_vp, _d, _dp, _dp]),
     "oe_ext_cpml_read_psi": (_i, [_vp, _i, _i, _i, _fp, _sz, _szp]),
     "oe_ext_ade_read_state": (_i, [_vp, _fp, _sz, _szp]),
     "oe_extension_reset": (_i, [_vp]),

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <complex>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -55,9 +56,11 @@ oe_status fail(oe_status code, const char* fmt, ...) {
 
 // ---------------------------------------------------------------- operator
 struct MatOp {
-    int pec;
+    int kind;  // 0 material, 1 PEC, 2 lumped resistor
     double eps_r, mu_r, se, sm;
     size_t box[6];
+    int dir;   // lumped
+    double R;  // lumped
 };
 
 struct oe_operator {
@@ -68,7 +71,7 @@ struct oe_operator {
 };
 
 // ------------------------------------------------------------ extensions
-enum ExtKind { EXT_CPML = 1, EXT_MUR = 2, EXT_ADE = 3, EXT_PROBE = 4 };
+enum ExtKind { EXT_CPML = 1, EXT_MUR = 2, EXT_ADE = 3, EXT_PROBE = 4, EXT_DFT = 5 };
 
 struct DevBuf {
     void* p = nullptr;
@@ -102,6 +105,12 @@ struct oe_extension {
     DevBuf probe_mem;   // data
     DevBuf probe_cnt;   // uint64 counter
     uint64_t probe_cap = 0;
+    // DFT (and lumped port)
+    DftDev dft{};
+    DevBuf dft_mem;
+    uint64_t dft_count = 0;
+    std::vector<double> dft_freqs;
+    bool is_port = false;
 };
 
 // ------------------------------------------------------------------ engine
@@ -301,11 +310,43 @@ oe_status materialise(oe_engine* e, const oe_operator* op) {
         HIP_OK(launch_fill(e->Cf + (size_t)(6 + d) * N, N, 1.0f, e->stream));
         HIP_OK(launch_fill(e->Cf + (size_t)(9 + d) * N, N, (float)(-(dt / (MU0 * op->d[d]))), e->stream));
     }
-    for (const MatOp& m : op->ops) {
+    for (size_t oi = 0; oi < op->ops.size(); ++oi) {
+        const MatOp& m = op->ops[oi];
         int box[6];
         const size_t lim[6] = {0, (size_t)g.nx, 0, (size_t)g.ny, 0, (size_t)g.nz};
         for (int q = 0; q < 6; ++q) box[q] = (int)std::min(m.box[q], lim[q | 1]);
-        if (m.pec) {  // src/fdtd/operator.rs:249-263
+        if (m.kind == 2) {
+            // Lumped resistor: per edge, the material in force before this op
+            // plus the resistor's conductivity (python/openEMS/ports.py:177-181).
+            const int d = m.dir, a = (d + 1) % 3, b = (d + 2) % 3;
+            const size_t n[3] = {m.box[1] - m.box[0], m.box[3] - m.box[2], m.box[5] - m.box[4]};
+            const double r_edge = m.R * (double)(n[a] * n[b]) / (double)n[d];
+            const double sig_add = op->d[d] / (r_edge * op->d[a] * op->d[b]);
+            for (int i = box[0]; i < box[1]; ++i)
+                for (int j = box[2]; j < box[3]; ++j)
+                    for (int k = box[4]; k < box[5]; ++k) {
+                        double eps_r = 1.0, se = 0.0;
+                        bool pec = false;
+                        for (size_t pj = 0; pj < oi; ++pj) {
+                            const MatOp& q = op->ops[pj];
+                            if (q.kind == 2) continue;
+                            if ((size_t)i >= q.box[0] && (size_t)i < q.box[1] && (size_t)j >= q.box[2] &&
+                                (size_t)j < q.box[3] && (size_t)k >= q.box[4] && (size_t)k < q.box[5]) {
+                                if (q.kind == 1) pec = true;
+                                else { pec = false; eps_r = q.eps_r; se = q.se; }
+                            }
+                        }
+                        if (pec) continue;
+                        const double eps = eps_r * EPS0;
+                        const double loss = (se + sig_add) * dt / (2.0 * eps);
+                        const int cell[6] = {i, i + 1, j, j + 1, k, k + 1};
+                        HIP_OK(launch_box_fill(e->Cf + (size_t)d * N, g, cell, (float)((1.0 - loss) / (1.0 + loss)), e->stream));
+                        HIP_OK(launch_box_fill(e->Cf + (size_t)(3 + d) * N, g, cell,
+                                               (float)((dt / eps) / (1.0 + loss) / op->d[d]), e->stream));
+                    }
+            continue;
+        }
+        if (m.kind == 1) {  // src/fdtd/operator.rs:249-263
             for (int d = 0; d < 6; ++d) HIP_OK(launch_box_fill(e->Cf + (size_t)d * N, g, box, 0.0f, e->stream));
             continue;
         }
@@ -420,7 +461,7 @@ oe_status oe_operator_set_material(oe_operator* op, double eps_r, double mu_r, d
     OE_TRY(check_region(op, region));
     if (!(eps_r > 0) || !(mu_r > 0)) return fail(OE_ERR_MATERIAL, "eps_r and mu_r must be positive");
     MatOp m{};
-    m.pec = 0; m.eps_r = eps_r; m.mu_r = mu_r; m.se = sigma_e; m.sm = sigma_m;
+    m.kind = 0; m.eps_r = eps_r; m.mu_r = mu_r; m.se = sigma_e; m.sm = sigma_m;
     std::memcpy(m.box, region, sizeof(m.box));
     op->ops.push_back(m);
     return OE_OK;
@@ -430,7 +471,25 @@ oe_status oe_operator_set_pec(oe_operator* op, const size_t region[6]) {
     if (!op || !region) return fail(OE_ERR_CONFIG, "null argument");
     OE_TRY(check_region(op, region));
     MatOp m{};
-    m.pec = 1;
+    m.kind = 1;
+    std::memcpy(m.box, region, sizeof(m.box));
+    op->ops.push_back(m);
+    return OE_OK;
+}
+
+oe_status oe_operator_add_lumped_element(oe_operator* op, int dir, const size_t region[6], double R) {
+    if (!op || !region) return fail(OE_ERR_CONFIG, "null argument");
+    OE_TRY(check_region(op, region));
+    if (dir < 0 || dir > 2) return fail(OE_ERR_CONFIG, "direction must be 0, 1 or 2");
+    if (!(R > 0.0) || !std::isfinite(R)) return fail(OE_ERR_MATERIAL, "lumped resistance must be positive");
+    if (region[1] > op->nx || region[3] > op->ny || region[5] > op->nz)
+        return fail(OE_ERR_GRID, "lumped element region exceeds the grid");
+    for (int a = 0; a < 3; ++a)
+        if (region[2 * a] == region[2 * a + 1]) return fail(OE_ERR_CONFIG, "lumped element region is empty");
+    MatOp m{};
+    m.kind = 2;
+    m.dir = dir;
+    m.R = R;
     std::memcpy(m.box, region, sizeof(m.box));
     op->ops.push_back(m);
     return OE_OK;
@@ -831,6 +890,168 @@ oe_status oe_ext_field_probe_create(oe_engine* e, uint32_t i, uint32_t j, uint32
     return OE_OK;
 }
 
+oe_status oe_ext_dft_create(oe_engine* e, const oe_dft_config* cfg, oe_extension** out) {
+    if (!e || !cfg || !out) return fail(OE_ERR_CONFIG, "null argument");
+    const size_t n = cfg->n_signals, nf = cfg->n_freqs;
+    if (n == 0 || n > (size_t)INT32_MAX || !cfg->offsets) return fail(OE_ERR_CONFIG, "need at least one signal");
+    if (nf == 0 || nf > 4096 || !cfg->freqs) return fail(OE_ERR_CONFIG, "need 1..4096 frequencies");
+    if (cfg->precision != OE_DFT_FP64 && cfg->precision != OE_DFT_FP32_KAHAN) return fail(OE_ERR_CONFIG, "bad DFT precision");
+    const size_t nt = cfg->offsets[n];
+    if (cfg->offsets[0] != 0 || (nt && (!cfg->component || !cfg->cell || !cfg->weight)))
+        return fail(OE_ERR_CONFIG, "bad DFT term arrays");
+    std::vector<uint8_t> is_h(n);
+    for (size_t m = 0; m < n; ++m) {
+        if (cfg->offsets[m + 1] < cfg->offsets[m] || cfg->offsets[m + 1] > nt) return fail(OE_ERR_CONFIG, "DFT offsets not monotone");
+        int kind = -1;
+        for (uint64_t t = cfg->offsets[m]; t < cfg->offsets[m + 1]; ++t) {
+            if (cfg->component[t] > 5) return fail(OE_ERR_CONFIG, "DFT term component out of range");
+            if (cfg->cell[t] >= e->g.N) return fail(OE_ERR_CONFIG, "DFT term cell out of the grid");
+            const int h = cfg->component[t] >= 3;
+            if (kind >= 0 && kind != h) return fail(OE_ERR_CONFIG, "signal %zu mixes E and H terms", m);
+            kind = h;
+        }
+        is_h[m] = kind == 1;
+    }
+    HIP_OK(hipSetDevice(e->device));
+    std::unique_ptr<oe_extension> x(new oe_extension());
+    x->kind = EXT_DFT;
+    x->eng = e;
+    x->dft_freqs.assign(cfg->freqs, cfg->freqs + nf);
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t b_off = al((n + 1) * 8), b_comp = al(std::max<size_t>(nt, 1) * 4), b_cell = al(std::max<size_t>(nt, 1) * 8);
+    const size_t b_w = al(std::max<size_t>(nt, 1) * 8), b_h = al(n), b_f = al(nf * 8);
+    const size_t b_acc = cfg->precision == OE_DFT_FP64 ? al(nf * n * 16) : al(nf * n * 16);
+    const size_t b_ser = al(cfg->record_steps * n * 8);
+    HIP_OK(x->dft_mem.alloc(b_off + b_comp + b_cell + b_w + b_h + b_f + b_acc + b_ser));
+    char* base = x->dft_mem.as<char>();
+    DftDev& d = x->dft;
+    d.n_sig = (int)n;
+    d.n_f = (int)nf;
+    d.prec = cfg->precision;
+    char* p = base;
+    d.off = reinterpret_cast<const uint64_t*>(p); HIP_OK(hipMemcpy(p, cfg->offsets, (n + 1) * 8, hipMemcpyHostToDevice)); p += b_off;
+    d.comp = reinterpret_cast<const uint32_t*>(p); if (nt) HIP_OK(hipMemcpy(p, cfg->component, nt * 4, hipMemcpyHostToDevice)); p += b_comp;
+    d.cell = reinterpret_cast<const uint64_t*>(p); if (nt) HIP_OK(hipMemcpy(p, cfg->cell, nt * 8, hipMemcpyHostToDevice)); p += b_cell;
+    d.w = reinterpret_cast<const double*>(p); if (nt) HIP_OK(hipMemcpy(p, cfg->weight, nt * 8, hipMemcpyHostToDevice)); p += b_w;
+    d.is_h = reinterpret_cast<const uint8_t*>(p); HIP_OK(hipMemcpy(p, is_h.data(), n, hipMemcpyHostToDevice)); p += b_h;
+    d.freqs = reinterpret_cast<const double*>(p); HIP_OK(hipMemcpy(p, cfg->freqs, nf * 8, hipMemcpyHostToDevice)); p += b_f;
+    HIP_OK(hipMemset(p, 0, b_acc));
+    if (cfg->precision == OE_DFT_FP64) d.acc64 = reinterpret_cast<double*>(p);
+    else d.acc32 = reinterpret_cast<float*>(p);
+    p += b_acc;
+    d.series = cfg->record_steps ? reinterpret_cast<double*>(p) : nullptr;
+    d.series_cap = cfg->record_steps;
+    e->exts.push_back(x.get());
+    *out = x.release();
+    return OE_OK;
+}
+
+oe_status oe_ext_dft_read(oe_extension* x, double* re, double* im, uint64_t* n_acc) {
+    if (!x || x->kind != EXT_DFT) return fail(OE_ERR_CONFIG, "not a DFT extension");
+    if (!x->eng) return fail(OE_ERR_CONFIG, "the extension's engine was destroyed");
+    oe_engine* e = x->eng;
+    HIP_OK(hipSetDevice(e->device));
+    HIP_OK(hipStreamSynchronize(e->stream));
+    const size_t cnt = (size_t)x->dft.n_f * x->dft.n_sig;
+    const double scale = 2.0 * e->dt;  // pulse convention (python/openEMS/utilities.py:29-35)
+    if (x->dft.prec == OE_DFT_FP64) {
+        std::vector<double> h(2 * cnt);
+        HIP_OK(hipMemcpy(h.data(), x->dft.acc64, h.size() * 8, hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < cnt; ++q) {
+            if (re) re[q] = scale * h[2 * q];
+            if (im) im[q] = scale * h[2 * q + 1];
+        }
+    } else {
+        std::vector<float> h(4 * cnt);
+        HIP_OK(hipMemcpy(h.data(), x->dft.acc32, h.size() * 4, hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < cnt; ++q) {
+            if (re) re[q] = scale * (double)h[4 * q];
+            if (im) im[q] = scale * (double)h[4 * q + 1];
+        }
+    }
+    if (n_acc) *n_acc = x->dft_count;
+    return OE_OK;
+}
+
+oe_status oe_ext_dft_read_series(oe_extension* x, double* out, size_t cap, size_t* n) {
+    if (!x || x->kind != EXT_DFT) return fail(OE_ERR_CONFIG, "not a DFT extension");
+    if (!x->eng) return fail(OE_ERR_CONFIG, "the extension's engine was destroyed");
+    oe_engine* e = x->eng;
+    const size_t avail = (size_t)std::min<uint64_t>(x->dft_count, x->dft.series_cap);
+    if (n) *n = avail;
+    if (out && cap && x->dft.series) {
+        HIP_OK(hipSetDevice(e->device));
+        HIP_OK(hipStreamSynchronize(e->stream));
+        HIP_OK(hipMemcpy(out, x->dft.series, std::min(cap, avail) * x->dft.n_sig * 8, hipMemcpyDeviceToHost));
+    }
+    return OE_OK;
+}
+
+oe_status oe_ext_lumped_port_create(oe_engine* e, int dir, const size_t r[6], int sign, const double* freqs,
+                                    size_t n_freqs, int precision, uint64_t record_steps, oe_extension** out) {
+    if (!e || !r || !out) return fail(OE_ERR_CONFIG, "null argument");
+    if (dir < 0 || dir > 2 || (sign != 1 && sign != -1)) return fail(OE_ERR_CONFIG, "bad port direction or sign");
+    const int n[3] = {e->g.nx, e->g.ny, e->g.nz};
+    for (int q = 0; q < 3; ++q)
+        if (r[2 * q] >= r[2 * q + 1] || r[2 * q + 1] > (size_t)n[q]) return fail(OE_ERR_GRID, "port region empty or outside the grid");
+    const int a = (dir + 1) % 3, b = (dir + 2) % 3;
+    const size_t lo[3] = {r[0], r[2], r[4]}, hi[3] = {r[1], r[3], r[5]};
+    auto lin = [&](size_t p0, size_t p1, size_t p2) { return ((uint64_t)p0 * n[1] + p1) * n[2] + p2; };
+    auto at = [&](size_t qd, size_t qa, size_t qb) {
+        size_t pos[3];
+        pos[dir] = qd; pos[a] = qa; pos[b] = qb;
+        return lin(pos[0], pos[1], pos[2]);
+    };
+    std::vector<uint64_t> off{0}, cell;
+    std::vector<uint32_t> comp;
+    std::vector<double> w;
+    // U: centre line of the cross-section, weight -sign * d_dir per edge
+    const size_t ca = lo[a] + (hi[a] - lo[a] - 1) / 2, cb = lo[b] + (hi[b] - lo[b] - 1) / 2;
+    for (size_t q = lo[dir]; q < hi[dir]; ++q) {
+        comp.push_back((uint32_t)dir); cell.push_back(at(q, ca, cb)); w.push_back(-sign * e->d[dir]);
+    }
+    off.push_back(comp.size());
+    // I: loop integral of H around every dir-edge of the cross-section at the middle index
+    const size_t mid = lo[dir] + (hi[dir] - lo[dir]) / 2;
+    for (size_t qa = lo[a]; qa < hi[a]; ++qa)
+        for (size_t qb = lo[b]; qb < hi[b]; ++qb) {
+            comp.push_back(3 + b); cell.push_back(at(mid, qa, qb)); w.push_back(sign * e->d[b]);
+            if (qa >= 1) { comp.push_back(3 + b); cell.push_back(at(mid, qa - 1, qb)); w.push_back(-sign * e->d[b]); }
+            comp.push_back(3 + a); cell.push_back(at(mid, qa, qb)); w.push_back(-sign * e->d[a]);
+            if (qb >= 1) { comp.push_back(3 + a); cell.push_back(at(mid, qa, qb - 1)); w.push_back(sign * e->d[a]); }
+        }
+    off.push_back(comp.size());
+    oe_dft_config c{};
+    c.n_signals = 2;
+    c.offsets = off.data();
+    c.component = comp.data();
+    c.cell = cell.data();
+    c.weight = w.data();
+    c.freqs = freqs;
+    c.n_freqs = n_freqs;
+    c.precision = precision;
+    c.record_steps = record_steps;
+    OE_TRY(oe_ext_dft_create(e, &c, out));
+    (*out)->is_port = true;
+    return OE_OK;
+}
+
+oe_status oe_ext_port_s11(oe_extension* x, double z_ref, double* s11_re, double* s11_im) {
+    if (!x || x->kind != EXT_DFT || !x->is_port) return fail(OE_ERR_CONFIG, "not a lumped port extension");
+    if (!(z_ref > 0.0)) return fail(OE_ERR_CONFIG, "z_ref must be positive");
+    const size_t nf = x->dft_freqs.size();
+    std::vector<double> re(2 * nf), im(2 * nf);
+    OE_TRY(oe_ext_dft_read(x, re.data(), im.data(), nullptr));
+    for (size_t f = 0; f < nf; ++f) {  // Port::CalcPort (python/openEMS/ports.py:136-139)
+        const std::complex<double> U(re[2 * f], im[2 * f]), I(re[2 * f + 1], im[2 * f + 1]);
+        const std::complex<double> inc = 0.5 * (U + I * z_ref);
+        const std::complex<double> s = (U - inc) / inc;
+        if (s11_re) s11_re[f] = s.real();
+        if (s11_im) s11_im[f] = s.imag();
+    }
+    return OE_OK;
+}
+
 static oe_status probe_reserve(oe_extension* x, uint64_t need) {
     if (need <= x->probe_cap) return OE_OK;
     oe_engine* e = x->eng;
@@ -915,6 +1136,11 @@ oe_status oe_extension_reset(oe_extension* x) {
         HIP_OK(hipMemsetAsync(x->ade.J, 0, 9 * cnt * sizeof(float), e->stream));
     }
     if (x->kind == EXT_PROBE) HIP_OK(hipMemsetAsync(x->probe_cnt.p, 0, sizeof(uint64_t), e->stream));
+    if (x->kind == EXT_DFT) {
+        void* acc = x->dft.prec == OE_DFT_FP64 ? (void*)x->dft.acc64 : (void*)x->dft.acc32;
+        HIP_OK(hipMemsetAsync(acc, 0, (size_t)x->dft.n_f * x->dft.n_sig * 16, e->stream));
+        x->dft_count = 0;
+    }
     HIP_OK(hipStreamSynchronize(e->stream));
     return OE_OK;
 }
@@ -945,7 +1171,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
 
     // Sort extensions by hook.
     const CpmlArgs* cp = nullptr;
-    std::vector<oe_extension*> murs, ades, probes;
+    std::vector<oe_extension*> murs, ades, probes, dfts;
     for (size_t q = 0; q < b->n_extensions; ++q) {
         oe_extension* x = b->extensions[q];
         if (!x) return fail(OE_ERR_CONFIG, "extension %zu is null", q);
@@ -959,6 +1185,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             case EXT_MUR: murs.push_back(x); break;
             case EXT_ADE: ades.push_back(x); break;
             case EXT_PROBE: probes.push_back(x); break;
+            case EXT_DFT: dfts.push_back(x); break;
             default: return fail(OE_ERR_CONFIG, "unknown extension kind");
         }
     }
@@ -1074,6 +1301,12 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             Timed tm(e, "point_ops");
             HIP_OK(launch_point_ops(e->F, e->g.N, e->exc_table.as<PointExc>(), (int)px.size(), st,
                                     e->probe_table.as<PointProbe>(), (int)pp.size(), e->stream));
+        }
+        for (auto* x : dfts) {
+            Timed tm(e, "dft");
+            HIP_OK(launch_dft(e->F, e->g.N, x->dft, x->dft_count, (double)(cur + 1) * e->dt, ((double)cur + 0.5) * e->dt,
+                              e->stream));
+            x->dft_count += 1;
         }
         e->timestep += 1;
         ++executed;

@@ -230,6 +230,39 @@ __global__ void k_ade_post(float* __restrict__ F, const float* __restrict__ Cf, 
     }
 }
 
+// One thread per signal: gather the float64 signal, record it, and update
+// the running sums of every frequency.  FP32_KAHAN keeps float32 sums with
+// Kahan compensation; the phase is evaluated in float64 in both modes.
+__global__ __launch_bounds__(256) void k_dft(const float* __restrict__ F, size_t N, DftDev d, uint64_t sample,
+                                            double tE, double tH) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= d.n_sig) return;
+    double s = 0.0;
+    for (uint64_t t = d.off[m]; t < d.off[m + 1]; ++t) s += d.w[t] * (double)F[(size_t)d.comp[t] * N + d.cell[t]];
+    if (d.series && sample < d.series_cap) d.series[sample * d.n_sig + m] = s;
+    const double tt = d.is_h[m] ? tH : tE;
+    for (int f = 0; f < d.n_f; ++f) {
+        double sn, cs;
+        sincos(2.0 * M_PI * d.freqs[f] * tt, &sn, &cs);
+        const size_t q = (size_t)f * d.n_sig + m;
+        if (d.prec == 0) {
+            d.acc64[2 * q] += s * cs;
+            d.acc64[2 * q + 1] += s * (-sn);
+        } else {
+            const float sv = (float)s;
+            float* a = d.acc32 + 4 * q;
+            const float yr = sv * (float)cs - a[2];
+            const float tr = a[0] + yr;
+            a[2] = (tr - a[0]) - yr;
+            a[0] = tr;
+            const float yi = sv * (float)(-sn) - a[3];
+            const float ti = a[1] + yi;
+            a[3] = (ti - a[1]) - yi;
+            a[1] = ti;
+        }
+    }
+}
+
 int grid_for(size_t n, int threads, int cap) {
     size_t b = (n + threads - 1) / threads;
     if (b < 1) b = 1;
@@ -277,6 +310,13 @@ hipError_t launch_mur_post(float* F, const Grid& g, const MurDev* faces, int n, 
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_mur_post, dim3(256, n), dim3(256), 0, s, F, g, faces);
     for (int f = 0; f < n; ++f) hipLaunchKernelGGL(k_mur_apply, dim3(256), dim3(256), 0, s, F, g, faces, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_dft(const float* F, size_t N, const DftDev& d, uint64_t sample, double tE, double tH,
+                      hipStream_t s) {
+    if (d.n_sig <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dft, dim3((d.n_sig + 255) / 256), dim3(256), 0, s, F, N, d, sample, tE, tH);
     return hipGetLastError();
 }
 

@@ -134,4 +134,21 @@ struct AdeDev {
 hipError_t launch_ade_pre(const float* F, size_t N, const AdeDev& a, hipStream_t s);
 hipError_t launch_ade_post(float* F, const float* Cf, size_t N, const AdeDev& a, hipStream_t s);
 
+// Running DFT of weighted field signals (one thread per signal).
+struct DftDev {
+    int n_sig, n_f, prec;
+    const uint64_t* off;
+    const uint32_t* comp;
+    const uint64_t* cell;
+    const double* w;
+    const uint8_t* is_h;
+    const double* freqs;
+    double* acc64;   // [n_f][n_sig][2]
+    float* acc32;    // [n_f][n_sig][4]: sum re, sum im, compensation re, im
+    double* series;  // [cap][n_sig] or null
+    uint64_t series_cap;
+};
+hipError_t launch_dft(const float* F, size_t N, const DftDev& d, uint64_t sample, double tE, double tH,
+                      hipStream_t s);
+
 }  // namespace oe

@@ -68,6 +68,11 @@ class Operator:
         r = (C.c_size_t * 6)(*map(int, region))
         check(lib().oe_operator_set_pec(self._h, r))
 
+    def add_lumped_element(self, direction: int, region, R: float):
+        """openEMS LumpedElement resistor on the `direction` edges of region."""
+        r = (C.c_size_t * 6)(*map(int, region))
+        check(lib().oe_operator_add_lumped_element(self._h, int(direction), r, float(R)))
+
 
 # -------------------------------------------------------------- waveforms
 @dataclass
@@ -184,6 +189,34 @@ class Extension:
         check(lib().oe_ext_ade_read_state(self._h, out.ctypes.data_as(C.POINTER(C.c_float)), n.value, C.byref(n)))
         return out.reshape(3, -1)
 
+    # DFT / lumped port
+    def dft_values(self) -> np.ndarray:
+        """[n_freqs, n_signals] complex: 2 dt sum x exp(-j 2 pi f t)."""
+        nf, ns = self._dft_shape
+        re = np.empty(nf * ns)
+        im = np.empty(nf * ns)
+        n = C.c_uint64()
+        check(lib().oe_ext_dft_read(self._h, re.ctypes.data_as(C.POINTER(C.c_double)),
+                                    im.ctypes.data_as(C.POINTER(C.c_double)), C.byref(n)))
+        return (re + 1j * im).reshape(nf, ns)
+
+    def series(self) -> np.ndarray:
+        nf, ns = self._dft_shape
+        n = C.c_size_t()
+        check(lib().oe_ext_dft_read_series(self._h, None, 0, C.byref(n)))
+        out = np.empty((n.value, ns), np.float64)
+        if n.value:
+            check(lib().oe_ext_dft_read_series(self._h, out.ctypes.data_as(C.POINTER(C.c_double)), n.value, C.byref(n)))
+        return out
+
+    def s11(self, z_ref: float) -> np.ndarray:
+        nf, _ = self._dft_shape
+        re = np.empty(nf)
+        im = np.empty(nf)
+        check(lib().oe_ext_port_s11(self._h, float(z_ref), re.ctypes.data_as(C.POINTER(C.c_double)),
+                                    im.ctypes.data_as(C.POINTER(C.c_double))))
+        return re + 1j * im
+
     # probes
     def data(self) -> np.ndarray:
         n = C.c_size_t()
@@ -192,6 +225,19 @@ class Extension:
         if n.value:
             check(lib().oe_ext_probe_read(self._h, out.ctypes.data_as(C.POINTER(C.c_double)), n.value, C.byref(n)))
         return out
+
+
+def port_excitations(direction: int, region, samples: np.ndarray, sign: int = 1, amplitude: float = 1.0):
+    """The excitation box of a lumped port: a soft source of
+    -sign * amplitude * samples on every `direction` edge of region
+    (python/openEMS/ports.py:183-188)."""
+    v = (-sign * amplitude * np.asarray(samples, np.float64)).astype(np.float32)
+    out = []
+    for i in range(region[0], region[1]):
+        for j in range(region[2], region[3]):
+            for k in range(region[4], region[5]):
+                out.append(ScheduledExcitation((i, j, k), direction, v, True))
+    return out
 
 
 @dataclass
@@ -318,6 +364,44 @@ class HipEngine:
         check(lib().oe_ext_ade_create(self._h, kind, prm, p.ctypes.data_as(C.POINTER(C.c_uint32)), p.shape[0], mode,
                                       C.byref(h)))
         return Extension(self, h.value, "ade")
+
+    def dft(self, terms, freqs, precision: int = 0, record_steps: int = 0) -> Extension:
+        """Running DFT of signals; terms: per signal a list of (component 0..5, i, j, k, weight)."""
+        nx, ny, nz = self.shape
+        off = np.zeros(len(terms) + 1, np.uint64)
+        comp, cell, w = [], [], []
+        for m, tl in enumerate(terms):
+            for (c, i, j, k, wt) in tl:
+                comp.append(c)
+                cell.append((i * ny + j) * nz + k)
+                w.append(wt)
+            off[m + 1] = len(comp)
+        comp = np.asarray(comp, np.uint32)
+        cell = np.asarray(cell, np.uint64)
+        w = np.asarray(w, np.float64)
+        fr = np.ascontiguousarray(freqs, dtype=np.float64)
+        cfg = _ffi.DftConfig(len(terms), off.ctypes.data_as(C.POINTER(C.c_uint64)),
+                             comp.ctypes.data_as(C.POINTER(C.c_uint32)), cell.ctypes.data_as(C.POINTER(C.c_uint64)),
+                             w.ctypes.data_as(C.POINTER(C.c_double)), fr.ctypes.data_as(C.POINTER(C.c_double)),
+                             len(fr), int(precision), int(record_steps))
+        h = C.c_void_p()
+        check(lib().oe_ext_dft_create(self._h, C.byref(cfg), C.byref(h)))
+        x = Extension(self, h.value, "dft")
+        x._dft_shape = (len(fr), len(terms))
+        return x
+
+    def lumped_port(self, direction: int, region, freqs, sign: int = 1, precision: int = 0,
+                    record_steps: int = 0) -> Extension:
+        """U/I probes and their running DFT for a lumped port (openEMS LumpedPort)."""
+        r = (C.c_size_t * 6)(*map(int, region))
+        fr = np.ascontiguousarray(freqs, dtype=np.float64)
+        h = C.c_void_p()
+        check(lib().oe_ext_lumped_port_create(self._h, int(direction), r, int(sign),
+                                              fr.ctypes.data_as(C.POINTER(C.c_double)), len(fr), int(precision),
+                                              int(record_steps), C.byref(h)))
+        x = Extension(self, h.value, "port")
+        x._dft_shape = (len(fr), 2)
+        return x
 
     def field_probe(self, pos) -> Extension:
         h = C.c_void_p()

@@ -95,6 +95,26 @@ struct Probe {
     std::vector<double> data;  // 6 per sample
 };
 
+// Material operation, kept in order so that a lumped element can find the
+// material under each of its edges (the last operation covering the cell).
+struct MatOp {
+    int pec;
+    double eps_r, mu_r, se, sm;
+    size_t box[6];
+};
+
+// Signals recorded every step: signal m = sum_t w[t] * F[comp[t]][cell[t]]
+// (float64 sum in term order), e.g. a port voltage or current, or a field
+// sample on an NF2FF surface.
+struct SignalSet {
+    size_t n;
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> comp;
+    std::vector<uint64_t> cell;
+    std::vector<double> w;
+    std::vector<double> series;  // [steps][n]
+};
+
 template <class T>
 struct Sim {
     size_t nx, ny, nz, N;
@@ -107,6 +127,8 @@ struct Sim {
     std::vector<MurFace<T>> mur;
     std::vector<Ade<T>> ade;
     std::vector<Probe<T>> probes;
+    std::vector<MatOp> ops;
+    std::vector<SignalSet> signals;
     int threads = 0;
 
     size_t at(size_t i, size_t j, size_t k) const { return (i * ny + j) * nz + k; }
@@ -149,6 +171,7 @@ void set_material(Sim<T>& s, double eps_r, double mu_r, double sig_e, double sig
     T da_mat = (T)((1.0 - loss_m) / (1.0 + loss_m));
     double db_factor = -(dt / mu) / (1.0 + loss_m);
     T dbv[3] = {(T)(db_factor / s.dx), (T)(db_factor / s.dy), (T)(db_factor / s.dz)};
+    s.ops.push_back(MatOp{0, eps_r, mu_r, sig_e, sig_m, {i0, i1, j0, j1, k0, k1}});
     i1 = std::min(i1, s.nx); j1 = std::min(j1, s.ny); k1 = std::min(k1, s.nz);
     for (size_t i = i0; i < i1; ++i)
         for (size_t j = j0; j < j1; ++j)
@@ -164,6 +187,7 @@ void set_material(Sim<T>& s, double eps_r, double mu_r, double sig_e, double sig
 // Operator::set_pec (src/fdtd/operator.rs:249-263)
 template <class T>
 void set_pec(Sim<T>& s, size_t i0, size_t i1, size_t j0, size_t j1, size_t k0, size_t k1) {
+    s.ops.push_back(MatOp{1, 0, 0, 0, 0, {i0, i1, j0, j1, k0, k1}});
     i1 = std::min(i1, s.nx); j1 = std::min(j1, s.ny); k1 = std::min(k1, s.nz);
     for (size_t i = i0; i < i1; ++i)
         for (size_t j = j0; j < j1; ++j)
@@ -171,6 +195,55 @@ void set_pec(Sim<T>& s, size_t i0, size_t i1, size_t j0, size_t j1, size_t k0, s
                 size_t p = s.at(i, j, k);
                 for (int d = 0; d < 3; ++d) { s.ca[d][p] = T(0); s.cb[d][p] = T(0); }
             }
+}
+
+// Lumped resistor over the dir-edges of a box, as an openEMS LumpedElement
+// (python/openEMS/ports.py:177-181): the n_ser edges along dir are in series
+// and the n_par edge lines across it in parallel, so each edge carries
+// R_edge = R n_par / n_ser, i.e. the conductivity sigma += d_dir / (R_edge A)
+// added to the cell's own material before Ca/Cb are formed
+// (src/fdtd/operator.rs:199-209).  PEC cells are left PEC.
+template <class T>
+void add_lumped(Sim<T>& s, int dir, const size_t b[6], double R) {
+    double del[3] = {s.dx, s.dy, s.dz};
+    size_t n[3] = {b[1] - b[0], b[3] - b[2], b[5] - b[4]};
+    size_t n_ser = n[dir], n_par = n[(dir + 1) % 3] * n[(dir + 2) % 3];
+    double r_edge = R * (double)n_par / (double)n_ser;
+    double area = del[(dir + 1) % 3] * del[(dir + 2) % 3];
+    double sig_add = del[dir] / (r_edge * area);
+    for (size_t i = b[0]; i < std::min(b[1], s.nx); ++i)
+        for (size_t j = b[2]; j < std::min(b[3], s.ny); ++j)
+            for (size_t k = b[4]; k < std::min(b[5], s.nz); ++k) {
+                double eps_r = 1.0, se = 0.0;
+                bool pec = false;
+                for (const MatOp& m : s.ops) {
+                    if (i >= m.box[0] && i < m.box[1] && j >= m.box[2] && j < m.box[3] && k >= m.box[4] && k < m.box[5]) {
+                        if (m.pec) pec = true;
+                        else { pec = false; eps_r = m.eps_r; se = m.se; }
+                    }
+                }
+                if (pec) continue;
+                double eps = eps_r * EPS0;
+                double loss = (se + sig_add) * s.dt / (2.0 * eps);
+                size_t p = s.at(i, j, k);
+                s.ca[dir][p] = (T)((1.0 - loss) / (1.0 + loss));
+                s.cb[dir][p] = (T)((s.dt / eps) / (1.0 + loss) / del[dir]);
+            }
+}
+
+template <class T>
+void record_signals(Sim<T>& s) {
+    for (auto& G : s.signals) {
+        for (size_t m = 0; m < G.n; ++m) {
+            double acc = 0.0;
+            for (uint64_t t = G.off[m]; t < G.off[m + 1]; ++t) {
+                const uint32_t c = G.comp[t];
+                const T* F = c < 3 ? s.E[c].data() : s.H[c - 3].data();
+                acc += G.w[t] * (double)F[G.cell[t]];
+            }
+            G.series.push_back(acc);
+        }
+    }
 }
 
 // ---------------------------------------------------------------- CPML
@@ -856,6 +929,43 @@ size_t oro_probe_data(void* hv, size_t idx, double* out) {
     return n;
 }
 
+void oro_add_lumped(void* hv, int dir, const size_t box[6], double R) {
+    oro_handle* h = (oro_handle*)hv;
+    DISPATCH(h, add_lumped(s, dir, box, R));
+}
+
+size_t oro_add_signals(void* hv, size_t n, const uint64_t* off, const uint32_t* comp, const uint64_t* cell,
+                       const double* w) {
+    oro_handle* h = (oro_handle*)hv;
+    size_t idx = 0;
+    DISPATCH(h, {
+        SignalSet G;
+        G.n = n;
+        G.off.assign(off, off + n + 1);
+        size_t nt = off[n];
+        G.comp.assign(comp, comp + nt);
+        G.cell.assign(cell, cell + nt);
+        G.w.assign(w, w + nt);
+        s.signals.push_back(std::move(G));
+        idx = s.signals.size() - 1;
+    });
+    return idx;
+}
+
+// number of recorded steps; out gets [steps][n]
+size_t oro_signal_series(void* hv, size_t idx, double* out) {
+    oro_handle* h = (oro_handle*)hv;
+    size_t steps = 0;
+    DISPATCH(h, {
+        if (idx < s.signals.size()) {
+            auto& G = s.signals[idx];
+            steps = G.n ? G.series.size() / G.n : 0;
+            if (out) std::memcpy(out, G.series.data(), G.series.size() * sizeof(double));
+        }
+    });
+    return steps;
+}
+
 double oro_energy(void* hv, int which) {
     oro_handle* h = (oro_handle*)hv;
     double r = 0;
@@ -884,6 +994,7 @@ int oro_run_batch(void* hv, oro_batch* b) {
             step(s);
             apply_exc(s, b->exc, b->n_exc, b->samples, b->n_samples, st);
             sample_probes(s);
+            record_signals(s);
             s.timestep += 1;
             b->steps_executed += 1;
             if (b->sample_interval > 0 && cur % b->sample_interval == 0) {

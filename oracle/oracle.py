@@ -100,6 +100,11 @@ def lib():
         L.oro_add_field_probe.argtypes = [vp, u32, u32, u32]
         L.oro_probe_data.restype = sz
         L.oro_probe_data.argtypes = [vp, sz, dp]
+        L.oro_add_lumped.argtypes = [vp, C.c_int, C.POINTER(sz), d]
+        L.oro_add_signals.restype = sz
+        L.oro_add_signals.argtypes = [vp, sz, C.POINTER(u64), C.POINTER(u32), C.POINTER(u64), dp]
+        L.oro_signal_series.restype = sz
+        L.oro_signal_series.argtypes = [vp, sz, dp]
         L.oro_energy.restype = d
         L.oro_energy.argtypes = [vp, C.c_int]
         L.oro_yee_steps.argtypes = [vp, u64]
@@ -223,6 +228,34 @@ class OracleSim:
         out = np.empty(6 * n, np.float64)
         lib().oro_probe_data(self._h, idx, _dptr(out))
         return out.reshape(n, 6)
+
+    def add_lumped(self, direction: int, region, R: float):
+        arr = (C.c_size_t * 6)(*map(int, region))
+        lib().oro_add_lumped(self._h, direction, arr, float(R))
+
+    def add_signals(self, terms) -> int:
+        """terms: list (one per signal) of lists of (component 0..5, i, j, k, weight)."""
+        off = np.zeros(len(terms) + 1, np.uint64)
+        comp, cell, w = [], [], []
+        nx, ny, nz = self.shape
+        for m, tl in enumerate(terms):
+            for (c, i, j, k, wt) in tl:
+                comp.append(c)
+                cell.append((i * ny + j) * nz + k)
+                w.append(wt)
+            off[m + 1] = len(comp)
+        comp = np.asarray(comp, np.uint32)
+        cell = np.asarray(cell, np.uint64)
+        w = np.asarray(w, np.float64)
+        return lib().oro_add_signals(self._h, len(terms), off.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                     comp.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                     cell.ctypes.data_as(C.POINTER(C.c_uint64)), _dptr(w))
+
+    def signal_series(self, idx: int, n_signals: int) -> np.ndarray:
+        steps = lib().oro_signal_series(self._h, idx, None)
+        out = np.empty(steps * n_signals, np.float64)
+        lib().oro_signal_series(self._h, idx, _dptr(out))
+        return out.reshape(steps, n_signals)
 
     def energy(self):
         return lib().oro_energy(self._h, 0), lib().oro_energy(self._h, 1)
