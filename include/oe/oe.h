@@ -314,6 +314,7 @@ typedef struct {
     size_t n_freqs;
     int precision;             /* oe_dft_precision */
     uint64_t record_steps;
+    uint32_t time_stride;      /* accumulate every n-th step (0 or 1: every step); reads scale by n */
 } oe_dft_config;
 oe_status oe_ext_dft_create(oe_engine* eng, const oe_dft_config* cfg, oe_extension** out);
 
@@ -337,6 +338,69 @@ oe_status oe_ext_lumped_port_create(oe_engine* eng, int dir, const size_t region
 /* Port::CalcPort (python/openEMS/ports.py:115-152) on a lumped port's DFT:
  * uf_inc = (U + I z_ref)/2, uf_ref = U - uf_inc, S11 = uf_ref / uf_inc. */
 oe_status oe_ext_port_s11(oe_extension* port, double z_ref, double* s11_re, double* s11_im);
+
+/* ------------------------------------------------------------------ NF2FF
+ * The reference's Nf2ff (src/nf2ff/mod.rs:72-170) keeps the API but stubs
+ * the surface integral (integrate_surfaces returns zeros, :157-170;
+ * max_directivity returns 1, :66-69); the transform here is the one of
+ * matlab/AnalyzeNF2FF.m: J = n x H, M = -n x E on every face, radiation
+ * integrals with exp(+j k r_hat . r') and 2-D trapezoid weights,
+ * E_theta = -jk e^{-jkr}/(4 pi r) (L_phi + Z0 N_theta),
+ * E_phi = jk e^{-jkr}/(4 pi r) (L_theta - Z0 N_phi), P_rad = 1/2 sum
+ * Re((E x H*) . n) dA, D_max = 4 pi U_max / P_rad.  The per-face contraction
+ * over the first in-plane axis runs on the FP32 matrix cores; the phase
+ * factors, the second-axis reduction and P_rad are float64. */
+
+/* FarFieldResult (src/nf2ff/mod.rs:11-24) of one frequency.  The four arrays
+ * are [n_theta][n_phi], caller-owned; any may be null. */
+typedef struct {
+    double* e_theta_re;
+    double* e_theta_im;
+    double* e_phi_re;
+    double* e_phi_im;
+    double prad;       /* radiated power through the surface, W */
+    double dmax;       /* 4 pi U_max / prad (FarFieldResult::max_directivity) */
+    double gemm_ms;    /* device time of the matrix-core contractions */
+    double total_ms;   /* device time of the whole transform */
+    double gemm_flop;  /* real floating-point operations of the contractions */
+} oe_far_field;
+
+typedef struct oe_nf2ff oe_nf2ff;
+
+/* Nf2ff::new (src/nf2ff/mod.rs:98-105) on HIP device `device`. */
+oe_status oe_nf2ff_create(int device, double frequency, oe_nf2ff** out);
+void oe_nf2ff_destroy(oe_nf2ff* nf);
+
+/* Nf2ff::add_surface (src/nf2ff/mod.rs:108-110) with a NearFieldSurface
+ * (:78-91): the face of normal axis normal_dir (0..2) with outward sign
+ * +1/-1 at coordinate `position`, sampled at nodes c1[n1] x c2[n2] along the
+ * in-plane axes (normal_dir+1)%3 and (normal_dir+2)%3.  e_t and h_t hold the
+ * tangential phasors along those two axes as interleaved complex float64,
+ * [2][n1][n2][re, im].  Trapezoid weights follow from c1 and c2. */
+oe_status oe_nf2ff_add_surface(oe_nf2ff* nf, int normal_dir, int sign, double position, const double* c1,
+                               size_t n1, const double* c2, size_t n2, const double* e_t, const double* h_t);
+
+/* Nf2ff::calculate (src/nf2ff/mod.rs:117-154): far field at distance radius
+ * for every (theta[i], phi[j]); center is the phase reference point (null:
+ * the origin). */
+oe_status oe_nf2ff_calculate(oe_nf2ff* nf, const double* theta, size_t n_theta, const double* phi,
+                             size_t n_phi, double radius, const double center[3], oe_far_field* out);
+
+/* NF2FF recording box (openEMS CreateNF2FFBox, python/openEMS/nf2ff.py:26-80,
+ * with frequency-domain recording): a running DFT of the tangential E and H
+ * phasors on the closed surface through the mesh nodes box = {i0, i1, j0,
+ * j1, k0, k1} (inclusive node indices, 1 <= lo < hi <= n-1).  E at a node is
+ * the mean of the two edges that meet there, H the mean of the four
+ * surrounding faces (openEMS's interpolated dump).  directions is the
+ * 6-bit face mask (bit 2*axis + side, side 1 = upper face; 0 = all faces).
+ * Node coordinates are index * delta.  time_stride as in oe_dft_config. */
+oe_status oe_ext_nf2ff_create(oe_engine* eng, const size_t box[6], int directions, const double* freqs,
+                              size_t n_freqs, int precision, uint32_t time_stride, oe_extension** out);
+
+/* Nf2ff::calculate on the recorded phasors of frequency freqs[freq_index]. */
+oe_status oe_ext_nf2ff_calculate(oe_extension* ext, size_t freq_index, const double* theta, size_t n_theta,
+                                 const double* phi, size_t n_phi, double radius, const double center[3],
+                                 oe_far_field* out);
 
 /* CPML psi arrays / ADE state for inspection: field 0 = E, 1 = H; which 0 =
  * psi_pos, 1 = psi_neg.  Returns the element count in *n. */

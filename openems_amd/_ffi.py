@@ -63,7 +63,15 @@ class Waveform(C.Structure):
 class DftConfig(C.Structure):
     _fields_ = [("n_signals", C.c_size_t), ("offsets", C.POINTER(C.c_uint64)), ("component", C.POINTER(C.c_uint32)),
                 ("cell", C.POINTER(C.c_uint64)), ("weight", C.POINTER(C.c_double)), ("freqs", C.POINTER(C.c_double)),
-                ("n_freqs", C.c_size_t), ("precision", C.c_int), ("record_steps", C.c_uint64)]
+                ("n_freqs", C.c_size_t), ("precision", C.c_int), ("record_steps", C.c_uint64),
+                ("time_stride", C.c_uint32)]
+
+
+class FarField(C.Structure):
+    _fields_ = [("e_theta_re", C.POINTER(C.c_double)), ("e_theta_im", C.POINTER(C.c_double)),
+                ("e_phi_re", C.POINTER(C.c_double)), ("e_phi_im", C.POINTER(C.c_double)),
+                ("prad", C.c_double), ("dmax", C.c_double), ("gemm_ms", C.c_double), ("total_ms", C.c_double),
+                ("gemm_flop", C.c_double)]
 
 
 class CpmlConfig(C.Structure):
@@ -120,6 +128,12 @@ SIGNATURES = {
     "oe_ext_dft_read_series": (_i, [_vp, _dp, _sz, _szp]),
     "oe_ext_lumped_port_create": (_i, [_vp, _i, _szp, _i, _dp, _sz, _i, _u64, _pp]),
     "oe_ext_port_s11": (_i, [_vp, _d, _dp, _dp]),
+    "oe_nf2ff_create": (_i, [_i, _d, _pp]),
+    "oe_nf2ff_destroy": (None, [_vp]),
+    "oe_nf2ff_add_surface": (_i, [_vp, _i, _i, _d, _dp, _sz, _dp, _sz, _dp, _dp]),
+    "oe_nf2ff_calculate": (_i, [_vp, _dp, _sz, _dp, _sz, _d, _dp, C.POINTER(FarField)]),
+    "oe_ext_nf2ff_create": (_i, [_vp, _szp, _i, _dp, _sz, _i, _u32, _pp]),
+    "oe_ext_nf2ff_calculate": (_i, [_vp, _sz, _dp, _sz, _dp, _sz, _d, _dp, C.POINTER(FarField)]),
     "oe_ext_cpml_read_psi": (_i, [_vp, _i, _i, _i, _fp, _sz, _szp]),
     "oe_ext_ade_read_state": (_i, [_vp, _fp, _sz, _szp]),
     "oe_extension_reset": (_i, [_vp]),

@@ -73,18 +73,6 @@ struct oe_operator {
 // ------------------------------------------------------------ extensions
 enum ExtKind { EXT_CPML = 1, EXT_MUR = 2, EXT_ADE = 3, EXT_PROBE = 4, EXT_DFT = 5 };
 
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-    hipError_t alloc(size_t n) {
-        if (p) { (void)hipFree(p); p = nullptr; }
-        bytes = n;
-        return n ? hipMalloc(&p, n) : hipSuccess;
-    }
-    template <class T> T* as() const { return reinterpret_cast<T*>(p); }
-};
-
 struct oe_extension {
     int kind = 0;
     oe_engine* eng = nullptr;
@@ -108,9 +96,15 @@ struct oe_extension {
     // DFT (and lumped port)
     DftDev dft{};
     DevBuf dft_mem;
-    uint64_t dft_count = 0;
+    uint64_t dft_count = 0;   // accumulated samples
+    uint64_t dft_seen = 0;    // steps seen (time stride)
+    uint32_t dft_stride = 1;
     std::vector<double> dft_freqs;
     bool is_port = false;
+    // NF2FF box: faces whose phasors are consecutive DFT signals from nf_sig0[f]
+    bool is_nf2ff = false;
+    std::vector<Nf2ffFace> nf_faces;
+    std::vector<size_t> nf_sig0;
 };
 
 // ------------------------------------------------------------------ engine
@@ -917,6 +911,7 @@ oe_status oe_ext_dft_create(oe_engine* e, const oe_dft_config* cfg, oe_extension
     x->kind = EXT_DFT;
     x->eng = e;
     x->dft_freqs.assign(cfg->freqs, cfg->freqs + nf);
+    x->dft_stride = cfg->time_stride > 1 ? cfg->time_stride : 1;
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t b_off = al((n + 1) * 8), b_comp = al(std::max<size_t>(nt, 1) * 4), b_cell = al(std::max<size_t>(nt, 1) * 8);
     const size_t b_w = al(std::max<size_t>(nt, 1) * 8), b_h = al(n), b_f = al(nf * 8);
@@ -953,7 +948,7 @@ oe_status oe_ext_dft_read(oe_extension* x, double* re, double* im, uint64_t* n_a
     HIP_OK(hipSetDevice(e->device));
     HIP_OK(hipStreamSynchronize(e->stream));
     const size_t cnt = (size_t)x->dft.n_f * x->dft.n_sig;
-    const double scale = 2.0 * e->dt;  // pulse convention (python/openEMS/utilities.py:29-35)
+    const double scale = 2.0 * e->dt * x->dft_stride;  // pulse convention (python/openEMS/utilities.py:29-35)
     if (x->dft.prec == OE_DFT_FP64) {
         std::vector<double> h(2 * cnt);
         HIP_OK(hipMemcpy(h.data(), x->dft.acc64, h.size() * 8, hipMemcpyDeviceToHost));
@@ -1052,6 +1047,209 @@ oe_status oe_ext_port_s11(oe_extension* x, double z_ref, double* s11_re, double*
     return OE_OK;
 }
 
+// ------------------------------------------------------------------ NF2FF
+namespace {
+void far_field_out(const Nf2ffOut& o, oe_far_field* out) {
+    out->prad = o.prad;
+    out->dmax = o.dmax;
+    out->gemm_ms = o.gemm_ms;
+    out->total_ms = o.total_ms;
+    out->gemm_flop = o.gemm_flop;
+}
+Nf2ffOut far_field_in(const oe_far_field* out) {
+    Nf2ffOut o{};
+    o.et_re = out->e_theta_re;
+    o.et_im = out->e_theta_im;
+    o.ep_re = out->e_phi_re;
+    o.ep_im = out->e_phi_im;
+    return o;
+}
+oe_status angles_ok(const double* theta, size_t nt, const double* phi, size_t np, double r, oe_far_field* out) {
+    if (!theta || !phi || !out || nt == 0 || np == 0) return fail(OE_ERR_CONFIG, "need theta and phi angles");
+    if (nt * np > (size_t)INT32_MAX / 16) return fail(OE_ERR_CONFIG, "too many angles");
+    if (!(r > 0.0)) return fail(OE_ERR_CONFIG, "radius must be positive");
+    return OE_OK;
+}
+}  // namespace
+
+oe_status oe_ext_nf2ff_create(oe_engine* e, const size_t box[6], int directions, const double* freqs, size_t n_freqs,
+                              int precision, uint32_t time_stride, oe_extension** out) {
+    if (!e || !box || !out) return fail(OE_ERR_CONFIG, "null argument");
+    const int n[3] = {e->g.nx, e->g.ny, e->g.nz};
+    const size_t lo[3] = {box[0], box[2], box[4]}, hi[3] = {box[1], box[3], box[5]};
+    for (int q = 0; q < 3; ++q)
+        if (lo[q] < 1 || lo[q] >= hi[q] || hi[q] > (size_t)n[q] - 1)
+            return fail(OE_ERR_GRID, "NF2FF box nodes must satisfy 1 <= lo < hi <= n-1 on every axis");
+    if (directions == 0) directions = 0x3f;
+    if (directions & ~0x3f) return fail(OE_ERR_CONFIG, "directions is a 6-bit face mask");
+    auto lin = [&](const size_t p[3]) { return ((uint64_t)p[0] * n[1] + p[1]) * n[2] + p[2]; };
+    std::vector<uint64_t> off{0}, cell;
+    std::vector<uint32_t> comp;
+    std::vector<double> w;
+    std::vector<Nf2ffFace> faces;
+    std::vector<size_t> sig0;
+    size_t n_sig = 0;
+    for (int ax = 0; ax < 3; ++ax)
+        for (int side = 0; side < 2; ++side) {
+            if (!(directions & (1 << (2 * ax + side)))) continue;
+            const int a1 = (ax + 1) % 3, a2 = (ax + 2) % 3;
+            Nf2ffFace f{};
+            f.axis = ax;
+            f.sign = side ? 1 : -1;
+            const size_t qa = side ? hi[ax] : lo[ax];
+            f.c0 = (double)qa * e->d[ax];
+            for (size_t i = lo[a1]; i <= hi[a1]; ++i) f.c1.push_back((double)i * e->d[a1]);
+            for (size_t i = lo[a2]; i <= hi[a2]; ++i) f.c2.push_back((double)i * e->d[a2]);
+            sig0.push_back(n_sig);
+            for (size_t i1 = lo[a1]; i1 <= hi[a1]; ++i1)
+                for (size_t i2 = lo[a2]; i2 <= hi[a2]; ++i2) {
+                    size_t p[3];
+                    p[ax] = qa; p[a1] = i1; p[a2] = i2;
+                    // E_t at the node: mean of the two edges that meet there
+                    for (int c : {a1, a2}) {
+                        for (int d = -1; d <= 0; ++d) {
+                            size_t t[3] = {p[0], p[1], p[2]};
+                            t[c] += d;
+                            comp.push_back((uint32_t)c); cell.push_back(lin(t)); w.push_back(0.5);
+                        }
+                        off.push_back(comp.size());
+                    }
+                    // H_t at the node: mean of the four faces around it
+                    for (int c : {a1, a2}) {
+                        const int o = c == a1 ? a2 : a1;
+                        for (int dn = -1; dn <= 0; ++dn)
+                            for (int dt = -1; dt <= 0; ++dt) {
+                                size_t t[3] = {p[0], p[1], p[2]};
+                                t[ax] += dn;
+                                t[o] += dt;
+                                comp.push_back(3 + (uint32_t)c); cell.push_back(lin(t)); w.push_back(0.25);
+                            }
+                        off.push_back(comp.size());
+                    }
+                    n_sig += 4;
+                }
+            faces.push_back(std::move(f));
+        }
+    oe_dft_config c{};
+    c.n_signals = n_sig;
+    c.offsets = off.data();
+    c.component = comp.data();
+    c.cell = cell.data();
+    c.weight = w.data();
+    c.freqs = freqs;
+    c.n_freqs = n_freqs;
+    c.precision = precision;
+    c.record_steps = 0;
+    c.time_stride = time_stride;
+    OE_TRY(oe_ext_dft_create(e, &c, out));
+    (*out)->is_nf2ff = true;
+    (*out)->nf_faces = std::move(faces);
+    (*out)->nf_sig0 = std::move(sig0);
+    return OE_OK;
+}
+
+oe_status oe_ext_nf2ff_calculate(oe_extension* x, size_t fi, const double* theta, size_t nt, const double* phi,
+                                 size_t np, double radius, const double center[3], oe_far_field* out) {
+    if (!x || x->kind != EXT_DFT || !x->is_nf2ff) return fail(OE_ERR_CONFIG, "not an NF2FF extension");
+    if (!x->eng) return fail(OE_ERR_CONFIG, "the extension's engine was destroyed");
+    OE_TRY(angles_ok(theta, nt, phi, np, radius, out));
+    if (fi >= x->dft_freqs.size()) return fail(OE_ERR_CONFIG, "frequency index out of range");
+    oe_engine* e = x->eng;
+    HIP_OK(hipSetDevice(e->device));
+    const double zero[3] = {0, 0, 0};
+    const size_t base = fi * (size_t)x->dft.n_sig;
+    for (size_t f = 0; f < x->nf_faces.size(); ++f) {
+        Nf2ffFace& fc = x->nf_faces[f];
+        const size_t q = base + x->nf_sig0[f];
+        fc.src_kind = x->dft.prec == OE_DFT_FP64 ? 0 : 1;
+        fc.src = fc.src_kind == 0 ? (const void*)(x->dft.acc64 + 2 * q) : (const void*)(x->dft.acc32 + 4 * q);
+        fc.scale = 2.0 * e->dt * x->dft_stride;  // pulse convention (python/openEMS/utilities.py:29-35)
+    }
+    Nf2ffOut o = far_field_in(out);
+    HIP_OK(nf2ff_compute(x->nf_faces, x->dft_freqs[fi], theta, (int)nt, phi, (int)np, radius, center ? center : zero,
+                         e->stream, &o));
+    far_field_out(o, out);
+    return OE_OK;
+}
+
+struct oe_nf2ff {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double freq = 0.0;
+    std::vector<Nf2ffFace> faces;
+    std::vector<std::unique_ptr<DevBuf>> bufs;
+    ~oe_nf2ff() {
+        if (stream) {
+            (void)hipSetDevice(device);
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+};
+
+oe_status oe_nf2ff_create(int device, double frequency, oe_nf2ff** out) {
+    if (!out) return fail(OE_ERR_CONFIG, "null argument");
+    if (!(frequency > 0.0)) return fail(OE_ERR_CONFIG, "frequency must be positive");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(OE_ERR_DEVICE, "no HIP device");
+    if (device < 0 || device >= count) return fail(OE_ERR_DEVICE, "device %d out of range", device);
+    std::unique_ptr<oe_nf2ff> nf(new oe_nf2ff());
+    nf->device = device;
+    nf->freq = frequency;
+    HIP_OK(hipSetDevice(device));
+    HIP_OK(hipStreamCreateWithFlags(&nf->stream, hipStreamNonBlocking));
+    *out = nf.release();
+    return OE_OK;
+}
+
+void oe_nf2ff_destroy(oe_nf2ff* nf) { delete nf; }
+
+oe_status oe_nf2ff_add_surface(oe_nf2ff* nf, int normal_dir, int sign, double position, const double* c1, size_t n1,
+                               const double* c2, size_t n2, const double* e_t, const double* h_t) {
+    if (!nf || !c1 || !c2 || !e_t || !h_t) return fail(OE_ERR_CONFIG, "null argument");
+    if (normal_dir < 0 || normal_dir > 2 || (sign != 1 && sign != -1)) return fail(OE_ERR_CONFIG, "bad normal");
+    if (n1 == 0 || n2 == 0 || n1 * n2 > (size_t)INT32_MAX / 16) return fail(OE_ERR_CONFIG, "bad surface size");
+    const size_t nn = n1 * n2;
+    std::vector<double> v(8 * nn);  // [n1][n2][4] complex: E_a1, E_a2, H_a1, H_a2
+    for (size_t q = 0; q < nn; ++q)
+        for (int c = 0; c < 2; ++c) {
+            v[8 * q + 2 * c] = e_t[2 * (c * nn + q)];
+            v[8 * q + 2 * c + 1] = e_t[2 * (c * nn + q) + 1];
+            v[8 * q + 4 + 2 * c] = h_t[2 * (c * nn + q)];
+            v[8 * q + 4 + 2 * c + 1] = h_t[2 * (c * nn + q) + 1];
+        }
+    HIP_OK(hipSetDevice(nf->device));
+    std::unique_ptr<DevBuf> b(new DevBuf());
+    HIP_OK(b->alloc(v.size() * sizeof(double)));
+    HIP_OK(hipMemcpy(b->p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    Nf2ffFace f{};
+    f.axis = normal_dir;
+    f.sign = sign;
+    f.c0 = position;
+    f.c1.assign(c1, c1 + n1);
+    f.c2.assign(c2, c2 + n2);
+    f.src = b->p;
+    f.src_kind = 0;
+    f.scale = 1.0;
+    nf->faces.push_back(std::move(f));
+    nf->bufs.push_back(std::move(b));
+    return OE_OK;
+}
+
+oe_status oe_nf2ff_calculate(oe_nf2ff* nf, const double* theta, size_t nt, const double* phi, size_t np,
+                             double radius, const double center[3], oe_far_field* out) {
+    if (!nf) return fail(OE_ERR_CONFIG, "null argument");
+    OE_TRY(angles_ok(theta, nt, phi, np, radius, out));
+    if (nf->faces.empty()) return fail(OE_ERR_CONFIG, "no surfaces added");
+    const double zero[3] = {0, 0, 0};
+    HIP_OK(hipSetDevice(nf->device));
+    Nf2ffOut o = far_field_in(out);
+    HIP_OK(nf2ff_compute(nf->faces, nf->freq, theta, (int)nt, phi, (int)np, radius, center ? center : zero,
+                         nf->stream, &o));
+    far_field_out(o, out);
+    return OE_OK;
+}
+
 static oe_status probe_reserve(oe_extension* x, uint64_t need) {
     if (need <= x->probe_cap) return OE_OK;
     oe_engine* e = x->eng;
@@ -1140,6 +1338,7 @@ oe_status oe_extension_reset(oe_extension* x) {
         void* acc = x->dft.prec == OE_DFT_FP64 ? (void*)x->dft.acc64 : (void*)x->dft.acc32;
         HIP_OK(hipMemsetAsync(acc, 0, (size_t)x->dft.n_f * x->dft.n_sig * 16, e->stream));
         x->dft_count = 0;
+        x->dft_seen = 0;
     }
     HIP_OK(hipStreamSynchronize(e->stream));
     return OE_OK;
@@ -1303,6 +1502,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
                                     e->probe_table.as<PointProbe>(), (int)pp.size(), e->stream));
         }
         for (auto* x : dfts) {
+            if (x->dft_seen++ % x->dft_stride != 0) continue;
             Timed tm(e, "dft");
             HIP_OK(launch_dft(e->F, e->g.N, x->dft, x->dft_count, (double)(cur + 1) * e->dt, ((double)cur + 0.5) * e->dt,
                               e->stream));

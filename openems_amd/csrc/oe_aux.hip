@@ -233,32 +233,53 @@ __global__ void k_ade_post(float* __restrict__ F, const float* __restrict__ Cf, 
 // One thread per signal: gather the float64 signal, record it, and update
 // the running sums of every frequency.  FP32_KAHAN keeps float32 sums with
 // Kahan compensation; the phase is evaluated in float64 in both modes.
+// Each work-group first tabulates exp(-j 2 pi f t) for the E and H sample
+// times of a chunk of frequencies in LDS, so a thread's inner loop over
+// frequencies is two loads and the accumulator update.
+constexpr int DFT_FCHUNK = 128;
+
 __global__ __launch_bounds__(256) void k_dft(const float* __restrict__ F, size_t N, DftDev d, uint64_t sample,
                                             double tE, double tH) {
+    __shared__ double ph[2][DFT_FCHUNK][2];  // [E, H][f][cos, -sin]
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= d.n_sig) return;
+    const bool act = m < d.n_sig;
     double s = 0.0;
-    for (uint64_t t = d.off[m]; t < d.off[m + 1]; ++t) s += d.w[t] * (double)F[(size_t)d.comp[t] * N + d.cell[t]];
-    if (d.series && sample < d.series_cap) d.series[sample * d.n_sig + m] = s;
-    const double tt = d.is_h[m] ? tH : tE;
-    for (int f = 0; f < d.n_f; ++f) {
-        double sn, cs;
-        sincos(2.0 * M_PI * d.freqs[f] * tt, &sn, &cs);
-        const size_t q = (size_t)f * d.n_sig + m;
-        if (d.prec == 0) {
-            d.acc64[2 * q] += s * cs;
-            d.acc64[2 * q + 1] += s * (-sn);
-        } else {
-            const float sv = (float)s;
-            float* a = d.acc32 + 4 * q;
-            const float yr = sv * (float)cs - a[2];
-            const float tr = a[0] + yr;
-            a[2] = (tr - a[0]) - yr;
-            a[0] = tr;
-            const float yi = sv * (float)(-sn) - a[3];
-            const float ti = a[1] + yi;
-            a[3] = (ti - a[1]) - yi;
-            a[1] = ti;
+    int h = 0;
+    if (act) {
+        for (uint64_t t = d.off[m]; t < d.off[m + 1]; ++t) s += d.w[t] * (double)F[(size_t)d.comp[t] * N + d.cell[t]];
+        if (d.series && sample < d.series_cap) d.series[sample * d.n_sig + m] = s;
+        h = d.is_h[m] ? 1 : 0;
+    }
+    for (int f0 = 0; f0 < d.n_f; f0 += DFT_FCHUNK) {
+        const int nf = min(DFT_FCHUNK, d.n_f - f0);
+        __syncthreads();
+        for (int q = threadIdx.x; q < 2 * nf; q += blockDim.x) {
+            const int hq = q / nf, f = q - hq * nf;
+            double sn, cs;
+            sincos(2.0 * M_PI * d.freqs[f0 + f] * (hq ? tH : tE), &sn, &cs);
+            ph[hq][f][0] = cs;
+            ph[hq][f][1] = -sn;
+        }
+        __syncthreads();
+        if (!act) continue;
+        for (int f = 0; f < nf; ++f) {
+            const double cr = ph[h][f][0], ci = ph[h][f][1];
+            const size_t q = (size_t)(f0 + f) * d.n_sig + m;
+            if (d.prec == 0) {
+                d.acc64[2 * q] += s * cr;
+                d.acc64[2 * q + 1] += s * ci;
+            } else {
+                const float sv = (float)s;
+                float* a = d.acc32 + 4 * q;
+                const float yr = sv * (float)cr - a[2];
+                const float tr = a[0] + yr;
+                a[2] = (tr - a[0]) - yr;
+                a[0] = tr;
+                const float yi = sv * (float)ci - a[3];
+                const float ti = a[1] + yi;
+                a[3] = (ti - a[1]) - yi;
+                a[1] = ti;
+            }
         }
     }
 }

@@ -1,10 +1,12 @@
 // oe_internal.h -- types shared by the host runtime (oe_api.hip) and the
-// gfx950 kernels (oe_yee.hip, oe_aux.hip).  Not part of the public ABI.
+// gfx950 kernels (oe_yee.hip, oe_aux.hip, oe_nf2ff.hip).  Not part of the
+// public ABI.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 namespace oe {
 
@@ -12,6 +14,22 @@ constexpr double C0 = 299792458.0;
 constexpr double EPS0 = 8.854187817e-12;
 constexpr double MU0 = 1.256637062e-6;
 constexpr double Z0 = 376.730313668;
+
+// Owned device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    hipError_t alloc(size_t n) {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        bytes = n;
+        return n ? hipMalloc(&p, n) : hipSuccess;
+    }
+    template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
 
 // Field block: one allocation of 6 arrays of N floats, E(x,y,z) then H(x,y,z).
 // Coefficient block: 12 arrays of N floats, Ca(x,y,z) Cb(x,y,z) Da(x,y,z) Db(x,y,z).
@@ -150,5 +168,33 @@ struct DftDev {
 };
 hipError_t launch_dft(const float* F, size_t N, const DftDev& d, uint64_t sample, double tE, double tH,
                       hipStream_t s);
+
+// NF2FF (oe_nf2ff.hip).  One face of the closed surface: normal `axis`,
+// outward sign, position c0 and node coordinates c1, c2 along the in-plane
+// axes a1 = (axis+1)%3, a2 = (axis+2)%3.  Its phasors sit on the device as
+// [n1][n2][4] complex values (E_a1, E_a2, H_a1, H_a2) in one of two layouts:
+// src_kind 0 = float64 (re, im) pairs, 1 = float32 Kahan quads (sum re,
+// sum im, compensation re, im); every value is multiplied by `scale`.
+struct Nf2ffFace {
+    int axis, sign;
+    double c0;
+    std::vector<double> c1, c2;
+    const void* src;
+    int src_kind;
+    double scale;
+};
+struct Nf2ffOut {
+    double* et_re;  // [n_theta][n_phi] each, or null
+    double* et_im;
+    double* ep_re;
+    double* ep_im;
+    double prad, dmax;
+    double gemm_ms, total_ms, gemm_flop;
+};
+// Far field at distance r for every (theta, phi) (matlab/AnalyzeNF2FF.m).
+// Fails with hipErrorInvalidValue on empty input.
+hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const double* theta, int n_theta,
+                         const double* phi, int n_phi, double r, const double center[3], hipStream_t s,
+                         Nf2ffOut* out);
 
 }  // namespace oe

@@ -217,6 +217,12 @@ class Extension:
                                     im.ctypes.data_as(C.POINTER(C.c_double))))
         return re + 1j * im
 
+    # NF2FF box
+    def far_field(self, freq_index: int, theta, phi, radius: float = 1.0, center=(0.0, 0.0, 0.0)) -> "FarFieldResult":
+        """Nf2ff::calculate on the recorded surface phasors of freqs[freq_index]."""
+        return _far_field(lambda th, nt, ph, np_, r, c, out: lib().oe_ext_nf2ff_calculate(
+            self._h, int(freq_index), th, nt, ph, np_, r, c, out), self._freqs[freq_index], theta, phi, radius, center)
+
     # probes
     def data(self) -> np.ndarray:
         n = C.c_size_t()
@@ -225,6 +231,77 @@ class Extension:
         if n.value:
             check(lib().oe_ext_probe_read(self._h, out.ctypes.data_as(C.POINTER(C.c_double)), n.value, C.byref(n)))
         return out
+
+
+@dataclass
+class FarFieldResult:
+    """FarFieldResult (src/nf2ff/mod.rs:11-24) plus the radiated power, the
+    maximum directivity and the device timing of the transform."""
+    theta: np.ndarray
+    phi: np.ndarray
+    e_theta: np.ndarray  # [n_theta, n_phi] complex
+    e_phi: np.ndarray
+    frequency: float
+    prad: float
+    dmax: float
+    gemm_ms: float
+    total_ms: float
+    gemm_flop: float
+
+    def directivity(self) -> np.ndarray:
+        """|E|^2 normalised to its maximum (FarFieldResult::directivity, src/nf2ff/mod.rs:28-55)."""
+        p = np.abs(self.e_theta) ** 2 + np.abs(self.e_phi) ** 2
+        m = p.max()
+        return p / m if m > 0 else np.zeros_like(p)
+
+    def max_directivity(self) -> float:
+        return self.dmax
+
+
+def _far_field(call, frequency, theta, phi, radius, center) -> FarFieldResult:
+    th = np.ascontiguousarray(theta, dtype=np.float64).ravel()
+    ph = np.ascontiguousarray(phi, dtype=np.float64).ravel()
+    shape = (th.size, ph.size)
+    bufs = [np.empty(shape, np.float64) for _ in range(4)]
+    dp = C.POINTER(C.c_double)
+    out = _ffi.FarField(*[b.ctypes.data_as(dp) for b in bufs], 0.0, 0.0, 0.0, 0.0, 0.0)
+    c = (C.c_double * 3)(*map(float, center))
+    check(call(th.ctypes.data_as(dp), th.size, ph.ctypes.data_as(dp), ph.size, float(radius), c, C.byref(out)))
+    return FarFieldResult(th, ph, bufs[0] + 1j * bufs[1], bufs[2] + 1j * bufs[3], float(frequency), out.prad, out.dmax,
+                          out.gemm_ms, out.total_ms, out.gemm_flop)
+
+
+class Nf2ff:
+    """Nf2ff (src/nf2ff/mod.rs:72-154) on a HIP device: surfaces of tangential
+    E/H phasors in, far field out."""
+
+    def __init__(self, frequency: float, device: int = 0):
+        h = C.c_void_p()
+        check(lib().oe_nf2ff_create(int(device), float(frequency), C.byref(h)))
+        self._h = h
+        self.frequency = float(frequency)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().oe_nf2ff_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def add_surface(self, normal_dir: int, sign: int, position: float, c1, c2, e_t, h_t):
+        """e_t, h_t: complex [2, len(c1), len(c2)], the phasors along the
+        in-plane axes (normal_dir+1)%3 and (normal_dir+2)%3."""
+        c1 = np.ascontiguousarray(c1, dtype=np.float64)
+        c2 = np.ascontiguousarray(c2, dtype=np.float64)
+        shape = (2, c1.size, c2.size)
+        e = np.ascontiguousarray(np.broadcast_to(e_t, shape), dtype=np.complex128)
+        hh = np.ascontiguousarray(np.broadcast_to(h_t, shape), dtype=np.complex128)
+        dp = C.POINTER(C.c_double)
+        check(lib().oe_nf2ff_add_surface(self._h, int(normal_dir), int(sign), float(position), c1.ctypes.data_as(dp),
+                                         c1.size, c2.ctypes.data_as(dp), c2.size,
+                                         e.view(np.float64).ctypes.data_as(dp), hh.view(np.float64).ctypes.data_as(dp)))
+
+    def calculate(self, theta, phi, radius: float = 1.0, center=(0.0, 0.0, 0.0)) -> FarFieldResult:
+        return _far_field(lambda th, nt, ph, np_, r, c, out: lib().oe_nf2ff_calculate(self._h, th, nt, ph, np_, r, c, out),
+                          self.frequency, theta, phi, radius, center)
 
 
 def port_excitations(direction: int, region, samples: np.ndarray, sign: int = 1, amplitude: float = 1.0):
@@ -365,7 +442,7 @@ class HipEngine:
                                       C.byref(h)))
         return Extension(self, h.value, "ade")
 
-    def dft(self, terms, freqs, precision: int = 0, record_steps: int = 0) -> Extension:
+    def dft(self, terms, freqs, precision: int = 0, record_steps: int = 0, time_stride: int = 1) -> Extension:
         """Running DFT of signals; terms: per signal a list of (component 0..5, i, j, k, weight)."""
         nx, ny, nz = self.shape
         off = np.zeros(len(terms) + 1, np.uint64)
@@ -383,7 +460,7 @@ class HipEngine:
         cfg = _ffi.DftConfig(len(terms), off.ctypes.data_as(C.POINTER(C.c_uint64)),
                              comp.ctypes.data_as(C.POINTER(C.c_uint32)), cell.ctypes.data_as(C.POINTER(C.c_uint64)),
                              w.ctypes.data_as(C.POINTER(C.c_double)), fr.ctypes.data_as(C.POINTER(C.c_double)),
-                             len(fr), int(precision), int(record_steps))
+                             len(fr), int(precision), int(record_steps), int(time_stride))
         h = C.c_void_p()
         check(lib().oe_ext_dft_create(self._h, C.byref(cfg), C.byref(h)))
         x = Extension(self, h.value, "dft")
@@ -401,6 +478,22 @@ class HipEngine:
                                               int(record_steps), C.byref(h)))
         x = Extension(self, h.value, "port")
         x._dft_shape = (len(fr), 2)
+        return x
+
+    def nf2ff_box(self, box, freqs, directions: int = 0, precision: int = 0, time_stride: int = 1) -> Extension:
+        """NF2FF recording box through the mesh nodes box = (i0, i1, j0, j1, k0, k1),
+        inclusive (openEMS CreateNF2FFBox with frequency-domain recording)."""
+        b = (C.c_size_t * 6)(*map(int, box))
+        fr = np.ascontiguousarray(freqs, dtype=np.float64)
+        h = C.c_void_p()
+        check(lib().oe_ext_nf2ff_create(self._h, b, int(directions), fr.ctypes.data_as(C.POINTER(C.c_double)), len(fr),
+                                        int(precision), int(time_stride), C.byref(h)))
+        x = Extension(self, h.value, "nf2ff")
+        lo, hi = np.array(box[0::2]), np.array(box[1::2])
+        nodes = sum(int(np.prod([(hi[a] - lo[a] + 1) for a in ((ax + 1) % 3, (ax + 2) % 3)]))
+                    for ax in range(3) for side in range(2) if (directions or 0x3F) >> (2 * ax + side) & 1)
+        x._dft_shape = (len(fr), 4 * nodes)
+        x._freqs = fr
         return x
 
     def field_probe(self, pos) -> Extension:

@@ -1,0 +1,185 @@
+"""GPU NF2FF: the matrix-core transform against the oracle's float64
+restatement of matlab/AnalyzeNF2FF.m, and the recording box end to end.
+
+Tolerances (stated here, reported on every run):
+  far-field patterns  rel L2 <= 1e-5 against the float64 oracle (the
+                      contraction runs on FP32 matrix cores; phases, the
+                      second-axis reduction and P_rad are float64);
+  P_rad               rel 1e-12 (float64 throughout);
+  surface phasors     rel 1e-11 of the maximum against a float64 DFT of the
+                      oracle's signal series (FP64 running DFT);
+  FP32-Kahan DFT      far field within rel L2 1e-5 of the FP64-DFT one.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from openems_amd import MODE_PHYSICAL, CpmlBoundaries, EngineBatch, HipEngine, Nf2ff, Operator, ScheduledExcitation
+from oracle import excitation as X
+from oracle import nf2ff as N
+from oracle import oracle as O
+from oracle import port as P
+from tests.parity_util import say
+
+pytestmark = pytest.mark.gpu
+
+PATTERN_TOL = 1e-5
+
+
+def _rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _gpu_far_field(faces, f, th, ph, r=1.0, center=(0, 0, 0)):
+    nf = Nf2ff(f)
+    for fc in faces:
+        a1, a2 = (fc.axis + 1) % 3, (fc.axis + 2) % 3
+        nf.add_surface(fc.axis, fc.sign, fc.c0, fc.c1, fc.c2, fc.E[[a1, a2]], fc.H[[a1, a2]])
+    return nf.calculate(th, ph, r, center)
+
+
+def _check(name, faces, f, th, ph, r=1.0, center=(0, 0, 0)):
+    Et, Ep, prad, dmax = N.far_field(faces, f, th, ph, r=r, center=center)
+    g = _gpu_far_field(faces, f, th, ph, r, center)
+    et, ep = _rel_l2(g.e_theta, Et), _rel_l2(g.e_phi, Ep)
+    tot = math.sqrt((np.linalg.norm(g.e_theta - Et) ** 2 + np.linalg.norm(g.e_phi - Ep) ** 2)
+                    / (np.linalg.norm(Et) ** 2 + np.linalg.norm(Ep) ** 2))
+    say(f"[nf2ff] {name}: rel L2 E_theta {et:.2e} E_phi {ep:.2e} |E| {tot:.2e}; "
+        f"P_rad {g.prad:.6e} vs {prad:.6e}; D_max {g.dmax:.6f} vs {dmax:.6f}; "
+        f"gemm {g.gemm_ms:.3f} ms of {g.total_ms:.3f} ms")
+    assert tot <= PATTERN_TOL
+    assert abs(g.prad - prad) <= 1e-12 * abs(prad)
+    assert abs(g.dmax - dmax) <= 1e-5 * abs(dmax)
+    return g
+
+
+def test_hertzian_dipole_surfaces():
+    f = 3e9
+    lam = N.C0 / f
+    faces = N.box_faces([-0.5 * lam] * 3, [0.5 * lam] * 3, (41, 37, 45),
+                        lambda X_, Y, Z: N.hertzian_dipole_fields(1e-3, f, X_, Y, Z))
+    th = np.linspace(0, math.pi, 37)
+    ph = np.linspace(0, 2 * math.pi, 73)[:-1]
+    g = _check("hertzian dipole 41x37x45", faces, f, th, ph)
+    assert abs(g.dmax - 1.5) < 5e-3
+
+
+def test_random_phasors_nonuniform_offset_centre():
+    rng = np.random.default_rng(11)
+    lo, hi = [-0.03, -0.02, -0.025], [0.02, 0.035, 0.03]
+    steps = (19, 70, 33)
+    axes = [np.sort(rng.uniform(lo[a], hi[a], steps[a])) for a in range(3)]
+    for a in range(3):
+        axes[a][0], axes[a][-1] = lo[a], hi[a]
+    faces = []
+    for axis in range(3):
+        a1, a2 = (axis + 1) % 3, (axis + 2) % 3
+        for sign, c0 in ((-1, lo[axis]), (1, hi[axis])):
+            shape = (3, steps[a1], steps[a2])
+            E = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+            H = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / N.Z0
+            faces.append(N.Face(axis, sign, c0, axes[a1], axes[a2], E, H))
+    th = np.linspace(0.05, math.pi - 0.05, 23)
+    ph = np.linspace(0, 2 * math.pi, 41)
+    _check("random phasors 19x70x33", faces, 8e9, th, ph, r=3.0, center=(0.004, -0.002, 0.001))
+
+
+def test_angle_chunks_and_wide_faces():
+    """z faces 6 x 300 nodes: G rows are split over two GEMM launches at
+    181 x 180 angles; the oracle checks every 20th theta row."""
+    f = 10e9
+    lam = N.C0 / f
+    faces = N.box_faces([-0.1 * lam, -2 * lam, -0.08 * lam], [0.1 * lam, 2 * lam, 0.08 * lam], (6, 300, 5),
+                        lambda X_, Y, Z: N.hertzian_dipole_fields(1e-3, f, X_ + 0.001, Y, Z))
+    th = np.linspace(0, math.pi, 181)
+    ph = np.linspace(0, 2 * math.pi, 181)[:-1]
+    g = _gpu_far_field(faces, f, th, ph)
+    sub = slice(0, 181, 20)
+    Et, Ep, prad, _ = N.far_field(faces, f, th[sub], ph)
+    err = math.sqrt((np.linalg.norm(g.e_theta[sub] - Et) ** 2 + np.linalg.norm(g.e_phi[sub] - Ep) ** 2)
+                    / (np.linalg.norm(Et) ** 2 + np.linalg.norm(Ep) ** 2))
+    say(f"[nf2ff] 181x180 angles, 6x300x5 box: rel L2 {err:.2e} on every 20th theta; gemm {g.gemm_ms:.3f} ms")
+    assert err <= PATTERN_TOL
+    assert abs(g.prad - prad) <= 1e-12 * abs(prad)
+
+
+def _fdtd_dipole(precision, n=48, steps=900, stride=3):
+    d = 1e-3
+    op = Operator.uniform(n, n, n, d)
+    eng = HipEngine(op)
+    cp = eng.cpml(CpmlBoundaries(layers=(8,) * 6, mode=MODE_PHYSICAL))
+    box = (12, 36, 12, 36, 12, 36)
+    freqs = [8e9, 10e9, 12e9]
+    nf = eng.nf2ff_box(box, freqs, precision=precision, time_stride=stride)
+    t0, tau = X.openems_gaussian_params(5e9)
+    smp = X.schedule(lambda t: X.gaussian_modulated(t, 10e9, t0, tau), 0, steps, op.dt)
+    src = (24, 24, 24)
+    eng.run_batch(EngineBatch(num_steps=steps, excitations=[ScheduledExcitation(src, 2, smp)], extensions=[cp, nf]))
+    return eng, nf, freqs, op.dt, smp, box
+
+
+def test_fdtd_dipole_box_against_oracle():
+    n, steps, stride = 48, 900, 3
+    say(f"[nf2ff] fdtd dipole {n}^3, {steps} steps, stride {stride}")
+    eng, nf, freqs, dt, smp, box = _fdtd_dipole(0, n, steps, stride)
+    terms, faces = N.surface_terms((n, n, n), (1e-3,) * 3, box)
+    # surface phasors against the oracle's signals
+    sim = O.OracleSim(n, n, n, 1e-3)
+    sim.add_cpml((8,) * 6, mode=O.MODE_PHYSICAL)
+    sim.add_signals(terms)
+    sim.run_batch(steps, [(24, 24, 24, 2, True)], smp.astype(np.float64)[None])
+    E, H = eng.read_fields()
+    assert np.array_equal(np.concatenate([E, H]), sim.fields())
+    ser = sim.signal_series(0, len(terms))[::stride]
+    st = np.arange(0, steps, stride)
+    is_h = np.array([sig[0][0] >= 3 for sig in terms])
+    ref = np.empty((len(freqs), len(terms)), np.complex128)
+    ref[:, ~is_h] = P.dft(ser[:, ~is_h], (st + 1) * dt, freqs)  # sample spacing stride * dt
+    ref[:, is_h] = P.dft(ser[:, is_h], (st + 0.5) * dt, freqs)
+    got = nf.dft_values()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    say(f"[nf2ff] surface phasors: max |gpu - oracle| / max = {err:.2e} over {len(terms)} signals")
+    assert err < 1e-11
+    # far field of the GPU's phasors: transform parity and the dipole pattern
+    th = np.linspace(0, math.pi, 37)
+    ph = np.linspace(0, 2 * math.pi, 37)[:-1]
+    for q, f in enumerate(freqs):
+        g = nf.far_field(q, th, ph)
+        Et, Ep, prad, dmax = N.far_field(N.faces_from_phasors(got[q], (1e-3,) * 3, faces), f, th, ph)
+        e = math.sqrt((np.linalg.norm(g.e_theta - Et) ** 2 + np.linalg.norm(g.e_phi - Ep) ** 2)
+                      / (np.linalg.norm(Et) ** 2 + np.linalg.norm(Ep) ** 2))
+        say(f"[nf2ff] {f / 1e9:.0f} GHz: rel L2 {e:.2e}; P_rad {g.prad:.4e}; D_max {g.dmax:.4f} (oracle {dmax:.4f}, "
+            f"Hertzian 1.5)")
+        assert e <= PATTERN_TOL
+        assert abs(g.prad - prad) <= 1e-12 * abs(prad)
+        assert g.prad > 0
+        # physics: a single-edge source is a Hertzian dipole.  The pattern is
+        # sin(theta) and its sphere-integrated directivity 1.5; D_max divides
+        # by the surface Poynting flux, which the absorber's residual
+        # reflections in this small box move by a few percent (the oracle's
+        # own phasors give the same D_max).
+        pat = np.abs(g.e_theta[:, 0])
+        assert np.max(np.abs(pat / pat.max() - np.sin(th))) < 0.03
+        U = np.abs(g.e_theta) ** 2 + np.abs(g.e_phi) ** 2
+        p_int = np.trapezoid(U.mean(axis=1) * 2 * math.pi * np.sin(th), th)
+        d_int = 4 * math.pi * U.max() / p_int
+        say(f"[nf2ff] {f / 1e9:.0f} GHz: pattern-integrated directivity {d_int:.4f}")
+        assert abs(d_int - 1.5) < 0.01
+        assert abs(g.dmax - 1.5) < 0.2
+
+
+def test_fdtd_dipole_fp32_kahan_far_field():
+    _, a, freqs, _, _, _ = _fdtd_dipole(0)
+    _, b, _, _, _, _ = _fdtd_dipole(1)
+    th = np.linspace(0, math.pi, 19)
+    ph = np.linspace(0, 2 * math.pi, 19)[:-1]
+    for q in range(len(freqs)):
+        ga = a.far_field(q, th, ph)
+        gb = b.far_field(q, th, ph)
+        e = math.sqrt((np.linalg.norm(gb.e_theta - ga.e_theta) ** 2 + np.linalg.norm(gb.e_phi - ga.e_phi) ** 2)
+                      / (np.linalg.norm(ga.e_theta) ** 2 + np.linalg.norm(ga.e_phi) ** 2))
+        say(f"[nf2ff] fp32-kahan vs fp64 DFT, {freqs[q] / 1e9:.0f} GHz: rel L2 {e:.2e}, "
+            f"P_rad rel {abs(gb.prad / ga.prad - 1):.2e}")
+        assert e <= PATTERN_TOL
+        assert abs(gb.prad / ga.prad - 1) < 1e-5
