@@ -167,6 +167,76 @@ def cpu_baseline(wl, budget_s=12.0, max_steps=4000):
     }
 
 
+def bench_nf2ff(args):
+    """NF2FF transform of the 384x384x256 workload's recording box: nodes
+    18..366 x 18..366 x 18..238 (10 cells inside the 8-cell CPML), 0.5 mm
+    cells, 181 x 360 (theta, phi), frequencies 5, 10, 15 GHz, seeded
+    synthetic surface phasors already resident on the device."""
+    import numpy as np
+    import torch
+
+    from openems_amd import Nf2ff
+
+    d = 0.5e-3
+    lo, hi = (18, 18, 18), (366, 366, 238)
+    axes = [np.arange(lo[a], hi[a] + 1) * d for a in range(3)]
+    rng = np.random.default_rng(7)
+    th = np.linspace(0, np.pi, 181)
+    ph = np.linspace(0, 2 * np.pi, 361)[:-1]
+    freqs = (5e9, 10e9, 15e9)
+    nfs = []
+    for f in freqs:
+        nf = Nf2ff(f)
+        for ax in range(3):
+            a1, a2 = (ax + 1) % 3, (ax + 2) % 3
+            for side in (0, 1):
+                shape = (2, axes[a1].size, axes[a2].size)
+                e = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+                h = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / 376.73
+                nf.add_surface(ax, 1 if side else -1, (hi if side else lo)[ax] * d, axes[a1], axes[a2], e, h)
+        nfs.append(nf)
+    log(f"[bench] nf2ff: box {[a.size for a in axes]} nodes, {th.size}x{ph.size} angles, {len(freqs)} frequencies")
+    for _ in range(args.warmup):
+        for nf in nfs:
+            nf.calculate(th, ph)
+    torch.cuda.synchronize()
+    gemm_ms = total_ms = flop = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for nf in nfs:
+            r = nf.calculate(th, ph)
+            gemm_ms += r.gemm_ms
+            total_ms += r.total_ms
+            flop += r.gemm_flop
+    wall = time.perf_counter() - t0
+    n_calc = args.steps * len(freqs)
+    tflops = flop / (gemm_ms * 1e-3) / 1e12
+    out = {
+        "metric": "NF2FF far-field transform, ms per frequency (181x360 angles)",
+        "value": round(wall / n_calc * 1e3, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (matrix cores) / f64 (phases, reduction, P_rad)",
+        "data": "synthetic",
+        "config": {"workload": "nf2ff box of the 384x384x256 CPML workload: 349x349x221 surface nodes, "
+                               "0.5 mm cells, 181x360 angles, 5/10/15 GHz",
+                   "device_ms_per_frequency": round(total_ms / n_calc, 3)},
+        "roofline": {"bound": "mfma", "kernel": "k_sgemm_mfma (v_mfma_f32_32x32x2f32)",
+                     "achieved": round(tflops, 2), "peak": 157.3, "unit": "TFLOP/s",
+                     "frac": round(tflops / 157.3, 4), "traffic": None,
+                     "flop_per_frequency": flop / n_calc,
+                     "gemm_ms_per_frequency": round(gemm_ms / n_calc, 3)},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,6 +251,8 @@ def main():
     ap.add_argument("--tiles", type=str, default="", help="bx,by,ci,vec override")
     ap.add_argument("--summarize-rocprof", nargs="+", metavar="DB",
                     help="print a markdown summary of rocprofv3 result databases and exit")
+    ap.add_argument("--nf2ff", action="store_true",
+                    help="time the NF2FF transform of the 384x384x256 workload's box instead (steps = repetitions)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 10 device copies of 1 GiB (torch copy_), to scale HBM byte counters")
     args = ap.parse_args()
@@ -204,6 +276,10 @@ def main():
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.nf2ff:
+        bench_nf2ff(args)
+        return
 
     import torch
 

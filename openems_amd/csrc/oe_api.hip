@@ -105,6 +105,7 @@ struct oe_extension {
     bool is_nf2ff = false;
     std::vector<Nf2ffFace> nf_faces;
     std::vector<size_t> nf_sig0;
+    Nf2ffCachePtr nf_cache;
 };
 
 // ------------------------------------------------------------------ engine
@@ -1166,8 +1167,9 @@ oe_status oe_ext_nf2ff_calculate(oe_extension* x, size_t fi, const double* theta
         fc.scale = 2.0 * e->dt * x->dft_stride;  // pulse convention (python/openEMS/utilities.py:29-35)
     }
     Nf2ffOut o = far_field_in(out);
+    if (!x->nf_cache) x->nf_cache.reset(nf2ff_cache_new());
     HIP_OK(nf2ff_compute(x->nf_faces, x->dft_freqs[fi], theta, (int)nt, phi, (int)np, radius, center ? center : zero,
-                         e->stream, &o));
+                         e->stream, x->nf_cache.get(), &o));
     far_field_out(o, out);
     return OE_OK;
 }
@@ -1178,6 +1180,7 @@ struct oe_nf2ff {
     double freq = 0.0;
     std::vector<Nf2ffFace> faces;
     std::vector<std::unique_ptr<DevBuf>> bufs;
+    Nf2ffCachePtr cache{nf2ff_cache_new()};
     ~oe_nf2ff() {
         if (stream) {
             (void)hipSetDevice(device);
@@ -1245,7 +1248,7 @@ oe_status oe_nf2ff_calculate(oe_nf2ff* nf, const double* theta, size_t nt, const
     HIP_OK(hipSetDevice(nf->device));
     Nf2ffOut o = far_field_in(out);
     HIP_OK(nf2ff_compute(nf->faces, nf->freq, theta, (int)nt, phi, (int)np, radius, center ? center : zero,
-                         nf->stream, &o));
+                         nf->stream, nf->cache.get(), &o));
     far_field_out(o, out);
     return OE_OK;
 }

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 namespace oe {
@@ -96,6 +97,31 @@ enum { AX_X = 1, AX_Y = 2, AX_Z = 4 };
 // Kernel launchers (oe_yee.hip).
 hipError_t launch_update_h(const YeeArgs& a, const CpmlArgs* c, int axes, hipStream_t s);
 hipError_t launch_update_e(const YeeArgs& a, const CpmlArgs* c, int axes, hipStream_t s);
+
+// Sweep: many structures per launch.  A structure is its grid, field and
+// coefficient blocks and CPML state; a box is a part of one structure with
+// that structure's tile shape; wg_box maps every work-group to its box.
+struct SweepStruct {
+    Grid g;
+    float* F;
+    const float* Cf;
+    CpmlArgs c;
+};
+struct SweepBox {
+    int sid;
+    int i0, i1, j0, j1;
+    int n_jb, n_kb;
+    int bx, by, ci;
+    int first;  // first work-group of the box
+};
+struct SweepArgs {
+    const SweepStruct* st;
+    const SweepBox* box;
+    const int* wg_box;
+    int xcd_remap;
+};
+// h: H (else E) half-step over n_wg work-groups; axes 0, AX_Z or all faces.
+hipError_t launch_sweep(bool h, const SweepArgs& a, int n_wg, int axes, int vec, hipStream_t s);
 
 // Auxiliary kernels (oe_aux.hip).
 hipError_t launch_fill(float* p, size_t n, float v, hipStream_t s);
@@ -191,10 +217,20 @@ struct Nf2ffOut {
     double prad, dmax;
     double gemm_ms, total_ms, gemm_flop;
 };
+// Angle plan (distinct direction cosines per axis) and work buffers of the
+// transform, reused by later calls with the same angles.
+struct Nf2ffCache;
+Nf2ffCache* nf2ff_cache_new();
+void nf2ff_cache_free(Nf2ffCache* c);
+struct Nf2ffCacheDeleter {
+    void operator()(Nf2ffCache* c) const { nf2ff_cache_free(c); }
+};
+typedef std::unique_ptr<Nf2ffCache, Nf2ffCacheDeleter> Nf2ffCachePtr;
+
 // Far field at distance r for every (theta, phi) (matlab/AnalyzeNF2FF.m).
 // Fails with hipErrorInvalidValue on empty input.
 hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const double* theta, int n_theta,
                          const double* phi, int n_phi, double r, const double center[3], hipStream_t s,
-                         Nf2ffOut* out);
+                         Nf2ffCache* cache, Nf2ffOut* out);
 
 }  // namespace oe

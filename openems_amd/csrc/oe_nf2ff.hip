@@ -1,23 +1,29 @@
-// oe_nf2ff.hip -- near-field to far-field transform kernels for gfx950.
+// oe_nf2ff.hip -- near-field to far-field transform on gfx950.
 //
-// For one face of the NF2FF box with in-plane node coordinates c1[i1], c2[i2]
-// and complex surface currents B_c(i1, i2) (c = J_a1, J_a2, M_a1, M_a2 times
-// the trapezoid weight along axis 1), the radiation integrals of
-// matlab/AnalyzeNF2FF.m:186-197 for every angle a factor as
-//   G[a, i2, c] = sum_i1 exp(+j k kappa1_a c1[i1]) B_c(i1, i2)      (MFMA GEMM)
-//   S[a, c]     = exp(+j k kappa0_a c0) sum_i2 w2[i2] exp(+j k kappa2_a c2[i2]) G[a, i2, c]
-// where kappa = (sin t cos p, sin t sin p, cos t) is split over the face's
-// normal axis (kappa0) and in-plane axes (kappa1, kappa2).  The complex GEMM
-// is one real GEMM [Ar | Ai] x [[Br, Bi], [-Bi, Br]] = [Gr | Gi] on the FP32
-// matrix cores (v_mfma_f32_32x32x2f32); the phase matrix is generated by its
-// own kernel in float64 and stored as float32.
+// The radiation integrals of matlab/AnalyzeNF2FF.m:186-197 for one face with
+// in-plane axes p, q (node coordinates x_p, x_q, trapezoid weights w_p, w_q)
+// and the four tangential equivalent currents B_c(ip, iq) factor per angle a
+// with direction cosines kappa = (sin t cos f, sin t sin f, cos t) as
+//   G[a, iq, c] = sum_ip exp(+j k kappa_p(a) x_p[ip]) w_p[ip] B_c(ip, iq)
+//   S[a, c]     = exp(+j k kappa_0(a) x_0) sum_iq w_q[iq] exp(+j k kappa_q(a) x_q[iq]) G[a, iq, c].
+// G depends on the angle only through kappa_p, and only through |kappa_p| up
+// to a sign: with C = cos(k |kappa_p| x_p), S = sin(k |kappa_p| x_p) and
+// s = sign(kappa_p), G = (C Br - s S Bi) + j (C Bi + s S Br).  So the first
+// contraction runs once per DISTINCT |kappa_p| (U values) as one real GEMM
+//   [C_u; S_u]_{u} (2U x n_p)  x  [Br | Bi] (n_p x 8 n_q)
+// on the FP32 matrix cores (v_mfma_f32_32x32x2f32), and p is the in-plane
+// axis with the fewer distinct values (|cos t| on a (t, f) grid has at most
+// n_t of them).  The second contraction, the per-angle phase and the
+// projection are float64.
 
 #include "oe_internal.h"
 
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstring>
 #include <memory>
+#include <numeric>
 #include <vector>
 
 #pragma clang fp contract(off)
@@ -27,61 +33,82 @@ namespace {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-constexpr int GBM = 128, GBN = 128, GBK = 16;
+constexpr int GBM = 128, GBN = 128, GBK = 16;  // M, N padding and K step of every GEMM
+constexpr int LDS_PAD = 32;  // row stride = tile + 32 words: the two half-waves read different bank halves
 
-// C[M x N] = A[M x K] * B[K x N], row-major float32; M, N multiples of 128,
-// K a multiple of 16 (the caller pads with zeros).  256 threads: 4 waves,
-// each owning a 64 x 64 sub-tile as 2 x 2 MFMA 32x32 blocks.
+// C[M x N] = A[M x K] * B[K x N], row-major float32; M, N multiples of TILE,
+// K a multiple of 16 (the caller pads with zeros).  256 threads: 4 waves in
+// a 2 x 2 grid, each owning a (TILE/2) x (TILE/2) sub-tile as MB x MB MFMA
+// 32x32 blocks (MB = TILE/64).  The next K step's tiles are loaded into
+// registers while the matrix cores work on the current one.
 // MFMA 32x32x2 operands: lane l supplies A[l % 32][l / 32] and
 // B[l / 32][l % 32]; accumulator register v of lane l holds
 // C[8 (v / 4) + 4 (l / 32) + (v % 4)][l % 32].
+template <int TILE>
 __global__ __launch_bounds__(256) void k_sgemm_mfma(const float* __restrict__ A, const float* __restrict__ B,
                                                    float* __restrict__ C, int M, int N, int K) {
-    __shared__ float As[GBK][GBM + 4];
-    __shared__ float Bs[GBK][GBN + 4];
+    constexpr int MB = TILE / 64;        // MFMA blocks per wave and dimension
+    constexpr int LV = TILE / 64;        // float4 loads per thread and matrix per K step
+    __shared__ float As[GBK][TILE + LDS_PAD];
+    __shared__ __align__(16) float Bs[GBK][TILE + LDS_PAD];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int bm = blockIdx.y * GBM, bn = blockIdx.x * GBN;
-    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-    floatx16 acc[2][2];
+    const int bm = blockIdx.y * TILE, bn = blockIdx.x * TILE;
+    const int wm = (wave >> 1) * (TILE / 2), wn = (wave & 1) * (TILE / 2);
+    // A: thread loads row ar, columns ac .. ac + 4 LV - 1 of the 16-wide K slice
+    const int ar = tid & (TILE - 1), ac = (tid / TILE) * 4 * LV;
+    // B: thread loads row br, columns bc .. bc + 4 LV - 1
+    const int br = tid / (TILE / (4 * LV)), bc = (tid % (TILE / (4 * LV))) * 4 * LV;
+    floatx16 acc[MB][MB];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < MB; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < MB; ++b)
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
 
-    for (int k0 = 0; k0 < K; k0 += GBK) {
-        {  // A tile: 128 rows x 16 columns, 8 floats per thread, stored transposed
-            const int r = tid >> 1, c = (tid & 1) * 8;
-            const float4* src = reinterpret_cast<const float4*>(A + (size_t)(bm + r) * K + k0 + c);
-            const float4 v0 = src[0], v1 = src[1];
-            As[c + 0][r] = v0.x; As[c + 1][r] = v0.y; As[c + 2][r] = v0.z; As[c + 3][r] = v0.w;
-            As[c + 4][r] = v1.x; As[c + 5][r] = v1.y; As[c + 6][r] = v1.z; As[c + 7][r] = v1.w;
+    float4 ra[LV], rb[LV];
+    auto load = [&](int k0) {
+        const float4* sa = reinterpret_cast<const float4*>(A + (size_t)(bm + ar) * K + k0 + ac);
+        const float4* sb = reinterpret_cast<const float4*>(B + (size_t)(k0 + br) * N + bn + bc);
+#pragma unroll
+        for (int q = 0; q < LV; ++q) {
+            ra[q] = sa[q];
+            rb[q] = sb[q];
         }
-        {  // B tile: 16 rows x 128 columns
-            const int r = tid >> 4, c = (tid & 15) * 8;
-            const float4* src = reinterpret_cast<const float4*>(B + (size_t)(k0 + r) * N + bn + c);
-            const float4 v0 = src[0], v1 = src[1];
-            Bs[r][c + 0] = v0.x; Bs[r][c + 1] = v0.y; Bs[r][c + 2] = v0.z; Bs[r][c + 3] = v0.w;
-            Bs[r][c + 4] = v1.x; Bs[r][c + 5] = v1.y; Bs[r][c + 6] = v1.z; Bs[r][c + 7] = v1.w;
+    };
+    load(0);
+    for (int k0 = 0; k0 < K; k0 += GBK) {
+#pragma unroll
+        for (int q = 0; q < LV; ++q) {
+            As[ac + 4 * q + 0][ar] = ra[q].x;
+            As[ac + 4 * q + 1][ar] = ra[q].y;
+            As[ac + 4 * q + 2][ar] = ra[q].z;
+            As[ac + 4 * q + 3][ar] = ra[q].w;
+            *reinterpret_cast<float4*>(&Bs[br][bc + 4 * q]) = rb[q];
         }
         __syncthreads();
+        if (k0 + GBK < K) load(k0 + GBK);
 #pragma unroll
         for (int kk = 0; kk < GBK; kk += 2) {
             const int kr = kk + (lane >> 5);
-            const float a0 = As[kr][wm + (lane & 31)], a1 = As[kr][wm + 32 + (lane & 31)];
-            const float b0 = Bs[kr][wn + (lane & 31)], b1 = Bs[kr][wn + 32 + (lane & 31)];
-            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
+            float av[MB], bv[MB];
+#pragma unroll
+            for (int q = 0; q < MB; ++q) {
+                av[q] = As[kr][wm + 32 * q + (lane & 31)];
+                bv[q] = Bs[kr][wn + 32 * q + (lane & 31)];
+            }
+#pragma unroll
+            for (int mi = 0; mi < MB; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < MB; ++ni)
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[mi], bv[ni], acc[mi][ni], 0, 0, 0);
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int mi = 0; mi < 2; ++mi)
+    for (int mi = 0; mi < MB; ++mi)
 #pragma unroll
-        for (int ni = 0; ni < 2; ++ni)
+        for (int ni = 0; ni < MB; ++ni)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 const int row = bm + wm + mi * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
@@ -90,36 +117,47 @@ __global__ __launch_bounds__(256) void k_sgemm_mfma(const float* __restrict__ A,
             }
 }
 
-// A'[a][i] = cos(k kappa1_a x1[i]) for i < n1, sin(k kappa1_a x1[i - n1]) for
-// n1 <= i < 2 n1, zero elsewhere and for rows a >= n_valid.  Rows are ld
-// floats; total = rows * ld.
-__global__ __launch_bounds__(256) void k_phase(float* __restrict__ Ap, int ld, size_t total, int n_valid, int n1,
-                                               const double* __restrict__ kap, const double* __restrict__ x1,
+// 128 x 128 tiles when they give the launch at least 512 work-groups, else
+// 64 x 64 tiles (4x the work-groups for the short GEMMs of few distinct
+// direction cosines).
+hipError_t launch_gemm(const float* A, const float* B, float* C, int M, int N, int K, hipStream_t s) {
+    if ((M / 128) * (N / 128) >= 512)
+        hipLaunchKernelGGL(k_sgemm_mfma<128>, dim3(N / 128, M / 128), dim3(256), 0, s, A, B, C, M, N, K);
+    else
+        hipLaunchKernelGGL(k_sgemm_mfma<64>, dim3(N / 64, M / 64), dim3(256), 0, s, A, B, C, M, N, K);
+    return hipGetLastError();
+}
+
+// Rows r of A (ld floats, total = rows * ld): u = u0 + r / 2 < u1 gives
+// cos (even r) or sin (odd r) of k uval[u] xp[i] for i < np, zero elsewhere.
+__global__ __launch_bounds__(256) void k_phase(float* __restrict__ A, int ld, size_t total, int u0, int u1, int np,
+                                               const double* __restrict__ uval, const double* __restrict__ xp,
                                                double k) {
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const int a = (int)(t / ld), i = (int)(t - (size_t)a * ld);
+        const int r = (int)(t / ld), i = (int)(t - (size_t)r * ld);
+        const int u = u0 + (r >> 1);
         float v = 0.f;
-        if (a < n_valid && i < 2 * n1) {
-            const int q = i < n1 ? i : i - n1;
+        if (u < u1 && i < np) {
             double sn, cs;
-            sincos(k * kap[a] * x1[q], &sn, &cs);
-            v = (float)(i < n1 ? cs : sn);
+            sincos(k * uval[u] * xp[i], &sn, &cs);
+            v = (float)((r & 1) ? sn : cs);
         }
-        Ap[t] = v;
+        A[t] = v;
     }
 }
 
-// B' = [[Br, Bi], [-Bi, Br]] for one face: row i1 (and n1 + i1), column
-// 4 i2 + c (and 4 n2 + 4 i2 + c) hold w1[i1] times the equivalent current
-// c = (J_a1, J_a2, M_a1, M_a2) at node (i1, i2), with J = n x H and
-// M = -n x E (matlab/AnalyzeNF2FF.m:186-191): J_a1 = -s H_a2, J_a2 = s H_a1,
-// M_a1 = s E_a2, M_a2 = -s E_a1 for the outward normal n = s e_axis.  The
-// block partial of the radiated power, 1/2 sum w1 w2 Re((E x H*) . n)
-// (matlab/AnalyzeNF2FF.m:201-209), goes to part[blockIdx.x].
+// B = [Br | Bi] for one face: row ip, columns 4 iq + c and 4 nq + 4 iq + c
+// hold w_p[ip] times the equivalent current c = (J_a1, J_a2, M_a1, M_a2) at
+// node (i1, i2) (ip = i1, iq = i2, or swapped when swap = 1), with J = n x H
+// and M = -n x E (matlab/AnalyzeNF2FF.m:186-191): J_a1 = -s H_a2,
+// J_a2 = s H_a1, M_a1 = s E_a2, M_a2 = -s E_a1 for the outward normal
+// n = s e_axis.  The block partial of the radiated power,
+// 1/2 sum w_p w_q Re((E x H*) . n) (matlab/AnalyzeNF2FF.m:201-209), goes to
+// part[blockIdx.x].
 template <int KIND>
 __global__ __launch_bounds__(256) void k_nf2ff_pack(const void* __restrict__ src, double scale, int n1, int n2,
-                                                    int sign, const double* __restrict__ w1,
-                                                    const double* __restrict__ w2, float* __restrict__ Bp, int ldb,
+                                                    int swap, int sign, const double* __restrict__ wp,
+                                                    const double* __restrict__ wq, float* __restrict__ B, int ldb,
                                                     double* __restrict__ part) {
     __shared__ double red[256];
     const int tid = threadIdx.x;
@@ -127,6 +165,7 @@ __global__ __launch_bounds__(256) void k_nf2ff_pack(const void* __restrict__ src
     double p = 0.0;
     if (q < n1 * n2) {
         const int i1 = q / n2, i2 = q - i1 * n2;
+        const int ip = swap ? i2 : i1, iq = swap ? i1 : i2, nq = swap ? n1 : n2;
         double vr[4], vi[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -140,20 +179,16 @@ __global__ __launch_bounds__(256) void k_nf2ff_pack(const void* __restrict__ src
                 vi[c] = scale * (double)v[1];
             }
         }
-        const double sg = sign, w = w1[i1];
+        const double sg = sign, w = wp[ip];
         const double cr[4] = {-sg * vr[3], sg * vr[2], sg * vr[1], -sg * vr[0]};
         const double ci[4] = {-sg * vi[3], sg * vi[2], sg * vi[1], -sg * vi[0]};
-        float* top = Bp + (size_t)i1 * ldb;
-        float* bot = Bp + (size_t)(n1 + i1) * ldb;
+        float* row = B + (size_t)ip * ldb;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const float re = (float)(w * cr[c]), im = (float)(w * ci[c]);
-            top[4 * i2 + c] = re;
-            top[4 * n2 + 4 * i2 + c] = im;
-            bot[4 * i2 + c] = -im;
-            bot[4 * n2 + 4 * i2 + c] = re;
+            row[4 * iq + c] = (float)(w * cr[c]);
+            row[4 * nq + 4 * iq + c] = (float)(w * ci[c]);
         }
-        p = 0.5 * w * w2[i2] * sg * (vr[0] * vr[3] + vi[0] * vi[3] - vr[1] * vr[2] - vi[1] * vi[2]);
+        p = 0.5 * w * wq[iq] * sg * (vr[0] * vr[3] + vi[0] * vi[3] - vr[1] * vr[2] - vi[1] * vi[2]);
     }
     red[tid] = p;
     __syncthreads();
@@ -164,50 +199,140 @@ __global__ __launch_bounds__(256) void k_nf2ff_pack(const void* __restrict__ src
     if (tid == 0) part[blockIdx.x] = red[0];
 }
 
-// S[a, c] = exp(j k kappa0_a x0) sum_i2 w2[i2] exp(j k kappa2_a x2[i2]) G[a, i2, c]
-// for the 4 currents of one face, added to NL[a] (complex float64 Nx, Ny, Nz,
-// Lx, Ly, Lz).  One wave per angle, lanes over i2, a fixed-order shuffle
-// reduction.  G row a holds Gr at 4 i2 + c and Gi at 4 n2 + 4 i2 + c.
-__global__ __launch_bounds__(256) void k_nf2ff_reduce(const float* __restrict__ G, int ldg, int n_ang, int n2,
-                                                      const double* __restrict__ kap2,
-                                                      const double* __restrict__ kap0,
-                                                      const double* __restrict__ x2, const double* __restrict__ w2,
-                                                      double x0, double k, int a1, int a2, double* __restrict__ NL) {
-    const int a = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (a >= n_ang) return;
-    const float* g = G + (size_t)a * ldg;
-    const double kk = k * kap2[a];
+struct ReduceArgs {
+    const float* G;        // GEMM output of this chunk: rows 2 (u - u0) (C) and 2 (u - u0) + 1 (S)
+    int ldg;
+    const int* order;      // angles sorted by u
+    int b0, b1;            // this chunk's range of order[]
+    int u0;
+    const int* uidx;       // per angle
+    const float* usgn;     // per angle: sign of kappa_p
+    int nq;
+    const double* kapq;    // per angle
+    const double* kap0;    // per angle
+    const double* xq;
+    const double* wq;
+    int uniform;           // xq[i] = xq[0] + i dxq (phases by recurrence)
+    double dxq, x0, k;
+    int slot[4];           // NL slot of current c
+    double* NL;            // [n_ang][6] complex
+};
+
+// A work-group takes 64 consecutive angles of the u-sorted order (one per
+// lane, so the lanes of a wave mostly share their G rows and the row loads
+// are broadcasts); its RW waves split the iq range and their partial sums
+// are added in a fixed order through LDS.
+constexpr int RW = 8;
+
+__global__ __launch_bounds__(64 * RW) void k_nf2ff_reduce(ReduceArgs r) {
+    __shared__ double part[RW - 1][8][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int pos = r.b0 + blockIdx.x * 64 + lane;
+    const bool act = pos < r.b1;
+    const int a = r.order[act ? pos : r.b0];
+    const double sg = r.usgn[a];
+    const float* gc = r.G + (size_t)(2 * (r.uidx[a] - r.u0)) * r.ldg;
+    const float* gs = gc + r.ldg;
+    const double kk = r.k * r.kapq[a];
+    const int per = (r.nq + RW - 1) / RW;
+    const int i0 = wave * per, i1 = min(r.nq, i0 + per);
     double sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
-    for (int i2 = lane; i2 < n2; i2 += 64) {
+    double pr = 1, pi = 0, str = 1, sti = 0;
+    if (r.uniform && i0 < i1) {
+        sincos(kk * r.xq[i0], &pi, &pr);
+        sincos(kk * r.dxq, &sti, &str);
+    }
+    for (int iq = i0; iq < i1; ++iq) {
         double sn, cs;
-        sincos(kk * x2[i2], &sn, &cs);
-        const double wr = w2[i2] * cs, wi = w2[i2] * sn;
-        const float4 gr = *reinterpret_cast<const float4*>(g + 4 * i2);
-        const float4 gi = *reinterpret_cast<const float4*>(g + 4 * n2 + 4 * i2);
-        const float grv[4] = {gr.x, gr.y, gr.z, gr.w}, giv[4] = {gi.x, gi.y, gi.z, gi.w};
+        if (r.uniform) {
+            cs = pr;
+            sn = pi;
+            const double t = pr * str - pi * sti;
+            pi = pr * sti + pi * str;
+            pr = t;
+        } else {
+            sincos(kk * r.xq[iq], &sn, &cs);
+        }
+        const double wr = r.wq[iq] * cs, wi = r.wq[iq] * sn;
+        const float4 cr4 = *reinterpret_cast<const float4*>(gc + 4 * iq);
+        const float4 ci4 = *reinterpret_cast<const float4*>(gc + 4 * r.nq + 4 * iq);
+        const float4 sr4 = *reinterpret_cast<const float4*>(gs + 4 * iq);
+        const float4 si4 = *reinterpret_cast<const float4*>(gs + 4 * r.nq + 4 * iq);
+        const float gcr[4] = {cr4.x, cr4.y, cr4.z, cr4.w}, gci[4] = {ci4.x, ci4.y, ci4.z, ci4.w};
+        const float gsr[4] = {sr4.x, sr4.y, sr4.z, sr4.w}, gsi[4] = {si4.x, si4.y, si4.z, si4.w};
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            sr[c] += wr * (double)grv[c] - wi * (double)giv[c];
-            si[c] += wr * (double)giv[c] + wi * (double)grv[c];
+            const double gr = (double)gcr[c] - sg * (double)gsi[c];
+            const double gi = (double)gci[c] + sg * (double)gsr[c];
+            sr[c] += wr * gr - wi * gi;
+            si[c] += wr * gi + wi * gr;
         }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
+    if (wave > 0) {
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            sr[c] += __shfl_xor(sr[c], o, 64);
-            si[c] += __shfl_xor(si[c], o, 64);
+            part[wave - 1][c][lane] = sr[c];
+            part[wave - 1][4 + c][lane] = si[c];
         }
-    if (lane != 0) return;
+    }
+    __syncthreads();
+    if (wave != 0 || !act) return;
+    for (int w = 0; w < RW - 1; ++w)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            sr[c] += part[w][c][lane];
+            si[c] += part[w][4 + c][lane];
+        }
     double sn, cs;
-    sincos(k * kap0[a] * x0, &sn, &cs);
-    const int slot[4] = {a1, a2, 3 + a1, 3 + a2};  // J -> N, M -> L
+    sincos(r.k * r.kap0[a] * r.x0, &sn, &cs);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        double* o = NL + ((size_t)a * 6 + slot[c]) * 2;
+        double* o = r.NL + ((size_t)a * 6 + r.slot[c]) * 2;
         o[0] += cs * sr[c] - sn * si[c];
         o[1] += cs * si[c] + sn * sr[c];
     }
+}
+
+// E_theta = -fac (L_phi + Z0 N_theta), E_phi = fac (L_theta - Z0 N_phi) with
+// fac = jk e^{-jkr}/(4 pi r) (matlab/AnalyzeNF2FF.m:98-105) and
+// U = r^2/(2 Z0)(|E_theta|^2 + |E_phi|^2) (:112): out = [4][n_ang]
+// (E_theta re, im, E_phi re, im); the block maximum of U goes to umax[block].
+__global__ __launch_bounds__(256) void k_nf2ff_project(const double* __restrict__ NL, const double* __restrict__ theta,
+                                                       const double* __restrict__ phi, int n_phi, size_t n_ang,
+                                                       double fr, double fi, double r, double* __restrict__ out,
+                                                       double* __restrict__ umax) {
+    __shared__ double red[256];
+    const size_t a = (size_t)blockIdx.x * 256 + threadIdx.x;
+    double u = 0.0;
+    if (a < n_ang) {
+        const int it = (int)(a / n_phi), ip = (int)(a - (size_t)it * n_phi);
+        double st, ct, sp, cp;
+        sincos(theta[it], &st, &ct);
+        sincos(phi[ip], &sp, &cp);
+        const double* v = NL + 12 * a;
+        const double ntr = v[0] * (ct * cp) + v[2] * (ct * sp) - v[4] * st;
+        const double nti = v[1] * (ct * cp) + v[3] * (ct * sp) - v[5] * st;
+        const double npr = -v[0] * sp + v[2] * cp, npi = -v[1] * sp + v[3] * cp;
+        const double ltr = v[6] * (ct * cp) + v[8] * (ct * sp) - v[10] * st;
+        const double lti = v[7] * (ct * cp) + v[9] * (ct * sp) - v[11] * st;
+        const double lpr = -v[6] * sp + v[8] * cp, lpi = -v[7] * sp + v[9] * cp;
+        const double ar = lpr + Z0 * ntr, ai = lpi + Z0 * nti;  // E_theta = -fac * A
+        const double br = ltr - Z0 * npr, bi = lti - Z0 * npi;  // E_phi = fac * B
+        const double etr = -(fr * ar - fi * ai), eti = -(fr * ai + fi * ar);
+        const double epr = fr * br - fi * bi, epi = fr * bi + fi * br;
+        out[a] = etr;
+        out[n_ang + a] = eti;
+        out[2 * n_ang + a] = epr;
+        out[3 * n_ang + a] = epi;
+        u = r * r / (2.0 * Z0) * (etr * etr + eti * eti + epr * epr + epi * epi);
+    }
+    red[threadIdx.x] = u;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) umax[blockIdx.x] = red[0];
 }
 
 inline size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
@@ -222,8 +347,19 @@ std::vector<double> trapz_weights(const std::vector<double>& c) {
     return w;
 }
 
+// x[i] == x[0] + i * d within 1e-9 d
+bool uniform_spacing(const std::vector<double>& x, double* d) {
+    *d = x.size() > 1 ? (x.back() - x.front()) / (double)(x.size() - 1) : 0.0;
+    for (size_t i = 0; i < x.size(); ++i)
+        if (std::fabs(x[i] - (x.front() + (double)i * *d)) > 1e-9 * std::fabs(*d)) return false;
+    return true;
+}
+
 struct EventPair {
     hipEvent_t a = nullptr, b = nullptr;
+    EventPair() = default;
+    EventPair(const EventPair&) = delete;
+    EventPair& operator=(const EventPair&) = delete;
     ~EventPair() {
         if (a) (void)hipEventDestroy(a);
         if (b) (void)hipEventDestroy(b);
@@ -238,62 +374,177 @@ struct EventPair {
 
 }  // namespace
 
-// Rows of A'/G per GEMM launch are capped so that G stays within 64 Mi floats.
+// Angle plan and work buffers, kept between calls with the same angles.
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() { if (p) (void)hipHostFree(p); }
+    hipError_t alloc(size_t n) {
+        if (p) { (void)hipHostFree(p); p = nullptr; }
+        bytes = n;
+        return n ? hipHostMalloc(&p, n, hipHostMallocDefault) : hipSuccess;
+    }
+};
+
+struct Nf2ffCache {
+    std::vector<double> theta, phi;
+    size_t n_ang = 0;
+    DevBuf kap;  // [3][n_ang] double, then theta[nt], phi[np]
+    PinnedBuf host;  // far-field staging
+    struct Axis {
+        int U = 0;
+        std::vector<int> start;  // U + 1 offsets into order
+        DevBuf uval;             // U doubles (|kappa| of each distinct value)
+        DevBuf uidx;             // n_ang ints
+        DevBuf usgn;             // n_ang floats
+        DevBuf order;            // n_ang ints, angles sorted by u
+    } ax[3];
+    DevBuf work;
+};
+
+Nf2ffCache* nf2ff_cache_new() { return new Nf2ffCache(); }
+void nf2ff_cache_free(Nf2ffCache* c) { delete c; }
+
+static hipError_t plan_angles(Nf2ffCache& c, const double* theta, int nt, const double* phi, int np, hipStream_t s) {
+    if (c.n_ang && c.theta.size() == (size_t)nt && c.phi.size() == (size_t)np &&
+        std::equal(c.theta.begin(), c.theta.end(), theta) && std::equal(c.phi.begin(), c.phi.end(), phi))
+        return hipSuccess;
+    c.n_ang = 0;
+    const size_t n = (size_t)nt * np;
+    std::vector<double> kap(3 * n);
+    for (int it = 0; it < nt; ++it)
+        for (int ip = 0; ip < np; ++ip) {
+            const size_t a = (size_t)it * np + ip;
+            kap[a] = std::sin(theta[it]) * std::cos(phi[ip]);
+            kap[n + a] = std::sin(theta[it]) * std::sin(phi[ip]);
+            kap[2 * n + a] = std::cos(theta[it]);
+        }
+    kap.insert(kap.end(), theta, theta + nt);
+    kap.insert(kap.end(), phi, phi + np);
+    NF_OK(c.kap.alloc(kap.size() * sizeof(double)));
+    NF_OK(hipMemcpyAsync(c.kap.p, kap.data(), kap.size() * 8, hipMemcpyHostToDevice, s));
+    std::vector<int> order(n), uidx(n);
+    std::vector<float> sgn(n);
+    std::vector<double> uval;
+    for (int x = 0; x < 3; ++x) {
+        const double* kx = kap.data() + x * n;
+        std::iota(order.begin(), order.end(), 0);
+        std::sort(order.begin(), order.end(), [&](int p, int q) {
+            const double a = std::fabs(kx[p]), b = std::fabs(kx[q]);
+            return a < b || (a == b && p < q);
+        });
+        uval.clear();
+        auto& A = c.ax[x];
+        A.start.clear();
+        for (size_t i = 0; i < n; ++i) {
+            const double v = std::fabs(kx[order[i]]);
+            // values within 1e-12 of a group's first one share its row: the
+            // phase error k * 1e-12 * |x| is far below float32 resolution
+            if (uval.empty() || v - uval.back() > 1e-12) {
+                uval.push_back(v);
+                A.start.push_back((int)i);
+            }
+            uidx[order[i]] = (int)uval.size() - 1;
+            sgn[order[i]] = kx[order[i]] < 0.0 ? -1.f : 1.f;
+        }
+        A.start.push_back((int)n);
+        A.U = (int)uval.size();
+        NF_OK(A.uval.alloc(uval.size() * 8));
+        NF_OK(A.uidx.alloc(n * 4));
+        NF_OK(A.usgn.alloc(n * 4));
+        NF_OK(A.order.alloc(n * 4));
+        NF_OK(hipMemcpyAsync(A.uval.p, uval.data(), uval.size() * 8, hipMemcpyHostToDevice, s));
+        NF_OK(hipMemcpyAsync(A.uidx.p, uidx.data(), n * 4, hipMemcpyHostToDevice, s));
+        NF_OK(hipMemcpyAsync(A.usgn.p, sgn.data(), n * 4, hipMemcpyHostToDevice, s));
+        NF_OK(hipMemcpyAsync(A.order.p, order.data(), n * 4, hipMemcpyHostToDevice, s));
+        NF_OK(hipStreamSynchronize(s));  // host vectors are reused for the next axis
+    }
+    c.theta.assign(theta, theta + nt);
+    c.phi.assign(phi, phi + np);
+    c.n_ang = n;
+    return hipSuccess;
+}
+
+// G stays within 64 Mi floats per GEMM launch; larger U is split into chunks
+// of u, each with the angles whose u falls in it.
 hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const double* theta, int n_theta,
                          const double* phi, int n_phi, double r, const double center[3], hipStream_t s,
-                         Nf2ffOut* out) {
-    if (faces.empty() || n_theta <= 0 || n_phi <= 0 || !theta || !phi || !out || !(r > 0) || !(freq > 0))
+                         Nf2ffCache* cache, Nf2ffOut* out) {
+    if (faces.empty() || n_theta <= 0 || n_phi <= 0 || !theta || !phi || !out || !cache || !(r > 0) || !(freq > 0))
         return hipErrorInvalidValue;
     const size_t n_ang = (size_t)n_theta * n_phi;
     if (n_ang > (size_t)INT32_MAX / 16) return hipErrorInvalidValue;
-    const double k = 2.0 * M_PI * freq / C0;
-    std::vector<double> kap(3 * n_ang);
-    for (int it = 0; it < n_theta; ++it)
-        for (int ip = 0; ip < n_phi; ++ip) {
-            const size_t a = (size_t)it * n_phi + ip;
-            kap[a] = std::sin(theta[it]) * std::cos(phi[ip]);
-            kap[n_ang + a] = std::sin(theta[it]) * std::sin(phi[ip]);
-            kap[2 * n_ang + a] = std::cos(theta[it]);
-        }
-    size_t Kmax = 16, Nmax = 128, coord_len = 0, blocks = 0;
     for (const auto& f : faces) {
         const size_t n1 = f.c1.size(), n2 = f.c2.size();
         if (n1 < 1 || n2 < 1 || !f.src || f.axis < 0 || f.axis > 2 || (f.src_kind != 0 && f.src_kind != 1))
             return hipErrorInvalidValue;
         if (n1 * n2 > (size_t)INT32_MAX / 16) return hipErrorInvalidValue;
-        Kmax = std::max(Kmax, round_up(2 * n1, GBK));
-        Nmax = std::max(Nmax, round_up(8 * n2, GBN));
-        coord_len += 2 * (n1 + n2);
-        blocks += (n1 * n2 + 255) / 256;
+    }
+    NF_OK(hipStreamSynchronize(s));
+    NF_OK(plan_angles(*cache, theta, n_theta, phi, n_phi, s));
+    Nf2ffCache& C = *cache;
+    const double k = 2.0 * M_PI * freq / C0;
+    const double* d_kap = C.kap.as<double>();
+
+    // per face: first-contraction axis p (fewer distinct |kappa|), sizes
+    struct Plan {
+        int p, q, np, nq, K, N, swap;
+        std::vector<double> xp, xq;
+    };
+    std::vector<Plan> plans;
+    size_t Kmax = GBK, Nmax = GBN, coord_len = 0, blocks = 0;
+    for (const auto& f : faces) {
+        const int a1 = (f.axis + 1) % 3, a2 = (f.axis + 2) % 3;
+        Plan pl{};
+        pl.swap = C.ax[a2].U < C.ax[a1].U;
+        pl.p = pl.swap ? a2 : a1;
+        pl.q = pl.swap ? a1 : a2;
+        const auto& cp = pl.swap ? f.c2 : f.c1;
+        const auto& cq = pl.swap ? f.c1 : f.c2;
+        pl.np = (int)cp.size();
+        pl.nq = (int)cq.size();
+        pl.K = (int)round_up(pl.np, GBK);
+        pl.N = (int)round_up(8 * (size_t)pl.nq, GBN);
+        for (double v : cp) pl.xp.push_back(v - center[pl.p]);
+        for (double v : cq) pl.xq.push_back(v - center[pl.q]);
+        Kmax = std::max(Kmax, (size_t)pl.K);
+        Nmax = std::max(Nmax, (size_t)pl.N);
+        coord_len += 2 * ((size_t)pl.np + pl.nq);
+        blocks += (cp.size() * cq.size() + 255) / 256;
+        plans.push_back(std::move(pl));
     }
     const size_t budget = (size_t)64 << 20;
+    int Umax = 0;
+    for (const auto& pl : plans) Umax = std::max(Umax, C.ax[pl.p].U);
     size_t rows = std::max<size_t>(GBM, budget / Nmax / GBM * GBM);
-    rows = std::min(rows, round_up(n_ang, GBM));
+    rows = std::min(rows, round_up(2 * (size_t)Umax, GBM));
 
-    DevBuf dbl, flt;
-    const size_t n_dbl = 3 * n_ang + 12 * n_ang + coord_len + blocks;
-    NF_OK(dbl.alloc(n_dbl * sizeof(double)));
-    NF_OK(flt.alloc((Kmax * Nmax + rows * Kmax + rows * Nmax) * sizeof(float)));
-    double* d_kap = dbl.as<double>();
-    double* d_nl = d_kap + 3 * n_ang;
-    double* d_coord = d_nl + 12 * n_ang;
+    const size_t pblocks = (n_ang + 255) / 256;
+    const size_t n_dbl = 12 * n_ang + 4 * n_ang + pblocks + coord_len + blocks;
+    const size_t bytes = round_up(n_dbl * sizeof(double), 256) + (Kmax * Nmax + rows * Kmax + rows * Nmax) * sizeof(float);
+    if (C.work.bytes < bytes) NF_OK(C.work.alloc(bytes));
+    double* d_nl = C.work.as<double>();
+    double* d_ff = d_nl + 12 * n_ang;
+    double* d_umax = d_ff + 4 * n_ang;
+    double* d_coord = d_umax + pblocks;
     double* d_part = d_coord + coord_len;
-    float* Bp = flt.as<float>();
-    float* Ap = Bp + Kmax * Nmax;
-    float* G = Ap + rows * Kmax;
+    float* Bm = reinterpret_cast<float*>(reinterpret_cast<char*>(C.work.p) + round_up(n_dbl * sizeof(double), 256));
+    float* Am = Bm + Kmax * Nmax;
+    float* G = Am + rows * Kmax;
 
     std::vector<double> coords;
     coords.reserve(coord_len);
-    for (const auto& f : faces) {
-        const int a1 = (f.axis + 1) % 3, a2 = (f.axis + 2) % 3;
-        const auto w1 = trapz_weights(f.c1), w2 = trapz_weights(f.c2);
-        for (double c : f.c1) coords.push_back(c - center[a1]);
-        coords.insert(coords.end(), w1.begin(), w1.end());
-        for (double c : f.c2) coords.push_back(c - center[a2]);
-        coords.insert(coords.end(), w2.begin(), w2.end());
+    for (const auto& pl : plans) {
+        coords.insert(coords.end(), pl.xp.begin(), pl.xp.end());
+        const auto wp = trapz_weights(pl.xp);
+        coords.insert(coords.end(), wp.begin(), wp.end());
+        coords.insert(coords.end(), pl.xq.begin(), pl.xq.end());
+        const auto wq = trapz_weights(pl.xq);
+        coords.insert(coords.end(), wq.begin(), wq.end());
     }
-    NF_OK(hipStreamSynchronize(s));
-    NF_OK(hipMemcpyAsync(d_kap, kap.data(), kap.size() * 8, hipMemcpyHostToDevice, s));
     NF_OK(hipMemcpyAsync(d_coord, coords.data(), coords.size() * 8, hipMemcpyHostToDevice, s));
     NF_OK(hipMemsetAsync(d_nl, 0, 12 * n_ang * sizeof(double), s));
 
@@ -304,51 +555,81 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
     NF_OK(hipEventRecord(all.a, s));
     double flop = 0.0;
     size_t co = 0, bo = 0;
-    for (const auto& f : faces) {
+    for (size_t fi = 0; fi < faces.size(); ++fi) {
+        const Nf2ffFace& f = faces[fi];
+        const Plan& pl = plans[fi];
+        const auto& A = C.ax[pl.p];
         const int n1 = (int)f.c1.size(), n2 = (int)f.c2.size();
         const int a1 = (f.axis + 1) % 3, a2 = (f.axis + 2) % 3;
-        const int K = (int)round_up(2 * (size_t)n1, GBK), N = (int)round_up(8 * (size_t)n2, GBN);
-        const double* x1 = d_coord + co;
-        const double* w1 = x1 + n1;
-        const double* x2 = w1 + n1;
-        const double* w2 = x2 + n2;
-        co += 2 * ((size_t)n1 + n2);
+        const double* xp = d_coord + co;
+        const double* wp = xp + pl.np;
+        const double* xq = wp + pl.np;
+        const double* wq = xq + pl.nq;
+        co += 2 * ((size_t)pl.np + pl.nq);
         const int nb = (n1 * n2 + 255) / 256;
-        NF_OK(hipMemsetAsync(Bp, 0, (size_t)K * N * sizeof(float), s));
+        NF_OK(hipMemsetAsync(Bm, 0, (size_t)pl.K * pl.N * sizeof(float), s));
         if (f.src_kind == 0)
-            hipLaunchKernelGGL(k_nf2ff_pack<0>, dim3(nb), dim3(256), 0, s, f.src, f.scale, n1, n2, f.sign, w1, w2, Bp,
-                               N, d_part + bo);
+            hipLaunchKernelGGL(k_nf2ff_pack<0>, dim3(nb), dim3(256), 0, s, f.src, f.scale, n1, n2, pl.swap, f.sign,
+                               wp, wq, Bm, pl.N, d_part + bo);
         else
-            hipLaunchKernelGGL(k_nf2ff_pack<1>, dim3(nb), dim3(256), 0, s, f.src, f.scale, n1, n2, f.sign, w1, w2, Bp,
-                               N, d_part + bo);
+            hipLaunchKernelGGL(k_nf2ff_pack<1>, dim3(nb), dim3(256), 0, s, f.src, f.scale, n1, n2, pl.swap, f.sign,
+                               wp, wq, Bm, pl.N, d_part + bo);
         NF_OK(hipGetLastError());
         bo += nb;
-        const double x0 = f.c0 - center[f.axis];
-        for (size_t a0 = 0; a0 < n_ang; a0 += rows) {
-            const int m = (int)std::min(rows, n_ang - a0);
-            const int mp = (int)round_up(m, GBM);
-            const size_t total = (size_t)mp * K;
+        double dxq = 0.0;
+        const bool uni = uniform_spacing(pl.xq, &dxq);
+        ReduceArgs ra{};
+        ra.G = G;
+        ra.ldg = pl.N;
+        ra.order = A.order.as<int>();
+        ra.uidx = A.uidx.as<int>();
+        ra.usgn = A.usgn.as<float>();
+        ra.nq = pl.nq;
+        ra.kapq = d_kap + (size_t)pl.q * n_ang;
+        ra.kap0 = d_kap + (size_t)f.axis * n_ang;
+        ra.xq = xq;
+        ra.wq = wq;
+        ra.uniform = uni ? 1 : 0;
+        ra.dxq = dxq;
+        ra.x0 = f.c0 - center[f.axis];
+        ra.k = k;
+        ra.slot[0] = a1; ra.slot[1] = a2; ra.slot[2] = 3 + a1; ra.slot[3] = 3 + a2;
+        ra.NL = d_nl;
+        const int uchunk = (int)(rows / 2);
+        for (int u0 = 0; u0 < A.U; u0 += uchunk) {
+            const int u1 = std::min(A.U, u0 + uchunk);
+            const int mp = (int)round_up(2 * (size_t)(u1 - u0), GBM);
+            const size_t total = (size_t)mp * pl.K;
             const int pb = (int)std::min<size_t>((total + 255) / 256, 16384);
-            hipLaunchKernelGGL(k_phase, dim3(pb), dim3(256), 0, s, Ap, K, total, m, n1, d_kap + a1 * n_ang + a0, x1, k);
+            hipLaunchKernelGGL(k_phase, dim3(pb), dim3(256), 0, s, Am, pl.K, total, u0, u1, pl.np, A.uval.as<double>(),
+                               xp, k);
             NF_OK(hipGetLastError());
             gemm_ev.emplace_back(new EventPair());
             NF_OK(hipEventCreate(&gemm_ev.back()->a));
             NF_OK(hipEventCreate(&gemm_ev.back()->b));
             NF_OK(hipEventRecord(gemm_ev.back()->a, s));
-            hipLaunchKernelGGL(k_sgemm_mfma, dim3(N / GBN, mp / GBM), dim3(256), 0, s, Ap, Bp, G, mp, N, K);
-            NF_OK(hipGetLastError());
+            NF_OK(launch_gemm(Am, Bm, G, mp, pl.N, pl.K, s));
             NF_OK(hipEventRecord(gemm_ev.back()->b, s));
-            flop += 2.0 * mp * (double)N * K;
-            hipLaunchKernelGGL(k_nf2ff_reduce, dim3((m + 3) / 4), dim3(256), 0, s, G, N, m, n2,
-                               d_kap + a2 * n_ang + a0, d_kap + f.axis * n_ang + a0, x2, w2, x0, k, a1, a2,
-                               d_nl + 12 * a0);
+            flop += 2.0 * (2.0 * (u1 - u0)) * (8.0 * pl.nq) * pl.np;  // useful (unpadded) work
+            ra.b0 = A.start[u0];
+            ra.b1 = A.start[u1];
+            ra.u0 = u0;
+            const int na = ra.b1 - ra.b0;
+            hipLaunchKernelGGL(k_nf2ff_reduce, dim3((na + 63) / 64), dim3(64 * RW), 0, s, ra);
             NF_OK(hipGetLastError());
         }
     }
+    typedef std::complex<double> cd;
+    const cd fac = cd(0.0, k) * std::exp(cd(0.0, -k * r)) / (4.0 * M_PI * r);
+    hipLaunchKernelGGL(k_nf2ff_project, dim3((unsigned)pblocks), dim3(256), 0, s, d_nl, d_kap + 3 * n_ang,
+                       d_kap + 3 * n_ang + n_theta, n_phi, n_ang, fac.real(), fac.imag(), r, d_ff, d_umax);
+    NF_OK(hipGetLastError());
     NF_OK(hipEventRecord(all.b, s));
-    std::vector<double> nl(12 * n_ang), part(blocks);
-    NF_OK(hipMemcpyAsync(nl.data(), d_nl, nl.size() * 8, hipMemcpyDeviceToHost, s));
-    NF_OK(hipMemcpyAsync(part.data(), d_part, part.size() * 8, hipMemcpyDeviceToHost, s));
+    const size_t hb = (4 * n_ang + pblocks + blocks) * sizeof(double);
+    if (C.host.bytes < hb) NF_OK(C.host.alloc(hb));
+    double* h = reinterpret_cast<double*>(C.host.p);
+    NF_OK(hipMemcpyAsync(h, d_ff, (4 * n_ang + pblocks) * sizeof(double), hipMemcpyDeviceToHost, s));
+    NF_OK(hipMemcpyAsync(h + 4 * n_ang + pblocks, d_part, blocks * sizeof(double), hipMemcpyDeviceToHost, s));
     NF_OK(hipStreamSynchronize(s));
     float ms = 0.f;
     NF_OK(hipEventElapsedTime(&ms, all.a, all.b));
@@ -360,31 +641,13 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
     }
     out->gemm_flop = flop;
     double prad = 0.0;
-    for (double p : part) prad += p;
+    for (size_t i = 0; i < blocks; ++i) prad += h[4 * n_ang + pblocks + i];
     out->prad = prad;
-
-    // E_theta = -jk e^{-jkr}/(4 pi r) (L_phi + Z0 N_theta),
-    // E_phi = jk e^{-jkr}/(4 pi r) (L_theta - Z0 N_phi)  (matlab/AnalyzeNF2FF.m:98-112)
-    typedef std::complex<double> cd;
-    const cd fac = cd(0.0, k) * std::exp(cd(0.0, -k * r)) / (4.0 * M_PI * r);
     double umax = 0.0;
-    for (int it = 0; it < n_theta; ++it) {
-        const double st = std::sin(theta[it]), ct = std::cos(theta[it]);
-        for (int ip = 0; ip < n_phi; ++ip) {
-            const double sp = std::sin(phi[ip]), cp = std::cos(phi[ip]);
-            const size_t a = (size_t)it * n_phi + ip;
-            const double* v = nl.data() + 12 * a;
-            const cd Nx(v[0], v[1]), Ny(v[2], v[3]), Nz(v[4], v[5]), Lx(v[6], v[7]), Ly(v[8], v[9]), Lz(v[10], v[11]);
-            const cd Nt = Nx * (ct * cp) + Ny * (ct * sp) - Nz * st, Np = -Nx * sp + Ny * cp;
-            const cd Lt = Lx * (ct * cp) + Ly * (ct * sp) - Lz * st, Lp = -Lx * sp + Ly * cp;
-            const cd Et = -fac * (Lp + Z0 * Nt), Ep = fac * (Lt - Z0 * Np);
-            if (out->et_re) out->et_re[a] = Et.real();
-            if (out->et_im) out->et_im[a] = Et.imag();
-            if (out->ep_re) out->ep_re[a] = Ep.real();
-            if (out->ep_im) out->ep_im[a] = Ep.imag();
-            umax = std::max(umax, r * r / (2.0 * Z0) * (std::norm(Et) + std::norm(Ep)));
-        }
-    }
+    for (size_t i = 0; i < pblocks; ++i) umax = std::max(umax, h[4 * n_ang + i]);
+    double* dst[4] = {out->et_re, out->et_im, out->ep_re, out->ep_im};
+    for (int q = 0; q < 4; ++q)
+        if (dst[q]) std::memcpy(dst[q], h + q * n_ang, n_ang * sizeof(double));
     out->dmax = prad > 0.0 ? 4.0 * M_PI * umax / prad : 0.0;
     return hipSuccess;
 }

@@ -231,25 +231,24 @@ __device__ __forceinline__ void psi_store(const Psi<V>& s) {
 // AX: CPML axes compiled in (AX_X | AX_Y | AX_Z); PHYS: apply the field
 // correction (else psi only, as the reference).
 template <int V, int AX, bool PHYS>
-__global__ __launch_bounds__(256) void k_update_h(YeeArgs a, CpmlArgs c) {
-    constexpr bool CP = AX != 0;
-    const Tile t = decode<V>(a);
-    const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
+__device__ __forceinline__ void h_body(const Grid& g, float* __restrict__ F, const float* __restrict__ Cf,
+                                       const CpmlArgs& c, const Tile& t) {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
     if (t.k0 >= nz || t.j >= t.jend) return;
-    const size_t N = a.g.N;
+    const size_t N = g.N;
     const size_t plane = (size_t)ny * nz;
-    const float* __restrict__ Ex = a.F;
-    const float* __restrict__ Ey = a.F + N;
-    const float* __restrict__ Ez = a.F + 2 * N;
-    float* __restrict__ Hx = a.F + 3 * N;
-    float* __restrict__ Hy = a.F + 4 * N;
-    float* __restrict__ Hz = a.F + 5 * N;
-    const float* __restrict__ da0 = a.Cf + 6 * N;
-    const float* __restrict__ da1 = a.Cf + 7 * N;
-    const float* __restrict__ da2 = a.Cf + 8 * N;
-    const float* __restrict__ db0 = a.Cf + 9 * N;
-    const float* __restrict__ db1 = a.Cf + 10 * N;
-    const float* __restrict__ db2 = a.Cf + 11 * N;
+    const float* __restrict__ Ex = F;
+    const float* __restrict__ Ey = F + N;
+    const float* __restrict__ Ez = F + 2 * N;
+    float* __restrict__ Hx = F + 3 * N;
+    float* __restrict__ Hy = F + 4 * N;
+    float* __restrict__ Hz = F + 5 * N;
+    const float* __restrict__ da0 = Cf + 6 * N;
+    const float* __restrict__ da1 = Cf + 7 * N;
+    const float* __restrict__ da2 = Cf + 8 * N;
+    const float* __restrict__ db0 = Cf + 9 * N;
+    const float* __restrict__ db1 = Cf + 10 * N;
+    const float* __restrict__ db2 = Cf + 11 * N;
 
     const int j = t.j, k0 = t.k0;
     const bool hj = j + 1 < ny;
@@ -344,25 +343,24 @@ __global__ __launch_bounds__(256) void k_update_h(YeeArgs a, CpmlArgs c) {
 }
 
 template <int V, int AX, bool PHYS>
-__global__ __launch_bounds__(256) void k_update_e(YeeArgs a, CpmlArgs c) {
-    constexpr bool CP = AX != 0;
-    const Tile t = decode<V>(a);
-    const int ny = a.g.ny, nz = a.g.nz;
+__device__ __forceinline__ void e_body(const Grid& g, float* __restrict__ F, const float* __restrict__ Cf,
+                                       const CpmlArgs& c, const Tile& t) {
+    const int ny = g.ny, nz = g.nz;
     if (t.k0 >= nz || t.j >= t.jend) return;
-    const size_t N = a.g.N;
+    const size_t N = g.N;
     const size_t plane = (size_t)ny * nz;
-    float* __restrict__ Ex = a.F;
-    float* __restrict__ Ey = a.F + N;
-    float* __restrict__ Ez = a.F + 2 * N;
-    const float* __restrict__ Hx = a.F + 3 * N;
-    const float* __restrict__ Hy = a.F + 4 * N;
-    const float* __restrict__ Hz = a.F + 5 * N;
-    const float* __restrict__ ca0 = a.Cf;
-    const float* __restrict__ ca1 = a.Cf + N;
-    const float* __restrict__ ca2 = a.Cf + 2 * N;
-    const float* __restrict__ cb0 = a.Cf + 3 * N;
-    const float* __restrict__ cb1 = a.Cf + 4 * N;
-    const float* __restrict__ cb2 = a.Cf + 5 * N;
+    float* __restrict__ Ex = F;
+    float* __restrict__ Ey = F + N;
+    float* __restrict__ Ez = F + 2 * N;
+    const float* __restrict__ Hx = F + 3 * N;
+    const float* __restrict__ Hy = F + 4 * N;
+    const float* __restrict__ Hz = F + 5 * N;
+    const float* __restrict__ ca0 = Cf;
+    const float* __restrict__ ca1 = Cf + N;
+    const float* __restrict__ ca2 = Cf + 2 * N;
+    const float* __restrict__ cb0 = Cf + 3 * N;
+    const float* __restrict__ cb1 = Cf + 4 * N;
+    const float* __restrict__ cb2 = Cf + 5 * N;
 
     const int j = t.j, k0 = t.k0;
     const bool gj = j > 0;
@@ -463,6 +461,62 @@ __global__ __launch_bounds__(256) void k_update_e(YeeArgs a, CpmlArgs c) {
     }
 }
 
+// One grid: the tile comes from the launch's box list.
+template <int V, int AX, bool PHYS>
+__global__ __launch_bounds__(256) void k_update_h(YeeArgs a, CpmlArgs c) {
+    h_body<V, AX, PHYS>(a.g, a.F, a.Cf, c, decode<V>(a));
+}
+
+template <int V, int AX, bool PHYS>
+__global__ __launch_bounds__(256) void k_update_e(YeeArgs a, CpmlArgs c) {
+    e_body<V, AX, PHYS>(a.g, a.F, a.Cf, c, decode<V>(a));
+}
+
+// Many grids in one launch (a parameter sweep): work-group -> box of some
+// structure through the launch's table; each structure has its own tile
+// shape (bx * by <= 256, surplus lanes exit).
+template <int V>
+__device__ __forceinline__ bool decode_sweep(const SweepArgs& a, Tile& r, int& sid) {
+    int bid = blockIdx.x;
+    if (a.xcd_remap) {
+        const int per = gridDim.x >> 3;
+        if (bid < (per << 3)) bid = (bid & 7) * per + (bid >> 3);
+    }
+    const SweepBox& B = a.box[a.wg_box[bid]];
+    bid -= B.first;
+    const int kb = bid % B.n_kb;
+    const int t = bid / B.n_kb;
+    const int jb = t % B.n_jb;
+    const int ib = t / B.n_jb;
+    const int kx = threadIdx.x % B.bx;
+    const int jy = threadIdx.x / B.bx;
+    r.k0 = (kb * B.bx + kx) * V;
+    r.j = B.j0 + jb * B.by + jy;
+    r.i0 = B.i0 + ib * B.ci;
+    r.i1 = min(r.i0 + B.ci, B.i1);
+    r.jend = jy < B.by ? B.j1 : r.j;  // surplus lanes get an empty row range
+    sid = B.sid;
+    return true;
+}
+
+template <int V, int AX, bool PHYS>
+__global__ __launch_bounds__(256) void k_update_h_sweep(SweepArgs a) {
+    Tile t;
+    int sid;
+    decode_sweep<V>(a, t, sid);
+    const SweepStruct& S = a.st[sid];
+    h_body<V, AX, PHYS>(S.g, S.F, S.Cf, S.c, t);
+}
+
+template <int V, int AX, bool PHYS>
+__global__ __launch_bounds__(256) void k_update_e_sweep(SweepArgs a) {
+    Tile t;
+    int sid;
+    decode_sweep<V>(a, t, sid);
+    const SweepStruct& S = a.st[sid];
+    e_body<V, AX, PHYS>(S.g, S.F, S.Cf, S.c, t);
+}
+
 template <int V, int AX, bool PHYS>
 hipError_t launch_h_t(const YeeArgs& a, const CpmlArgs& c, hipStream_t s) {
     hipLaunchKernelGGL((k_update_h<V, AX, PHYS>), dim3(a.first[a.nbox]), dim3(256), 0, s, a, c);
@@ -498,6 +552,28 @@ hipError_t launch_update_h(const YeeArgs& a, const CpmlArgs* c, int axes, hipStr
 
 hipError_t launch_update_e(const YeeArgs& a, const CpmlArgs* c, int axes, hipStream_t s) {
     return a.t.vec == 4 ? dispatch<false, 4>(a, c, axes, s) : dispatch<false, 1>(a, c, axes, s);
+}
+
+// Sweep launches: physical CPML only (every structure of a sweep carries the
+// Roden-Gedney correction); axes 0 (no CPML), AX_Z or all faces.
+template <bool H, int V, int AX>
+static hipError_t sweep_t(const SweepArgs& a, int n_wg, hipStream_t s) {
+    if (H) hipLaunchKernelGGL((k_update_h_sweep<V, AX, AX != 0>), dim3(n_wg), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_update_e_sweep<V, AX, AX != 0>), dim3(n_wg), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <bool H, int V>
+static hipError_t sweep_dispatch(const SweepArgs& a, int n_wg, int axes, hipStream_t s) {
+    if (axes == 0) return sweep_t<H, V, 0>(a, n_wg, s);
+    if (axes == AX_Z) return sweep_t<H, V, AX_Z>(a, n_wg, s);
+    return sweep_t<H, V, AX_X | AX_Y | AX_Z>(a, n_wg, s);
+}
+
+hipError_t launch_sweep(bool h, const SweepArgs& a, int n_wg, int axes, int vec, hipStream_t s) {
+    if (n_wg <= 0) return hipSuccess;
+    if (h) return vec == 4 ? sweep_dispatch<true, 4>(a, n_wg, axes, s) : sweep_dispatch<true, 1>(a, n_wg, axes, s);
+    return vec == 4 ? sweep_dispatch<false, 4>(a, n_wg, axes, s) : sweep_dispatch<false, 1>(a, n_wg, axes, s);
 }
 
 }  // namespace oe

@@ -77,3 +77,119 @@ def cpml_boxes(n: int = 256, steps: int = 4000, count: int = 200, seed: int = 2,
 def apply_materials(op, wl: Workload):
     for eps, mu, se, sm, reg in wl.boxes:
         op.set_material(eps, mu, se, sm, reg)
+
+
+@dataclass
+class PortSpheres:
+    """The composite workload: CPML on five faces and first-order Mur on +z,
+    a 50-ohm z-directed lumped port (1 x 1 x 4 cells) driven by an openEMS
+    Gaussian, seeded Drude spheres (omega_p = 1e15 rad/s, gamma = 1e13 /s)
+    and Lorentz spheres (eps_inf = 2 as the background material of the
+    sphere, delta_eps = 3, omega_0 = 2 pi 10 GHz, gamma = 1e9 /s) in the
+    physical ADE mode, port V/I running DFT over 64 frequencies (1-20 GHz)
+    and an NF2FF recording box 10 cells inside the CPML at 5, 10 and 15 GHz.
+    The cell size is not fixed by the configuration; 0.5 mm is used."""
+    shape: Tuple[int, int, int]
+    delta: float
+    steps: int
+    layers: int
+    port_region: Tuple[int, int, int, int, int, int]
+    port_R: float
+    fc: float
+    spheres: List[Tuple[str, Tuple[int, int, int], int]]  # ("drude" | "lorentz", centre, radius)
+    dft_freqs: np.ndarray
+    nf2ff_box: Tuple[int, int, int, int, int, int]
+    nf2ff_freqs: Tuple[float, ...]
+
+    DRUDE = (1e15, 1e13)
+    LORENTZ = (2 * math.pi * 10e9, 1e9, 3.0)
+    LORENTZ_EPS_INF = 2.0
+
+    @property
+    def cells(self) -> int:
+        return int(np.prod(self.shape))
+
+    @property
+    def cpml_layers(self):
+        return (self.layers,) * 5 + (0,)
+
+    @property
+    def mur_faces(self):
+        return (False,) * 5 + (True,)
+
+    def sphere_cells(self, centre, radius) -> np.ndarray:
+        c = np.asarray(centre)
+        r = int(radius)
+        g = np.mgrid[-r:r + 1, -r:r + 1, -r:r + 1].reshape(3, -1).T
+        g = g[(g * g).sum(1) <= r * r]
+        return (g + c).astype(np.int64)
+
+    def columns(self, cells: np.ndarray):
+        """z-runs (i, j, k0, k1) covering the cells (one set_material box each)."""
+        out = []
+        order = np.lexsort((cells[:, 2], cells[:, 1], cells[:, 0]))
+        cs = cells[order]
+        start = 0
+        for q in range(1, len(cs) + 1):
+            if q == len(cs) or cs[q, 0] != cs[start, 0] or cs[q, 1] != cs[start, 1] or cs[q, 2] != cs[q - 1, 2] + 1:
+                out.append((int(cs[start, 0]), int(cs[start, 1]), int(cs[start, 2]), int(cs[q - 1, 2]) + 1))
+                start = q
+        return out
+
+    def excitation_samples(self, dt: float, n: int) -> np.ndarray:
+        """openEMS Gaussian (f0 = 0, cutoff fc) sampled at s * dt, float32."""
+        t0, tau = 9.0 / (2.0 * math.pi * self.fc), 3.0 / (2.0 * math.pi * self.fc)
+        t = np.arange(n, dtype=np.float64) * dt
+        return np.exp(-((t - t0) / tau) ** 2).astype(np.float32)
+
+
+def port_spheres(shape=(384, 384, 256), steps: int = 2000, n_spheres: int = 8, rmin: int = 4, rmax: int = 12,
+                 seed: int = 3, layers: int = 8, delta: float = 0.5e-3) -> PortSpheres:
+    nx, ny, nz = shape
+    cx, cy, cz = nx // 2, ny // 2, nz // 2
+    port = (cx, cx + 1, cy, cy + 1, cz - 2, cz + 2)
+    rng = np.random.default_rng(seed)
+    lo = layers + 10
+    spheres = []
+    placed = [((cx, cy, cz), 4)]  # keep clear of the port
+    for kind in ("drude", "lorentz"):
+        count = 0
+        tries = 0
+        while count < n_spheres and tries < 10000:
+            tries += 1
+            r = int(rng.integers(rmin, rmax + 1))
+            c = tuple(int(rng.integers(lo + r, s - lo - r)) for s in shape)
+            if any(sum((a - b) ** 2 for a, b in zip(c, pc)) <= (r + pr + 2) ** 2 for pc, pr in placed):
+                continue
+            placed.append((c, r))
+            spheres.append((kind, c, r))
+            count += 1
+    box = (lo, nx - lo, lo, ny - lo, lo, nz - lo)
+    return PortSpheres(shape, delta, steps, layers, port, 50.0, 20e9, spheres, np.linspace(1e9, 20e9, 64), box,
+                       (5e9, 10e9, 15e9))
+
+
+def build_port_spheres_hip(wl: PortSpheres, precision: int = 0, nf2ff_stride: int = 1, record_steps: int = 0):
+    """Operator, engine and extensions of the composite workload on the HIP
+    engine.  Returns (op, eng, exts dict)."""
+    from . import ADE_DRUDE, ADE_LORENTZ, MODE_PHYSICAL, CpmlBoundaries, HipEngine, Operator
+
+    op = Operator.uniform(*wl.shape, wl.delta)
+    ade_cells = {"drude": [], "lorentz": []}
+    for kind, c, r in wl.spheres:
+        cells = wl.sphere_cells(c, r)
+        ade_cells[kind].append(cells)
+        if kind == "lorentz":
+            for i, j, k0, k1 in wl.columns(cells):
+                op.set_material(wl.LORENTZ_EPS_INF, 1.0, 0.0, 0.0, (i, i + 1, j, j + 1, k0, k1))
+    op.add_lumped_element(2, wl.port_region, wl.port_R)
+    eng = HipEngine(op)
+    exts = {"cpml": eng.cpml(CpmlBoundaries(layers=wl.cpml_layers, mode=MODE_PHYSICAL)),
+            "mur": eng.mur(wl.mur_faces)}
+    if ade_cells["drude"]:
+        exts["drude"] = eng.ade(ADE_DRUDE, wl.DRUDE, np.concatenate(ade_cells["drude"]), MODE_PHYSICAL)
+    if ade_cells["lorentz"]:
+        exts["lorentz"] = eng.ade(ADE_LORENTZ, wl.LORENTZ, np.concatenate(ade_cells["lorentz"]), MODE_PHYSICAL)
+    exts["port"] = eng.lumped_port(2, wl.port_region, wl.dft_freqs, precision=precision, record_steps=record_steps)
+    exts["nf2ff"] = eng.nf2ff_box(wl.nf2ff_box, wl.nf2ff_freqs, precision=precision, time_stride=nf2ff_stride)
+    return op, eng, exts

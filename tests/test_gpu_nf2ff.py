@@ -86,22 +86,36 @@ def test_random_phasors_nonuniform_offset_centre():
 
 
 def test_angle_chunks_and_wide_faces():
-    """z faces 6 x 300 nodes: G rows are split over two GEMM launches at
-    181 x 180 angles; the oracle checks every 20th theta row."""
+    """z faces 6 x 300 nodes and 181 x 180 irregular angles: every direction
+    cosine is distinct, so the GEMM rows are split over several launches;
+    the oracle checks every 20th theta row.  A second call with other angles
+    and a third with the first ones exercise the cached angle plan."""
     f = 10e9
     lam = N.C0 / f
     faces = N.box_faces([-0.1 * lam, -2 * lam, -0.08 * lam], [0.1 * lam, 2 * lam, 0.08 * lam], (6, 300, 5),
                         lambda X_, Y, Z: N.hertzian_dipole_fields(1e-3, f, X_ + 0.001, Y, Z))
-    th = np.linspace(0, math.pi, 181)
-    ph = np.linspace(0, 2 * math.pi, 181)[:-1]
-    g = _gpu_far_field(faces, f, th, ph)
+    rng = np.random.default_rng(12)
+    th = np.sort(rng.uniform(0, math.pi, 181))
+    ph = np.sort(rng.uniform(0, 2 * math.pi, 180))
+    nf = Nf2ff(f)
+    for fc in faces:
+        a1, a2 = (fc.axis + 1) % 3, (fc.axis + 2) % 3
+        nf.add_surface(fc.axis, fc.sign, fc.c0, fc.c1, fc.c2, fc.E[[a1, a2]], fc.H[[a1, a2]])
+    g = nf.calculate(th, ph)
     sub = slice(0, 181, 20)
     Et, Ep, prad, _ = N.far_field(faces, f, th[sub], ph)
     err = math.sqrt((np.linalg.norm(g.e_theta[sub] - Et) ** 2 + np.linalg.norm(g.e_phi[sub] - Ep) ** 2)
                     / (np.linalg.norm(Et) ** 2 + np.linalg.norm(Ep) ** 2))
-    say(f"[nf2ff] 181x180 angles, 6x300x5 box: rel L2 {err:.2e} on every 20th theta; gemm {g.gemm_ms:.3f} ms")
+    say(f"[nf2ff] 181x180 irregular angles, 6x300x5 box: rel L2 {err:.2e} on every 20th theta; "
+        f"gemm {g.gemm_ms:.3f} ms, {g.gemm_flop / 1e9:.1f} GFLOP")
     assert err <= PATTERN_TOL
     assert abs(g.prad - prad) <= 1e-12 * abs(prad)
+    th2 = np.linspace(0.2, 2.9, 7)
+    g2 = nf.calculate(th2, ph[:50])
+    E2t, E2p, _, _ = N.far_field(faces, f, th2, ph[:50])
+    assert _rel_l2(g2.e_theta, E2t) <= PATTERN_TOL
+    g3 = nf.calculate(th, ph)
+    assert np.array_equal(g3.e_theta, g.e_theta) and np.array_equal(g3.e_phi, g.e_phi)
 
 
 def _fdtd_dipole(precision, n=48, steps=900, stride=3):
