@@ -237,6 +237,70 @@ def bench_nf2ff(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_sweep(args):
+    """The microstrip sweep: seeded structures (grids U{60..220} x U{60..120}
+    x U{30..48}, nz rounded up to a multiple of 4; 0.5 mm cells; PEC ground,
+    substrate eps_r U[2.2,10.2] 1-4 mm, PEC trace polyline of 2-6 segments
+    1-4 mm wide; 8-cell CPML; 50-ohm port at the trace start; 64-frequency
+    port DFT) run to -50 dB energy decay (whole-run peak) or max steps in one
+    sweep: one launch per kernel variant per half-step for all structures,
+    graph-replayed between energy checks every 100 steps, tile depth
+    autotuned first."""
+    import numpy as np
+    import torch
+
+    from openems_amd import Sweep
+    from openems_amd import workloads as W
+
+    structs = W.microstrip_sweep(args.sweep_count, seed=4)
+    log(f"[bench] sweep: building {len(structs)} structures, {sum(m.cells for m in structs) / 1e6:.1f} M cells")
+    t_build = time.perf_counter()
+    sw = Sweep()
+    keep = []
+    for ms in structs:
+        op, eng, cp, port, exc = W.build_microstrip_hip(ms, args.sweep_max_steps)
+        sw.add(eng, cp, port, exc)
+        keep.append((ms, eng, port))
+    torch.cuda.synchronize()
+    log(f"[bench] sweep: built in {time.perf_counter() - t_build:.1f} s; running")
+    t0 = time.perf_counter()
+    res = sw.run(args.sweep_max_steps, energy_decay_db=50.0, check_interval=100, use_graph=True, autotune=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    steps = np.array([r.timesteps_executed for r in res])
+    cells = np.array([ms.cells for ms, _, _ in keep])
+    updates = float((steps * cells).sum())
+    decayed = int(sum(r.termination_reason == 1 for r in res))
+    s11 = [port.s11(50.0) for _, _, port in keep]
+    finite = bool(all(np.isfinite(x).all() for x in s11))
+    tune = sw.tuning()
+    out = {
+        "metric": "microstrip sweep: wall time to -50 dB decay (or max steps) for every structure, plain method",
+        "value": round(wall, 3),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": int(steps.max()),
+        "warmup": 0,
+        "ms_per_step": round(wall / max(int(steps.max()), 1) * 1e3, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"microstrip sweep: {len(structs)} structures (seed 4), "
+                               f"{cells.sum() / 1e6:.1f} M cells, max {args.sweep_max_steps} steps, -50 dB decay",
+                   "tiles": tune},
+        "cell_updates_per_s": round(updates / wall / 1e6, 1),
+        "cell_updates_unit": "MCells/s",
+        "steps_min_median_max": [int(steps.min()), int(np.median(steps)), int(steps.max())],
+        "decayed": decayed,
+        "s11_finite": finite,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -251,6 +315,9 @@ def main():
     ap.add_argument("--tiles", type=str, default="", help="bx,by,ci,vec override")
     ap.add_argument("--summarize-rocprof", nargs="+", metavar="DB",
                     help="print a markdown summary of rocprofv3 result databases and exit")
+    ap.add_argument("--sweep", action="store_true", help="run the microstrip parameter sweep instead")
+    ap.add_argument("--sweep-count", type=int, default=256)
+    ap.add_argument("--sweep-max-steps", type=int, default=20000)
     ap.add_argument("--nf2ff", action="store_true",
                     help="time the NF2FF transform of the 384x384x256 workload's box instead (steps = repetitions)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -279,6 +346,9 @@ def main():
 
     if args.nf2ff:
         bench_nf2ff(args)
+        return
+    if args.sweep:
+        bench_sweep(args)
         return
 
     import torch

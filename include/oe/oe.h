@@ -402,6 +402,98 @@ oe_status oe_ext_nf2ff_calculate(oe_extension* ext, size_t freq_index, const dou
                                  const double* phi, size_t n_phi, double radius, const double center[3],
                                  oe_far_field* out);
 
+/* ------------------------------------------------------------------ sweep
+ * A parameter sweep: many structures stepped together on one device.  No
+ * counterpart in the reference, which runs one Simulation at a time
+ * (Simulation::run, src/fdtd/simulation.rs:255-326); each structure is an
+ * ordinary engine with its CPML (physical mode) and lumped-port DFT, and
+ * after a run its fields, port spectra and S11 are read through the usual
+ * calls.  Every half-step of all active structures is one launch per kernel
+ * variant; the steps between two energy checks are captured as one HIP graph
+ * and replayed.  Each structure stops on its own: energy is sampled after
+ * every step whose index is a multiple of check_interval (0 included), the
+ * peak is the whole-run maximum of those samples, and the structure stops
+ * when the energy falls energy_decay_db below it (the rule of
+ * src/fdtd/basic_engine.rs:128-141 with a persistent peak) or at max_steps.
+ * Structures share dt.  The engines and extensions must outlive the sweep's
+ * use of them; the engines must not be used concurrently. */
+typedef struct oe_sweep oe_sweep;
+typedef struct {
+    uint64_t max_steps;
+    int has_energy_decay_db;
+    double energy_decay_db;
+    uint32_t check_interval;  /* 0: 100 */
+    int use_graph;            /* replay captured graphs (else plain launches) */
+    int autotune;             /* time candidate tile depths on the zero state first */
+} oe_sweep_config;
+typedef struct {
+    uint64_t timesteps_executed;
+    int termination_reason;   /* oe_termination_reason */
+    double final_decay_db;
+    double peak_energy;
+    double final_energy;
+    double elapsed_seconds;   /* wall time of the whole run */
+} oe_sweep_entry;
+
+oe_status oe_sweep_create(int device, oe_sweep** out);
+void oe_sweep_destroy(oe_sweep* sw);
+/* Adds a structure: its engine, CPML (or null), port DFT (or null; at most
+ * 256 signals, every step) and excitations (sample s is applied at sweep
+ * step s).  *index receives its position. */
+oe_status oe_sweep_add(oe_sweep* sw, oe_engine* eng, oe_extension* cpml, oe_extension* port,
+                       const oe_excitation* excitations, size_t n_excitations, size_t* index);
+/* Runs every structure to its end; results has one entry per structure. */
+oe_status oe_sweep_run(oe_sweep* sw, const oe_sweep_config* cfg, oe_sweep_entry* results);
+/* The work-group depth along i in use and the autotuner's timings (ms per
+ * step of all structures' Yee updates) for its four candidates. */
+oe_status oe_sweep_tuning(const oe_sweep* sw, int* ci, double ms_per_step[4], int candidates[4]);
+oe_status oe_sweep_set_ci(oe_sweep* sw, int ci);
+
+/* ----------------------------------------------------- learned extrapolator
+ * No counterpart in the reference (whose only stop rule is the energy decay
+ * test, src/fdtd/basic_engine.rs:128-141).  A 1-D convolution + MLP network
+ * maps the early window of a port's V and I (L0 decimated samples, two
+ * channels, normalised) to `modes` damped complex exponentials that continue
+ * both signals for `tail` decimated samples:
+ *   conv (kernel 9, stride 2, padding 4, GELU) 2 -> c1 -> c2 -> c3,
+ *   fc1 (GELU) -> hidden, fc2 -> 6 numbers per mode: damping
+ *   softplus(o0) * 0.05 per sample, frequency sigmoid(o1) * 0.5 cycles per
+ *   sample, V residue o2 + j o3, I residue o4 + j o5;
+ *   V(n) = sum_m Re(aV_m exp((-alpha_m + j 2 pi f_m) n)), I likewise.
+ * Loss: mean squared error of both tails.  Every matrix product runs on the
+ * BF16 matrix cores with FP32 accumulation; training is Adam on FP32 master
+ * weights.  Parameters are float32 in the order W1 [c1][2][9], b1, W2
+ * [c2][c1][9], b2, W3 [c3][c2][9], b3, W4 [hidden][L3 * c3] (position-major
+ * flatten of the last convolution's [L3][c3] output), b4, W5
+ * [6 modes][hidden], b5.  Inputs are [batch][L0][2] (V, I interleaved),
+ * targets [batch][2][tail]. */
+typedef struct oe_extrap oe_extrap;
+typedef struct {
+    uint32_t L0;       /* window length (decimated samples), multiple of 8 */
+    uint32_t c1, c2, c3;
+    uint32_t hidden;
+    uint32_t modes;    /* <= 256 */
+    uint32_t batch;    /* largest batch a call may pass */
+    uint32_t tail;     /* predicted samples, <= 4096 */
+} oe_extrap_dims;
+/* params: n float32 in the order above, or NULL for He-normal weights drawn
+ * from seed and zero biases. */
+oe_status oe_extrap_create(int device, const oe_extrap_dims* dims, const float* params, uint64_t seed,
+                           oe_extrap** out);
+void oe_extrap_destroy(oe_extrap* x);
+size_t oe_extrap_param_count(const oe_extrap* x);
+oe_status oe_extrap_params(oe_extrap* x, float* out, size_t n);
+oe_status oe_extrap_set_params(oe_extrap* x, const float* in, size_t n);
+/* Modes [batch][6 modes] and (if tails is not NULL) tails [batch][2][tail]. */
+oe_status oe_extrap_forward(oe_extrap* x, const float* inputs, size_t batch, float* modes, float* tails);
+/* Loss and its gradient for every parameter (same order), no update. */
+oe_status oe_extrap_gradients(oe_extrap* x, const float* inputs, const float* targets, size_t batch,
+                              double* loss, float* grads);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) at learning rate lr; *loss is the
+ * loss before the step. */
+oe_status oe_extrap_train_step(oe_extrap* x, const float* inputs, const float* targets, size_t batch,
+                               float lr, double* loss);
+
 /* CPML psi arrays / ADE state for inspection: field 0 = E, 1 = H; which 0 =
  * psi_pos, 1 = psi_neg.  Returns the element count in *n. */
 oe_status oe_ext_cpml_read_psi(oe_extension* ext, int face, int field, int which, float* out,

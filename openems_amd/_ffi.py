@@ -74,6 +74,21 @@ class FarField(C.Structure):
                 ("gemm_flop", C.c_double)]
 
 
+class SweepConfig(C.Structure):
+    _fields_ = [("max_steps", C.c_uint64), ("has_energy_decay_db", C.c_int), ("energy_decay_db", C.c_double),
+                ("check_interval", C.c_uint32), ("use_graph", C.c_int), ("autotune", C.c_int)]
+
+
+class SweepEntry(C.Structure):
+    _fields_ = [("timesteps_executed", C.c_uint64), ("termination_reason", C.c_int), ("final_decay_db", C.c_double),
+                ("peak_energy", C.c_double), ("final_energy", C.c_double), ("elapsed_seconds", C.c_double)]
+
+
+class ExtrapDims(C.Structure):
+    _fields_ = [("L0", C.c_uint32), ("c1", C.c_uint32), ("c2", C.c_uint32), ("c3", C.c_uint32),
+                ("hidden", C.c_uint32), ("modes", C.c_uint32), ("batch", C.c_uint32), ("tail", C.c_uint32)]
+
+
 class CpmlConfig(C.Structure):
     _fields_ = [("layers", C.c_size_t * 6), ("grading_order", C.c_double), ("reflection_coeff", C.c_double),
                 ("alpha_max", C.c_double), ("kappa_max", C.c_double), ("mode", C.c_int)]
@@ -134,6 +149,20 @@ SIGNATURES = {
     "oe_nf2ff_calculate": (_i, [_vp, _dp, _sz, _dp, _sz, _d, _dp, C.POINTER(FarField)]),
     "oe_ext_nf2ff_create": (_i, [_vp, _szp, _i, _dp, _sz, _i, _u32, _pp]),
     "oe_ext_nf2ff_calculate": (_i, [_vp, _sz, _dp, _sz, _dp, _sz, _d, _dp, C.POINTER(FarField)]),
+    "oe_sweep_create": (_i, [_i, _pp]),
+    "oe_sweep_destroy": (None, [_vp]),
+    "oe_sweep_add": (_i, [_vp, _vp, _vp, _vp, C.POINTER(Excitation), _sz, _szp]),
+    "oe_sweep_run": (_i, [_vp, C.POINTER(SweepConfig), C.POINTER(SweepEntry)]),
+    "oe_sweep_tuning": (_i, [_vp, C.POINTER(_i), _dp, C.POINTER(_i)]),
+    "oe_sweep_set_ci": (_i, [_vp, _i]),
+    "oe_extrap_create": (_i, [_i, C.POINTER(ExtrapDims), _fp, _u64, _pp]),
+    "oe_extrap_destroy": (None, [_vp]),
+    "oe_extrap_param_count": (_sz, [_vp]),
+    "oe_extrap_params": (_i, [_vp, _fp, _sz]),
+    "oe_extrap_set_params": (_i, [_vp, _fp, _sz]),
+    "oe_extrap_forward": (_i, [_vp, _fp, _sz, _fp, _fp]),
+    "oe_extrap_gradients": (_i, [_vp, _fp, _fp, _sz, _dp, _fp]),
+    "oe_extrap_train_step": (_i, [_vp, _fp, _fp, _sz, C.c_float, _dp]),
     "oe_ext_cpml_read_psi": (_i, [_vp, _i, _i, _i, _fp, _sz, _szp]),
     "oe_ext_ade_read_state": (_i, [_vp, _fp, _sz, _szp]),
     "oe_extension_reset": (_i, [_vp]),

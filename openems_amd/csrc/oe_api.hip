@@ -7,8 +7,7 @@
 // fallback exists: every field update runs in a HIP kernel on the engine's
 // stream; a missing or failing device is reported as OE_ERR_DEVICE.
 
-#include "oe/oe.h"
-#include "oe_internal.h"
+#include "oe_objects.h"
 
 #include <algorithm>
 #include <chrono>
@@ -28,8 +27,9 @@ using namespace oe;
 // ------------------------------------------------------------------ errors
 namespace {
 thread_local std::string g_err;
+}  // namespace
 
-oe_status fail(oe_status code, const char* fmt, ...) {
+oe_status oe::fail(oe_status code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -38,112 +38,6 @@ oe_status fail(oe_status code, const char* fmt, ...) {
     g_err = buf;
     return code;
 }
-
-#define HIP_OK(expr)                                                                            \
-    do {                                                                                        \
-        hipError_t _e = (expr);                                                                 \
-        if (_e != hipSuccess)                                                                   \
-            return fail(OE_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
-                        __FILE__, __LINE__);                                                    \
-    } while (0)
-
-#define OE_TRY(expr)                      \
-    do {                                  \
-        oe_status _s = (expr);            \
-        if (_s != OE_OK) return _s;       \
-    } while (0)
-}  // namespace
-
-// ---------------------------------------------------------------- operator
-struct MatOp {
-    int kind;  // 0 material, 1 PEC, 2 lumped resistor
-    double eps_r, mu_r, se, sm;
-    size_t box[6];
-    int dir;   // lumped
-    double R;  // lumped
-};
-
-struct oe_operator {
-    size_t nx, ny, nz;
-    double d[3];
-    double dt;
-    std::vector<MatOp> ops;
-};
-
-// ------------------------------------------------------------ extensions
-enum ExtKind { EXT_CPML = 1, EXT_MUR = 2, EXT_ADE = 3, EXT_PROBE = 4, EXT_DFT = 5 };
-
-struct oe_extension {
-    int kind = 0;
-    oe_engine* eng = nullptr;
-    // CPML
-    CpmlArgs cpml{};
-    DevBuf cpml_mem;    // psi arrays
-    DevBuf cpml_tab;
-    size_t psi_count[6] = {0, 0, 0, 0, 0, 0};
-    // Mur
-    std::vector<MurDev> mur_faces;
-    DevBuf mur_mem;
-    DevBuf mur_dev;     // device copy of mur_faces
-    // ADE
-    AdeDev ade{};
-    DevBuf ade_mem;
-    // probe
-    uint32_t probe_idx = 0;
-    DevBuf probe_mem;   // data
-    DevBuf probe_cnt;   // uint64 counter
-    uint64_t probe_cap = 0;
-    // DFT (and lumped port)
-    DftDev dft{};
-    DevBuf dft_mem;
-    uint64_t dft_count = 0;   // accumulated samples
-    uint64_t dft_seen = 0;    // steps seen (time stride)
-    uint32_t dft_stride = 1;
-    std::vector<double> dft_freqs;
-    bool is_port = false;
-    // NF2FF box: faces whose phasors are consecutive DFT signals from nf_sig0[f]
-    bool is_nf2ff = false;
-    std::vector<Nf2ffFace> nf_faces;
-    std::vector<size_t> nf_sig0;
-    Nf2ffCachePtr nf_cache;
-};
-
-// ------------------------------------------------------------------ engine
-struct KernelStat {
-    uint64_t launches = 0;
-    double ms = 0.0;
-};
-
-struct oe_engine {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    Grid g{};
-    double d[3] = {0, 0, 0};
-    double dt = 0;
-    float* F = nullptr;   // 6N
-    float* Cf = nullptr;  // 12N
-    uint64_t timestep = 0;
-    Tiles tiles{};
-    int xcd_remap = 1;
-    double peak_persistent = 0.0;
-    oe_extension* cpml_owner = nullptr;
-    std::vector<oe_extension*> exts;  // live extensions bound to this engine
-    // energy
-    double* partials = nullptr;
-    double* energy_dev = nullptr;   // ring of [cap][2]
-    size_t energy_dev_cap = 0;
-    double* energy_host = nullptr;  // pinned
-    // point ops
-    DevBuf exc_samples;
-    DevBuf exc_table;
-    DevBuf probe_table;
-    // profiling: 0 off, n > 0 times every launch of every n-th step
-    int profiling = 0;
-    bool prof_step = false;
-    std::map<std::string, KernelStat> stats;
-    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
-    std::vector<hipEvent_t> event_pool;
-};
 
 namespace {
 

@@ -284,6 +284,123 @@ __global__ __launch_bounds__(256) void k_dft(const float* __restrict__ F, size_t
     }
 }
 
+// ---------------------------------------------------------------- sweep
+// The step index of a sweep lives on the device, so the kernels of one
+// step can be captured once in a graph and replayed for every step.
+
+// One thread per structure applies that structure's excitations in list order
+// (the order of k_point_ops), with the sample of the current step.
+__global__ void k_sweep_exc(const SweepExc* __restrict__ e, const int* __restrict__ first, int n_struct,
+                            const uint64_t* __restrict__ step) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_struct) return;
+    const uint64_t st = *step;
+    for (int q = first[s]; q < first[s + 1]; ++q) {
+        const float v = st < e[q].n_samples ? e[q].samples[st] : 0.f;
+        if (e[q].soft) *e[q].target += v;
+        else *e[q].target = v;
+    }
+}
+
+// One work-group per DFT extension: the signal values go to LDS, the phasors
+// of a chunk of frequencies too, then the threads update the (frequency,
+// signal) accumulators.  Same arithmetic as k_dft.
+__global__ __launch_bounds__(256) void k_sweep_dft(const SweepDft* __restrict__ list, const uint64_t* __restrict__ step,
+                                                   double dt) {
+    __shared__ double sv[256];
+    __shared__ uint8_t hv[256];
+    __shared__ double ph[2][DFT_FCHUNK][2];
+    const SweepDft& x = list[blockIdx.x];
+    const DftDev& d = x.d;
+    const uint64_t st = *step;
+    const uint64_t sample = x.sample0 + st;
+    const double cur = x.t0 + (double)st;
+    const double tE = (cur + 1.0) * dt, tH = (cur + 0.5) * dt;
+    const int m = threadIdx.x;
+    if (m < d.n_sig) {
+        double s = 0.0;
+        for (uint64_t t = d.off[m]; t < d.off[m + 1]; ++t) s += d.w[t] * (double)x.F[(size_t)d.comp[t] * x.N + d.cell[t]];
+        if (d.series && sample < d.series_cap) d.series[sample * d.n_sig + m] = s;
+        sv[m] = s;
+        hv[m] = d.is_h[m] ? 1 : 0;
+    }
+    for (int f0 = 0; f0 < d.n_f; f0 += DFT_FCHUNK) {
+        const int nf = min(DFT_FCHUNK, d.n_f - f0);
+        __syncthreads();
+        for (int q = threadIdx.x; q < 2 * nf; q += blockDim.x) {
+            const int hq = q / nf, f = q - hq * nf;
+            double sn, cs;
+            sincos(2.0 * M_PI * d.freqs[f0 + f] * (hq ? tH : tE), &sn, &cs);
+            ph[hq][f][0] = cs;
+            ph[hq][f][1] = -sn;
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < nf * d.n_sig; q += blockDim.x) {
+            const int f = q / d.n_sig, g = q - f * d.n_sig;
+            const double s = sv[g];
+            const double cr = ph[hv[g]][f][0], ci = ph[hv[g]][f][1];
+            const size_t a = (size_t)(f0 + f) * d.n_sig + g;
+            if (d.prec == 0) {
+                d.acc64[2 * a] += s * cr;
+                d.acc64[2 * a + 1] += s * ci;
+            } else {
+                const float svf = (float)s;
+                float* acc = d.acc32 + 4 * a;
+                const float yr = svf * (float)cr - acc[2];
+                const float tr = acc[0] + yr;
+                acc[2] = (tr - acc[0]) - yr;
+                acc[0] = tr;
+                const float yi = svf * (float)ci - acc[3];
+                const float ti = acc[1] + yi;
+                acc[3] = (ti - acc[1]) - yi;
+                acc[1] = ti;
+            }
+        }
+    }
+}
+
+__global__ void k_sweep_step(uint64_t* step) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+}
+
+// Per-structure energy with the partition of k_energy (EN_BLOCKS blocks per
+// structure, the same strides), so every structure's sum equals a
+// single-grid engine's.
+__global__ __launch_bounds__(EN_THREADS) void k_sweep_energy(const SweepStruct* __restrict__ st,
+                                                             const int* __restrict__ ids,
+                                                             double* __restrict__ partials) {
+    __shared__ double se[EN_THREADS], sh[EN_THREADS];
+    const SweepStruct& S = st[ids[blockIdx.y]];
+    const size_t N = S.g.N;
+    const float* F = S.F;
+    double ae = 0.0, ah = 0.0;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double e = (double)F[(size_t)d * N + i];
+            const double h = (double)F[(size_t)(d + 3) * N + i];
+            ae += e * e;
+            ah += h * h;
+        }
+    }
+    se[threadIdx.x] = ae;
+    sh[threadIdx.x] = ah;
+    __syncthreads();
+    for (int s = EN_THREADS / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            se[threadIdx.x] += se[threadIdx.x + s];
+            sh[threadIdx.x] += sh[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double* p = partials + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 2;
+        p[0] = se[0];
+        p[1] = sh[0];
+    }
+}
+
 int grid_for(size_t n, int threads, int cap) {
     size_t b = (n + threads - 1) / threads;
     if (b < 1) b = 1;
@@ -331,6 +448,33 @@ hipError_t launch_mur_post(float* F, const Grid& g, const MurDev* faces, int n, 
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_mur_post, dim3(256, n), dim3(256), 0, s, F, g, faces);
     for (int f = 0; f < n; ++f) hipLaunchKernelGGL(k_mur_apply, dim3(256), dim3(256), 0, s, F, g, faces, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_exc(const SweepExc* e, const int* first, int n_struct, const uint64_t* step, hipStream_t s) {
+    if (n_struct <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sweep_exc, dim3((n_struct + 63) / 64), dim3(64), 0, s, e, first, n_struct, step);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_dft(const SweepDft* list, int n, const uint64_t* step, double dt, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sweep_dft, dim3(n), dim3(256), 0, s, list, step, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_step(uint64_t* step, hipStream_t s) {
+    hipLaunchKernelGGL(k_sweep_step, dim3(1), dim3(64), 0, s, step);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_energy(const SweepStruct* st, const int* ids, int n, double* partials, double* out,
+                               hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sweep_energy, dim3(EN_BLOCKS, n), dim3(EN_THREADS), 0, s, st, ids, partials);
+    for (int q = 0; q < n; ++q)
+        hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(EN_THREADS), 0, s, partials + (size_t)q * 2 * EN_BLOCKS,
+                           EN_BLOCKS, out + 2 * q);
     return hipGetLastError();
 }
 

@@ -195,6 +195,28 @@ struct DftDev {
 hipError_t launch_dft(const float* F, size_t N, const DftDev& d, uint64_t sample, double tE, double tH,
                       hipStream_t s);
 
+// Sweep step pieces (oe_aux.hip).  The step index is read from *step.
+struct SweepExc {
+    float* target;           // field element
+    const float* samples;
+    uint64_t n_samples;
+    uint32_t soft;
+    uint32_t pad;
+};
+struct SweepDft {
+    const float* F;          // the structure's field block
+    size_t N;
+    DftDev d;                // at most 256 signals
+    uint64_t sample0;        // sample index of sweep step 0
+    double t0;               // timestep of sweep step 0
+};
+hipError_t launch_sweep_exc(const SweepExc* e, const int* first, int n_struct, const uint64_t* step, hipStream_t s);
+hipError_t launch_sweep_dft(const SweepDft* list, int n, const uint64_t* step, double dt, hipStream_t s);
+hipError_t launch_sweep_step(uint64_t* step, hipStream_t s);
+// Energies [n][2] of structures st[ids[0..n)] (partials: n * 2 * energy_blocks() doubles).
+hipError_t launch_sweep_energy(const SweepStruct* st, const int* ids, int n, double* partials, double* out,
+                               hipStream_t s);
+
 // NF2FF (oe_nf2ff.hip).  One face of the closed surface: normal `axis`,
 // outward sign, position c0 and node coordinates c1, c2 along the in-plane
 // axes a1 = (axis+1)%3, a2 = (axis+2)%3.  Its phasors sit on the device as

@@ -304,6 +304,70 @@ class Nf2ff:
                           self.frequency, theta, phi, radius, center)
 
 
+@dataclass
+class SweepResult:
+    timesteps_executed: int
+    termination_reason: int
+    final_decay_db: float
+    peak_energy: float
+    final_energy: float
+    elapsed_seconds: float
+
+
+class Sweep:
+    """Many structures stepped together (oe_sweep_*): each is a HipEngine with
+    its CPML, lumped-port DFT and excitations; all half-steps of all active
+    structures run as one launch per kernel variant, replayed from a captured
+    HIP graph between energy checks."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().oe_sweep_create(int(device), C.byref(h)))
+        self._h = h
+        self._keep = []
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().oe_sweep_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def add(self, eng: "HipEngine", cpml: Optional[Extension] = None, port: Optional[Extension] = None,
+            excitations: Sequence[ScheduledExcitation] = ()) -> int:
+        n = len(excitations)
+        ex = (_ffi.Excitation * max(n, 1))()
+        arrays = []
+        for q, x in enumerate(excitations):
+            a = np.ascontiguousarray(x.samples, dtype=np.float32)
+            arrays.append(a)
+            i, j, k = x.position
+            ex[q] = _ffi.Excitation(i, j, k, x.direction, int(bool(x.soft_source)),
+                                    a.ctypes.data_as(C.POINTER(C.c_float)), a.size)
+        idx = C.c_size_t()
+        check(lib().oe_sweep_add(self._h, eng._h, cpml._h if cpml else None, port._h if port else None, ex, n,
+                                 C.byref(idx)))
+        self._keep.append((eng, cpml, port))
+        return idx.value
+
+    def run(self, max_steps: int, energy_decay_db: Optional[float] = None, check_interval: int = 100,
+            use_graph: bool = True, autotune: bool = True) -> List[SweepResult]:
+        cfg = _ffi.SweepConfig(int(max_steps), int(energy_decay_db is not None), float(energy_decay_db or 0.0),
+                               int(check_interval), int(use_graph), int(autotune))
+        out = (_ffi.SweepEntry * len(self._keep))()
+        check(lib().oe_sweep_run(self._h, C.byref(cfg), out))
+        return [SweepResult(e.timesteps_executed, e.termination_reason, e.final_decay_db, e.peak_energy,
+                            e.final_energy, e.elapsed_seconds) for e in out]
+
+    def tuning(self):
+        ci = C.c_int()
+        ms = (C.c_double * 4)()
+        cand = (C.c_int * 4)()
+        check(lib().oe_sweep_tuning(self._h, C.byref(ci), ms, cand))
+        return {"ci": ci.value, "candidates": list(cand), "ms_per_step": list(ms)}
+
+    def set_ci(self, ci: int):
+        check(lib().oe_sweep_set_ci(self._h, int(ci)))
+
+
 def port_excitations(direction: int, region, samples: np.ndarray, sign: int = 1, amplitude: float = 1.0):
     """The excitation box of a lumped port: a soft source of
     -sign * amplitude * samples on every `direction` edge of region

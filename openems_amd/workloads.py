@@ -193,3 +193,104 @@ def build_port_spheres_hip(wl: PortSpheres, precision: int = 0, nf2ff_stride: in
     exts["port"] = eng.lumped_port(2, wl.port_region, wl.dft_freqs, precision=precision, record_steps=record_steps)
     exts["nf2ff"] = eng.nf2ff_box(wl.nf2ff_box, wl.nf2ff_freqs, precision=precision, time_stride=nf2ff_stride)
     return op, eng, exts
+
+
+@dataclass
+class Microstrip:
+    """One structure of the microstrip sweep, in cells of 0.5 mm: PEC ground
+    slab at k = 8 (just inside the -z CPML), substrate of eps_r over
+    k = 9 .. 9 + h, PEC trace polyline one cell thick at k = 9 + h, 8-cell
+    CPML on all faces, a 50-ohm z-directed lumped port from the ground to the
+    trace start, driven by an openEMS Gaussian (f0 = 0, fc = 20 GHz)."""
+    shape: Tuple[int, int, int]
+    eps_r: float
+    h: int
+    width: int
+    segments: List[Tuple[int, int, int, int]]  # trace boxes (i0, i1, j0, j1)
+    port: Tuple[int, int]                      # (i, j) of the port edge column
+    delta: float = 0.5e-3
+    layers: int = 8
+    ground_k: int = 8
+    R: float = 50.0
+    fc: float = 20e9
+
+    @property
+    def cells(self) -> int:
+        return int(np.prod(self.shape))
+
+    @property
+    def port_region(self):
+        i, j = self.port
+        return (i, i + 1, j, j + 1, self.ground_k + 1, self.ground_k + 1 + self.h)
+
+    def excitation_samples(self, dt: float, n: int) -> np.ndarray:
+        t0, tau = 9.0 / (2.0 * math.pi * self.fc), 3.0 / (2.0 * math.pi * self.fc)
+        t = np.arange(n, dtype=np.float64) * dt
+        return np.exp(-((t - t0) / tau) ** 2).astype(np.float32)
+
+    def apply(self, op):
+        """Ground, substrate, trace and port resistor on an operator (the HIP
+        Operator or the oracle, whose lumped call is add_lumped)."""
+        nx, ny, nz = self.shape
+        g = self.ground_k
+        op.set_pec((0, nx, 0, ny, g, g + 1))
+        op.set_material(self.eps_r, 1.0, 0.0, 0.0, (0, nx, 0, ny, g + 1, g + 1 + self.h))
+        for i0, i1, j0, j1 in self.segments:
+            op.set_pec((i0, i1, j0, j1, g + 1 + self.h, g + 2 + self.h))
+        lumped = getattr(op, "add_lumped_element", None) or getattr(op, "add_lumped")
+        lumped(2, self.port_region, self.R)
+
+
+def microstrip(rng: np.random.Generator, layers: int = 8) -> Microstrip:
+    """Grid U{60..220} x U{60..120} x U{30..48} cells (nz rounded up to a
+    multiple of 4 so that every row holds whole float4 lanes), eps_r
+    U[2.2, 10.2], substrate 1-4 mm, trace width 1-4 mm, 2-6 Manhattan
+    segments kept 6 cells clear of the CPML."""
+    nx = int(rng.integers(60, 221))
+    ny = int(rng.integers(60, 121))
+    nz = int(rng.integers(30, 49))
+    nz = (nz + 3) // 4 * 4
+    eps = float(rng.uniform(2.2, 10.2))
+    h = int(rng.integers(2, 9))
+    w = int(rng.integers(2, 9))
+    lo = layers + 6
+    hi = (nx - lo - w, ny - lo - w)
+    x, y = int(rng.integers(lo, hi[0])), int(rng.integers(lo, hi[1]))
+    port = (x + w // 2, y + w // 2)  # middle of the trace's first cell row
+    segs = []
+    n_seg = int(rng.integers(2, 7))
+    along_x = bool(rng.integers(0, 2))
+    for _ in range(n_seg):
+        if along_x:
+            nxp = int(rng.integers(lo, hi[0]))
+            a, b = sorted((x, nxp))
+            segs.append((a, b + w, y, y + w))
+            x = nxp
+        else:
+            nyp = int(rng.integers(lo, hi[1]))
+            a, b = sorted((y, nyp))
+            segs.append((x, x + w, a, b + w))
+            y = nyp
+        along_x = not along_x
+    return Microstrip((nx, ny, nz), eps, h, w, segs, port)
+
+
+def microstrip_sweep(count: int = 256, seed: int = 4) -> List[Microstrip]:
+    rng = np.random.default_rng(seed)
+    return [microstrip(rng) for _ in range(count)]
+
+
+def build_microstrip_hip(ms: Microstrip, max_steps: int, precision: int = 0, record_steps: int = 0,
+                         freqs=None):
+    """(op, eng, cpml, port, excitations) of one sweep structure on the HIP engine."""
+    from . import MODE_PHYSICAL, CpmlBoundaries, HipEngine, Operator
+    from .engine import port_excitations
+
+    op = Operator.uniform(*ms.shape, ms.delta)
+    ms.apply(op)
+    eng = HipEngine(op)
+    cp = eng.cpml(CpmlBoundaries(layers=(ms.layers,) * 6, mode=MODE_PHYSICAL))
+    fr = np.linspace(1e9, 20e9, 64) if freqs is None else freqs
+    port = eng.lumped_port(2, ms.port_region, fr, precision=precision, record_steps=record_steps)
+    exc = port_excitations(2, ms.port_region, ms.excitation_samples(op.dt, max_steps))
+    return op, eng, cp, port, exc

@@ -1,0 +1,219 @@
+"""Learned ringdown extrapolator, restated in float64 numpy.
+
+TEST INFRASTRUCTURE ONLY: the checker of the HIP training and inference
+kernels (openems_amd/csrc/oe_learn.hip).  No counterpart exists in the
+reference; the decay criterion it shortens is src/fdtd/basic_engine.rs:128-141.
+
+Network (per sample; B samples in a batch):
+  x [2, L0]  port V and I over the early window, decimated and normalised
+  conv_l     y[co, t] = b[co] + sum_{ci,k} W[co, ci, k] x[ci, 2 t + k - 4]
+             (kernel 9, stride 2, zero padding 4), then GELU (tanh form),
+             for channels 2 -> c1 -> c2 -> c3
+  fc1        h = GELU(W4 flat + b4), flat = conv3 output [L3 * c3] (position major,
+             channels last, as the device kernels keep activations [B, L, C])
+  fc2        o = W5 h + b5, 6 numbers per mode
+  modes      alpha = softplus(o0) * A_SCALE (decay per decimated sample),
+             f = sigmoid(o1) * F_MAX (cycles per decimated sample),
+             z = exp(-alpha + j 2 pi f), residues aV = o2 + j o3, aI = o4 + j o5
+  tail       V(n) = sum_m Re(aV_m z_m^n), I(n) likewise, n = 0 .. T - 1
+  loss       mean over batch, n and {V, I} of (tail - target)^2
+
+With use_bf16=True both operands of every matrix product, forward and
+backward (weights, activations, im2col columns, back-propagated gradients),
+are rounded to bfloat16 first, as the matrix-core kernels see them; sums,
+biases, GELU and the mode synthesis stay float64.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Dict, List, Tuple
+
+import numpy as np
+
+KERNEL, STRIDE, PAD = 9, 2, 4
+A_SCALE = 0.05
+F_MAX = 0.5
+
+
+@dataclass
+class Dims:
+    L0: int = 256
+    c: Tuple[int, int, int] = (64, 128, 128)
+    hidden: int = 384
+    modes: int = 32
+
+    @property
+    def L(self):
+        out = [self.L0]
+        for _ in range(3):
+            out.append(out[-1] // STRIDE)
+        return out
+
+    @property
+    def flat(self):
+        return self.c[2] * self.L[3]
+
+    def shapes(self) -> Dict[str, Tuple[int, ...]]:
+        c0 = (2,) + tuple(self.c)
+        s = {}
+        for l in range(3):
+            s[f"W{l + 1}"] = (c0[l + 1], c0[l], KERNEL)
+            s[f"b{l + 1}"] = (c0[l + 1],)
+        s["W4"] = (self.hidden, self.flat)
+        s["b4"] = (self.hidden,)
+        s["W5"] = (6 * self.modes, self.hidden)
+        s["b5"] = (6 * self.modes,)
+        return s
+
+
+ORDER = ["W1", "b1", "W2", "b2", "W3", "b3", "W4", "b4", "W5", "b5"]
+
+
+def init_params(d: Dims, seed: int = 0) -> Dict[str, np.ndarray]:
+    rng = np.random.default_rng(seed)
+    p = {}
+    for k, sh in d.shapes().items():
+        if k.startswith("W"):
+            fan_in = int(np.prod(sh[1:]))
+            p[k] = rng.standard_normal(sh) * math.sqrt(2.0 / fan_in)
+        else:
+            p[k] = np.zeros(sh)
+    return p
+
+
+def bf16(a: np.ndarray) -> np.ndarray:
+    """Round to bfloat16 (nearest even) and back to float64."""
+    f = np.asarray(a, np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def gelu(x):
+    c = math.sqrt(2.0 / math.pi)
+    return 0.5 * x * (1.0 + np.tanh(c * (x + 0.044715 * x ** 3)))
+
+
+def gelu_grad(x):
+    c = math.sqrt(2.0 / math.pi)
+    u = c * (x + 0.044715 * x ** 3)
+    t = np.tanh(u)
+    return 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * c * (1.0 + 3 * 0.044715 * x * x)
+
+
+def im2col(x: np.ndarray) -> np.ndarray:
+    """x [B, C, L] -> columns [B, L/2, C*K] with column index ci*K + k."""
+    B, C, L = x.shape
+    Lo = L // STRIDE
+    xp = np.zeros((B, C, L + 2 * PAD))
+    xp[:, :, PAD:PAD + L] = x
+    idx = STRIDE * np.arange(Lo)[:, None] + np.arange(KERNEL)[None, :]
+    cols = xp[:, :, idx]                      # [B, C, Lo, K]
+    return cols.transpose(0, 2, 1, 3).reshape(B, Lo, C * KERNEL)
+
+
+def col2im(dcols: np.ndarray, C: int, L: int) -> np.ndarray:
+    B, Lo, _ = dcols.shape
+    d = dcols.reshape(B, Lo, C, KERNEL).transpose(0, 2, 1, 3)
+    dx = np.zeros((B, C, L + 2 * PAD))
+    idx = STRIDE * np.arange(Lo)[:, None] + np.arange(KERNEL)[None, :]
+    for t in range(Lo):
+        dx[:, :, idx[t]] += d[:, :, t, :]
+    return dx[:, :, PAD:PAD + L]
+
+
+def modes_of(o: np.ndarray, n_modes: int):
+    o = o.reshape(o.shape[0], n_modes, 6)
+    alpha = np.logaddexp(0.0, o[..., 0]) * A_SCALE
+    f = F_MAX / (1.0 + np.exp(-o[..., 1]))
+    aV = o[..., 2] + 1j * o[..., 3]
+    aI = o[..., 4] + 1j * o[..., 5]
+    return alpha, f, aV, aI
+
+
+def synth(alpha, f, aV, aI, T: int):
+    """Tails [B, 2, T]."""
+    n = np.arange(T)
+    z = np.exp((-alpha + 2j * math.pi * f)[..., None] * n)       # [B, M, T]
+    V = np.real((aV[..., None] * z).sum(1))
+    I = np.real((aI[..., None] * z).sum(1))
+    return np.stack([V, I], 1)
+
+
+def forward(p, x, T: int, d: Dims, use_bf16: bool = False):
+    """Returns (tails [B, 2, T], cache)."""
+    q = (lambda a: bf16(a)) if use_bf16 else (lambda a: a)
+    cache = {"x": x}
+    h = x
+    for l in range(3):
+        W = p[f"W{l + 1}"]
+        cols = im2col(h)                                              # [B, Lo, Ci*K]
+        pre = np.einsum("btk,ok->bot", q(cols), q(W.reshape(W.shape[0], -1))) + p[f"b{l + 1}"][None, :, None]
+        cache[f"cols{l}"] = cols
+        cache[f"pre{l}"] = pre
+        cache[f"in{l}"] = h
+        h = gelu(pre)
+    flat = h.transpose(0, 2, 1).reshape(h.shape[0], -1)  # [B, L3 * c3], channels last
+    cache["flat"] = flat
+    pre4 = q(flat) @ q(p["W4"]).T + p["b4"]
+    cache["pre4"] = pre4
+    h4 = gelu(pre4)
+    cache["h4"] = h4
+    o = q(h4) @ q(p["W5"]).T + p["b5"]
+    cache["o"] = o
+    tails = synth(*modes_of(o, d.modes), T)
+    return tails, cache
+
+
+def loss_and_grads(p, x, target, d: Dims, use_bf16: bool = False):
+    """Mean squared tail error and its gradient for every parameter."""
+    q = (lambda a: bf16(a)) if use_bf16 else (lambda a: a)
+    B, _, T = target.shape
+    tails, c = forward(p, x, T, d, use_bf16)
+    r = tails - target
+    loss = float((r * r).mean())
+    dt = 2.0 * r / r.size                                             # dL/dtails
+    o = c["o"]
+    M = d.modes
+    om = o.reshape(B, M, 6)
+    alpha, f, aV, aI = modes_of(o, M)
+    n = np.arange(T)
+    w = (-alpha + 2j * math.pi * f)[..., None]                        # [B, M, 1]
+    z = np.exp(w * n)                                                 # [B, M, T]
+    gV = dt[:, 0][:, None, :]
+    gI = dt[:, 1][:, None, :]
+    # d Re(a z^n) / d Re(a) = Re(z^n), / d Im(a) = -Im(z^n)
+    dav_r = (gV * z.real).sum(-1)
+    dav_i = (gV * -z.imag).sum(-1)
+    dai_r = (gI * z.real).sum(-1)
+    dai_i = (gI * -z.imag).sum(-1)
+    # d Re(a z^n) / d w = Re(a n z^n) for real part of w, -Im(a n z^n) for imag
+    s = (gV * (aV[..., None] * n * z)).sum(-1) + (gI * (aI[..., None] * n * z)).sum(-1)
+    d_alpha = -s.real                                                 # w_re = -alpha
+    d_f = 2 * math.pi * (-s.imag)                                     # w_im = 2 pi f
+    do = np.zeros_like(om)
+    do[..., 0] = d_alpha * A_SCALE / (1.0 + np.exp(-om[..., 0]))
+    sig = 1.0 / (1.0 + np.exp(-om[..., 1]))
+    do[..., 1] = d_f * F_MAX * sig * (1 - sig)
+    do[..., 2], do[..., 3], do[..., 4], do[..., 5] = dav_r, dav_i, dai_r, dai_i
+    do = do.reshape(B, 6 * M)
+    g = {}
+    g["W5"] = q(do).T @ q(c["h4"])
+    g["b5"] = do.sum(0)
+    dh4 = q(do) @ q(p["W5"])
+    dpre4 = dh4 * gelu_grad(c["pre4"])
+    g["W4"] = q(dpre4).T @ q(c["flat"])
+    g["b4"] = dpre4.sum(0)
+    dflat = q(dpre4) @ q(p["W4"])
+    dh = dflat.reshape(B, -1, d.c[2]).transpose(0, 2, 1)
+    for l in (2, 1, 0):
+        W = p[f"W{l + 1}"]
+        dpre = dh * gelu_grad(c[f"pre{l}"])                           # [B, Co, Lo]
+        cols = c[f"cols{l}"]
+        g[f"W{l + 1}"] = np.einsum("bot,btk->ok", q(dpre), q(cols)).reshape(W.shape)
+        g[f"b{l + 1}"] = dpre.sum((0, 2))
+        if l > 0:
+            dcols = np.einsum("bot,ok->btk", q(dpre), q(W.reshape(W.shape[0], -1)))
+            dh = col2im(dcols, W.shape[1], c[f"in{l}"].shape[2])
+    return loss, g

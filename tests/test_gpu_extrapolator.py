@@ -1,0 +1,80 @@
+"""The extrapolator's HIP kernels against the float64 restatement
+(oracle/extrapolator.py run with bf16 operands, as the matrix cores see them).
+
+Tolerances: the device sums in float32 in its own order and rounds its own
+float32 intermediates to bf16, so an operand can land one bf16 ulp (2^-8)
+away from the oracle's; per tensor the gradients must agree to a relative L2
+of 2e-2, the loss to 1e-3, the predicted tails to 1e-2.
+"""
+import numpy as np
+import pytest
+
+from openems_amd.extrapolator import Extrapolator, ExtrapolatorDims, pack, unpack
+from oracle import extrapolator as X
+from tests.parity_util import say
+
+pytestmark = pytest.mark.gpu
+
+SMALL = ExtrapolatorDims(L0=64, c1=8, c2=16, c3=16, hidden=32, modes=4, batch=6, tail=50)
+ODIMS = X.Dims(L0=64, c=(8, 16, 16), hidden=32, modes=4)
+
+
+def _case(seed=0):
+    rng = np.random.default_rng(seed)
+    p = X.init_params(ODIMS, seed=seed)
+    for k in p:
+        if k.startswith("b"):
+            p[k] = 0.05 * rng.standard_normal(p[k].shape)
+    # round the parameters to float32 so both sides start from the same values
+    p = {k: v.astype(np.float32).astype(np.float64) for k, v in p.items()}
+    n = np.arange(ODIMS.L0)
+    x = np.stack([np.stack([np.exp(-0.02 * n) * np.sin(0.3 * n + b), np.exp(-0.03 * n) * np.cos(0.2 * n + 2 * b)])
+                  for b in range(SMALL.batch)]).astype(np.float32)
+    m = np.arange(SMALL.tail)
+    tgt = np.stack([np.stack([0.5 * np.exp(-0.05 * m) * np.cos(0.25 * m + b), 0.3 * np.exp(-0.04 * m) * np.sin(0.15 * m)])
+                    for b in range(SMALL.batch)]).astype(np.float32)
+    return p, x, tgt
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_forward_matches_oracle():
+    p, x, _ = _case(1)
+    ex = Extrapolator(SMALL, params=pack(p, SMALL))
+    modes, tails = ex.forward(x)
+    ref, c = X.forward(p, x.astype(np.float64), SMALL.tail, ODIMS, use_bf16=True)
+    e_o = _rel(modes, c["o"])
+    e_t = _rel(tails, ref)
+    say(f"[extrap] forward: modes rel L2 {e_o:.2e}, tails rel L2 {e_t:.2e}")
+    assert e_o < 1e-2 and e_t < 1e-2
+
+
+def test_gradients_match_oracle():
+    p, x, tgt = _case(2)
+    ex = Extrapolator(SMALL, params=pack(p, SMALL))
+    loss, g = ex.gradients(x, tgt)
+    ref_loss, ref_g = X.loss_and_grads(p, x.astype(np.float64), tgt.astype(np.float64), ODIMS, use_bf16=True)
+    say(f"[extrap] loss {loss:.6e} vs oracle {ref_loss:.6e}")
+    assert abs(loss - ref_loss) <= 1e-3 * abs(ref_loss)
+    gd = unpack(g, SMALL)
+    for k in X.ORDER:
+        e = _rel(gd[k], ref_g[k])
+        say(f"[extrap] grad {k}: rel L2 {e:.2e}")
+        assert e < 2e-2, k
+    # the float64 oracle without bf16 rounding is within bf16 resolution
+    _, ref64 = X.loss_and_grads(p, x.astype(np.float64), tgt.astype(np.float64), ODIMS, use_bf16=False)
+    for k in X.ORDER:
+        assert _rel(gd[k], ref64[k]) < 0.2, k
+
+
+def test_params_roundtrip_and_training_reduces_loss():
+    p, x, tgt = _case(3)
+    flat = pack(p, SMALL)
+    ex = Extrapolator(SMALL, params=flat)
+    assert np.array_equal(ex.params(), flat)
+    losses = [ex.train_step(x, tgt, 3e-3) for _ in range(150)]
+    say(f"[extrap] training: loss {losses[0]:.4e} -> {losses[-1]:.4e}")
+    assert losses[-1] < 0.2 * losses[0]
+    assert np.all(np.isfinite(ex.params()))

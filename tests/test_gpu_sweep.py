@@ -1,0 +1,97 @@
+"""The multi-structure sweep against single-engine runs of the same
+structures (and one structure against the oracle).
+
+A sweep must give every structure exactly what its own engine gives with
+the matching termination configuration (decay test every check_interval
+steps, energy sampled at the same steps, whole-run peak): the same number of
+steps, the same reason, bit-identical fields and port signals, identical
+port spectra.
+"""
+import numpy as np
+import pytest
+
+from openems_amd import EngineBatch, Sweep
+from openems_amd import workloads as W
+from oracle import oracle as O
+from oracle import port as P
+from tests.parity_util import say
+
+pytestmark = pytest.mark.gpu
+
+MAX_STEPS = 700
+CHECK = 50
+DECAY = 20.0
+
+
+def _structures():
+    rng = np.random.default_rng(21)
+    out = []
+    for shape in ((44, 40, 32), (60, 48, 36), (52, 44, 28), (40, 52, 40)):
+        ms = W.microstrip(rng)
+        # shrink to test size, keeping the generator's geometry rules
+        w = min(ms.width, 4)
+        x0, y0 = 16, 16
+        segs = [(x0, shape[0] - 16, y0, y0 + w), (shape[0] - 16 - w, shape[0] - 16, y0, shape[1] - 16)]
+        out.append(W.Microstrip(shape, ms.eps_r, min(ms.h, 5), w, segs, (x0 + w // 2, y0 + w // 2)))
+    # one grid whose rows do not hold whole float4 lanes (scalar lanes)
+    out.append(W.Microstrip((42, 38, 30), 4.4, 3, 3, [(14, 28, 14, 17)], (15, 15)))
+    return out
+
+
+def _single(ms):
+    op, eng, cp, port, exc = W.build_microstrip_hip(ms, MAX_STEPS, record_steps=MAX_STEPS)
+    res = eng.run_batch(EngineBatch(max_timesteps=MAX_STEPS, excitations=exc, extensions=[cp, port],
+                                    energy_decay_db=DECAY, check_interval=CHECK, sample_interval=CHECK,
+                                    track_peak=True, persistent_peak=True))
+    return eng, port, res
+
+
+def _sweep(structs, use_graph, autotune):
+    sw = Sweep()
+    keep = []
+    for ms in structs:
+        op, eng, cp, port, exc = W.build_microstrip_hip(ms, MAX_STEPS, record_steps=MAX_STEPS)
+        sw.add(eng, cp, port, exc)
+        keep.append((eng, port))
+    res = sw.run(MAX_STEPS, energy_decay_db=DECAY, check_interval=CHECK, use_graph=use_graph, autotune=autotune)
+    return sw, keep, res
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_sweep_matches_single_engines(use_graph):
+    structs = _structures()
+    sw, keep, res = _sweep(structs, use_graph, autotune=use_graph)
+    say(f"[sweep] graph={use_graph} tuning {sw.tuning()}")
+    for q, ms in enumerate(structs):
+        eng1, port1, r1 = _single(ms)
+        eng2, port2 = keep[q]
+        say(f"[sweep] {ms.shape}: sweep {res[q].timesteps_executed} steps (reason {res[q].termination_reason}), "
+            f"single {r1.timesteps_executed} (reason {r1.termination_reason})")
+        assert res[q].timesteps_executed == r1.timesteps_executed
+        assert res[q].termination_reason == r1.termination_reason
+        assert eng2.timestep == eng1.timestep
+        E1, H1 = eng1.read_fields()
+        E2, H2 = eng2.read_fields()
+        assert np.array_equal(E1, E2) and np.array_equal(H1, H2)
+        assert np.array_equal(port1.series(), port2.series())
+        assert np.array_equal(port1.dft_values(), port2.dft_values())
+        assert np.array_equal(port1.s11(50.0), port2.s11(50.0))
+    assert any(r.termination_reason == 1 for r in res), "no structure reached the decay criterion"
+
+
+def test_sweep_structure_against_oracle():
+    ms = _structures()[1]
+    steps = 300
+    sw = Sweep()
+    op, eng, cp, port, exc = W.build_microstrip_hip(ms, steps, record_steps=steps)
+    sw.add(eng, cp, port, exc)
+    sw.run(steps, check_interval=CHECK)
+    sim = O.OracleSim(*ms.shape, ms.delta)
+    ms.apply(sim)
+    sim.add_cpml((ms.layers,) * 6, mode=O.MODE_PHYSICAL)
+    sim.add_signals(P.lumped_port_terms(ms.shape, (ms.delta,) * 3, 2, ms.port_region))
+    sim.run_batch(steps, [(e.position[0], e.position[1], e.position[2], e.direction, True) for e in exc],
+                  np.array([e.samples.astype(np.float64) for e in exc]))
+    E, H = eng.read_fields()
+    assert np.array_equal(np.concatenate([E, H]), sim.fields())
+    assert np.array_equal(port.series(), sim.signal_series(0, 2))
