@@ -301,6 +301,77 @@ def bench_sweep(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_learned(args):
+    """Learned-stop vs plain microstrip sweeps.  Training data: full runs of
+    --learned-train structures (seed 5) recorded to --learned-max-steps;
+    held-out: --learned-test structures (seed 6) run both plain (to -50 dB)
+    and learned (window 2048 steps + 256-step check, fallback to the plain
+    run when the check's squared error exceeds 1e-4 of the window energy).
+    Reports wall times, the accepted fraction and max |dS11| over 1-20 GHz
+    against the plain runs."""
+    import numpy as np
+
+    from openems_amd import learned_stop as LS
+    from openems_amd import workloads as W
+    from openems_amd.extrapolator import Extrapolator, ExtrapolatorDims
+
+    window, dec, check, T = 2048, 8, 256, args.learned_max_steps
+    tail = (T - window) // dec
+    rng_tr = np.random.default_rng(5)
+    rng_te = np.random.default_rng(6)
+    train_s = [W.microstrip(rng_tr) for _ in range(args.learned_train)]
+    test_s = [W.microstrip(rng_te) for _ in range(args.learned_test)]
+    X, Y = [], []
+    t_data = time.perf_counter()
+    for b0 in range(0, len(train_s), 256):
+        run = LS.run_sweep(train_s[b0:b0 + 256], T, 50.0, record_steps=T)
+        x, y = LS.dataset(run, window, dec, tail)
+        X.append(x)
+        Y.append(y)
+        log(f"[bench] learned: training data {b0 + len(run.u)}/{len(train_s)}, sweep {run.wall:.1f} s")
+    X = np.concatenate(X)
+    Y = np.concatenate(Y)
+    t_data = time.perf_counter() - t_data
+    dims = ExtrapolatorDims(L0=window // dec, batch=min(256, len(X)), tail=tail)
+    model = Extrapolator(dims, seed=0)
+    t_train = time.perf_counter()
+    losses = LS.train(model, X, Y, args.learned_epochs, 1e-3, log=lambda m: log(f"[bench] learned: {m}"))
+    t_train = time.perf_counter() - t_train
+    plain = LS.run_sweep(test_s, T, 50.0, record_steps=0)
+    learned = LS.learned_sweep(model, test_s, T, window, dec, check)
+    ds11 = np.abs(learned.s11 - plain.s11).max(axis=1)
+    ok = ds11 <= 1e-3
+    out = {
+        "metric": "sweep time-to-|dS11|<=1e-3, learned-stop vs plain",
+        "value": round(learned.wall_learned / plain.wall, 4),
+        "unit": "learned wall / plain wall",
+        "n_gpus": 1,
+        "steps": int(plain.steps.max()),
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 fields, bf16 network (fp32 accumulate)",
+        "data": "synthetic",
+        "config": {"workload": f"microstrip sweep, {len(train_s)} training runs (seed 5), {len(test_s)} held-out "
+                               f"(seed 6), max {T} steps, window {window}, check {check}, decimation {dec}",
+                   "params": int(model.n_params)},
+        "plain_wall_s": round(plain.wall, 3),
+        "learned_wall_s": round(learned.wall_learned, 3),
+        "fallback_wall_s": round(learned.wall_fallback, 3),
+        "accepted": int(learned.accepted.sum()),
+        "max_dS11_accepted": float(ds11[learned.accepted].max()) if learned.accepted.any() else None,
+        "within_1e-3": int(ok.sum()),
+        "train_loss_first_last": [losses[0], losses[-1]],
+        "data_s": round(t_data, 1),
+        "train_s": round(t_train, 1),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -318,6 +389,11 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="run the microstrip parameter sweep instead")
     ap.add_argument("--sweep-count", type=int, default=256)
     ap.add_argument("--sweep-max-steps", type=int, default=20000)
+    ap.add_argument("--learned", action="store_true", help="learned-stop vs plain sweep (trains the extrapolator)")
+    ap.add_argument("--learned-train", type=int, default=512)
+    ap.add_argument("--learned-test", type=int, default=64)
+    ap.add_argument("--learned-max-steps", type=int, default=12000)
+    ap.add_argument("--learned-epochs", type=int, default=40)
     ap.add_argument("--nf2ff", action="store_true",
                     help="time the NF2FF transform of the 384x384x256 workload's box instead (steps = repetitions)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -349,6 +425,9 @@ def main():
         return
     if args.sweep:
         bench_sweep(args)
+        return
+    if args.learned:
+        bench_learned(args)
         return
 
     import torch

@@ -1,0 +1,48 @@
+"""Host logic of the learned-stop sweep (no GPU): the closed-form DFT of a
+mode sum against a direct sum, the training-pair layout, and CalcPort."""
+import math
+
+import numpy as np
+
+from openems_amd.extrapolator import training_pair
+from openems_amd.learned_stop import calc_s11, mode_params, tail_dft
+
+
+def test_tail_dft_closed_form_matches_direct_sum():
+    rng = np.random.default_rng(0)
+    s = -rng.uniform(0.001, 0.05, 3) + 2j * math.pi * rng.uniform(0.01, 0.2, 3)
+    a = rng.standard_normal(3) + 1j * rng.standard_normal(3)
+    dt, D, start, k0, k1 = 1e-12, 8, 100, 140, 900
+    freqs = np.array([1e9, 7e9, 19e9])
+    k = np.arange(k0, k1)
+    x = np.real((a[:, None] * np.exp(s[:, None] * (k - start) / D)).sum(0))
+    for off in (1.0, 0.5):
+        ref = np.array([2 * dt * (x * np.exp(-2j * math.pi * f * (k + off) * dt)).sum() for f in freqs])
+        got = tail_dft(s, a, start, k0, k1, D, dt, off, freqs)
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_mode_params_match_network_definition():
+    o = np.array([[0.3, -1.0, 1.0, 2.0, -0.5, 0.25]], np.float32)
+    s, aV, aI = mode_params(o, 1)
+    alpha = math.log1p(math.exp(0.3)) * 0.05
+    f = 0.5 / (1 + math.exp(1.0))
+    assert abs(s[0, 0] - (-alpha + 2j * math.pi * f)) < 1e-7
+    assert aV[0, 0] == 1 + 2j and aI[0, 0] == -0.5 + 0.25j
+
+
+def test_training_pair_layout():
+    v = np.arange(40, dtype=float)
+    i = -np.arange(40, dtype=float)
+    x, t, (sv, si) = training_pair(v, i, window=16, decimate=4, tail=5)
+    assert x.shape == (2, 4) and t.shape == (2, 5)
+    assert sv == 15 and si == 15
+    assert np.allclose(x[0], np.array([0, 4, 8, 12]) / 15)
+    assert np.allclose(t[0], np.array([16, 20, 24, 28, 32]) / 15)
+    assert np.allclose(t[1], -np.array([16, 20, 24, 28, 32]) / 15)
+
+
+def test_calc_s11_limits():
+    I = np.array([1 + 1j, 2.0])
+    assert np.allclose(calc_s11(50 * I, I), 0)
+    assert np.allclose(calc_s11(0 * I, I), -1)
