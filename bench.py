@@ -98,7 +98,35 @@ def summarize_rocprof(paths):
                 for r in rows[:10]:
                     out.append(f"| `{r[0][:90]}` | {r[1]} | {r[2]} | {r[3]:.0f} | {r[4]:.0f} | {r[5]:.0f} |")
                 out.append("")
+            names = {r[1] for r in rows}
+            if {"SQ_VALU_MFMA_BUSY_CYCLES", "GRBM_GUI_ACTIVE"} <= names:
+                out.append(mfma_summary(con))
     return "\n".join(out)
+
+
+def mfma_summary(con, xcds: int = 8, simds: int = 1024):
+    """Matrix-core busy fraction per kernel and grid size: the sum of
+    SQ_VALU_MFMA_BUSY_CYCLES over all SIMDs divided by (GRBM_GUI_ACTIVE per
+    XCD x SIMDs); GRBM_GUI_ACTIVE as stored is summed over the 8 XCDs.  FLOP
+    from SQ_INSTS_VALU_MFMA_MOPS_F32 x 512."""
+    rows = con.execute("select dispatch_id, kernel_name, grid_size, counter_name, sum(value) from counters_collection "
+                       "group by dispatch_id, counter_name").fetchall()
+    per = {}
+    for disp, name, grid, cn, v in rows:
+        d = per.setdefault(disp, {"name": name, "grid": grid})
+        d[cn] = v
+    groups = {}
+    for d in per.values():
+        if "SQ_VALU_MFMA_BUSY_CYCLES" not in d or d.get("SQ_VALU_MFMA_BUSY_CYCLES", 0) == 0:
+            continue
+        groups.setdefault((d["name"][:70], d["grid"]), []).append(d)
+    out = ["| kernel | work-items | dispatches | MFMA busy % of SIMD-cycles | MFMA GFLOP |", "|---|---|---|---|---|"]
+    for (name, grid), ds in sorted(groups.items()):
+        busy = sum(d["SQ_VALU_MFMA_BUSY_CYCLES"] for d in ds)
+        gui = sum(d["GRBM_GUI_ACTIVE"] for d in ds) / xcds
+        fl = sum(d.get("SQ_INSTS_VALU_MFMA_MOPS_F32", 0) for d in ds) * 512 / 1e9
+        out.append(f"| `{name}` | {grid} | {len(ds)} | {100 * busy / (gui * simds):.1f} | {fl:.2f} |")
+    return "\n".join(out) + "\n"
 
 
 def measured_traffic(kernel_key):

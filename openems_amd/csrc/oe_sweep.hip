@@ -84,9 +84,10 @@ struct oe_sweep {
 
 namespace {
 
-// Boxes of one structure as the single-grid engine splits it: with CPML the
-// x/y face slabs (all-faces kernel) and the interior (z-faces kernel), else
-// the whole grid (plain kernel).
+// Boxes of one structure: with CPML the x-face slabs, the y-face slabs, the
+// corner columns and the interior (z faces only), each with the kernel
+// variant compiled for the faces it can hold; without CPML the whole grid
+// (plain kernel).
 void structure_boxes(const oe_sweep* sw, int sid, const Item& it, std::vector<Group>& groups) {
     const Grid& g = it.eng->g;
     const int vec = it.vec;
@@ -124,13 +125,18 @@ void structure_boxes(const oe_sweep* sw, int sid, const Item& it, std::vector<Gr
     const int m = cp->on_mask;
     const int ix0 = (m & 1) ? cp->L[0] : 0, ix1 = (m & 2) ? cp->g0[1] : g.nx;
     const int jy0 = (m & 4) ? cp->L[2] : 0, jy1 = (m & 8) ? cp->g0[3] : g.ny;
+    // x slabs without the y-slab rows (x and z faces), y slabs without the
+    // x-slab planes (y and z faces), the four corner columns (all faces):
+    // the lighter kernels keep 3 waves per SIMD where the all-faces one has 2.
     const int all = AX_X | AX_Y | AX_Z;
-    add(all, 0, ix0, 0, g.ny);
-    add(all, ix1, g.nx, 0, g.ny);
-    if (ix1 > ix0) {
-        add(all, ix0, ix1, 0, jy0);
-        add(all, ix0, ix1, jy1, g.ny);
-    }
+    add(AX_X | AX_Z, 0, ix0, jy0, jy1);
+    add(AX_X | AX_Z, ix1, g.nx, jy0, jy1);
+    add(AX_Y | AX_Z, ix0, ix1, 0, jy0);
+    add(AX_Y | AX_Z, ix0, ix1, jy1, g.ny);
+    add(all, 0, ix0, 0, jy0);
+    add(all, 0, ix0, jy1, g.ny);
+    add(all, ix1, g.nx, 0, jy0);
+    add(all, ix1, g.nx, jy1, g.ny);
     add((m & 48) ? AX_Z : 0, ix0, ix1, jy0, jy1);
 }
 

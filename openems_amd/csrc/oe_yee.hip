@@ -530,7 +530,8 @@ hipError_t launch_e_t(const YeeArgs& a, const CpmlArgs& c, hipStream_t s) {
 }
 
 // Instantiated variants: plain (no CPML), z faces only (interior of a CPML
-// grid) and all faces (the x/y slab boxes), each in compat and physical mode.
+// grid), x and z faces (x slabs), y and z faces (y slabs) and all faces
+// (corner columns), each in compat and physical mode.
 template <bool H, int V>
 hipError_t dispatch(const YeeArgs& a, const CpmlArgs* c, int axes, hipStream_t s) {
     const CpmlArgs none{};
@@ -539,7 +540,15 @@ hipError_t dispatch(const YeeArgs& a, const CpmlArgs* c, int axes, hipStream_t s
         if (c->physical) return H ? launch_h_t<V, AX_Z, true>(a, *c, s) : launch_e_t<V, AX_Z, true>(a, *c, s);
         return H ? launch_h_t<V, AX_Z, false>(a, *c, s) : launch_e_t<V, AX_Z, false>(a, *c, s);
     }
-    constexpr int ALL = AX_X | AX_Y | AX_Z;
+    constexpr int XZ = AX_X | AX_Z, YZ = AX_Y | AX_Z, ALL = AX_X | AX_Y | AX_Z;
+    if (axes == XZ) {
+        if (c->physical) return H ? launch_h_t<V, XZ, true>(a, *c, s) : launch_e_t<V, XZ, true>(a, *c, s);
+        return H ? launch_h_t<V, XZ, false>(a, *c, s) : launch_e_t<V, XZ, false>(a, *c, s);
+    }
+    if (axes == YZ) {
+        if (c->physical) return H ? launch_h_t<V, YZ, true>(a, *c, s) : launch_e_t<V, YZ, true>(a, *c, s);
+        return H ? launch_h_t<V, YZ, false>(a, *c, s) : launch_e_t<V, YZ, false>(a, *c, s);
+    }
     if (c->physical) return H ? launch_h_t<V, ALL, true>(a, *c, s) : launch_e_t<V, ALL, true>(a, *c, s);
     return H ? launch_h_t<V, ALL, false>(a, *c, s) : launch_e_t<V, ALL, false>(a, *c, s);
 }
@@ -565,9 +574,13 @@ static hipError_t sweep_t(const SweepArgs& a, int n_wg, hipStream_t s) {
 
 template <bool H, int V>
 static hipError_t sweep_dispatch(const SweepArgs& a, int n_wg, int axes, hipStream_t s) {
-    if (axes == 0) return sweep_t<H, V, 0>(a, n_wg, s);
-    if (axes == AX_Z) return sweep_t<H, V, AX_Z>(a, n_wg, s);
-    return sweep_t<H, V, AX_X | AX_Y | AX_Z>(a, n_wg, s);
+    switch (axes) {
+        case 0: return sweep_t<H, V, 0>(a, n_wg, s);
+        case AX_Z: return sweep_t<H, V, AX_Z>(a, n_wg, s);
+        case AX_X | AX_Z: return sweep_t<H, V, AX_X | AX_Z>(a, n_wg, s);
+        case AX_Y | AX_Z: return sweep_t<H, V, AX_Y | AX_Z>(a, n_wg, s);
+        default: return sweep_t<H, V, AX_X | AX_Y | AX_Z>(a, n_wg, s);
+    }
 }
 
 hipError_t launch_sweep(bool h, const SweepArgs& a, int n_wg, int axes, int vec, hipStream_t s) {
