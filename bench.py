@@ -329,6 +329,65 @@ def bench_sweep(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_composite(args):
+    """The composite workload at full size: 384x384x256 cells of 0.5 mm, CPML
+    (8 cells, physical) on five faces, first-order Mur on +z, a 50-ohm port
+    (1x1x4 cells) with an openEMS Gaussian (fc = 20 GHz), 8 Drude and 8
+    Lorentz spheres (seeded), port DFT over 64 frequencies (1-20 GHz, FP64),
+    NF2FF box 10 cells inside the CPML recording 5/10/15 GHz every
+    --nf2ff-stride steps; then the far field at 181x360 angles for the three
+    frequencies."""
+    import numpy as np
+    import torch
+
+    from openems_amd import EngineBatch
+    from openems_amd import workloads as W
+    from openems_amd.engine import port_excitations
+
+    wl = W.port_spheres(steps=args.steps)
+    op, eng, exts = W.build_port_spheres_hip(wl, precision=0, nf2ff_stride=args.nf2ff_stride)
+    smp = wl.excitation_samples(op.dt, args.warmup + args.steps)
+    exc = port_excitations(2, wl.port_region, smp)
+    log(f"[bench] composite {wl.shape}: {len(wl.spheres)} spheres, warmup {args.warmup}")
+    if args.warmup:
+        eng.run_batch(EngineBatch(num_steps=args.warmup, excitations=exc, extensions=list(exts.values())))
+    torch.cuda.synchronize()
+    exc = port_excitations(2, wl.port_region, smp[args.warmup:])
+    t0 = time.perf_counter()
+    eng.run_batch(EngineBatch(num_steps=args.steps, excitations=exc, extensions=list(exts.values())))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    th = np.linspace(0, np.pi, 181)
+    ph = np.linspace(0, 2 * np.pi, 361)[:-1]
+    t1 = time.perf_counter()
+    ffs = [exts["nf2ff"].far_field(q, th, ph) for q in range(len(wl.nf2ff_freqs))]
+    t_ff = time.perf_counter() - t1
+    s11 = exts["port"].s11(50.0)
+    out = {
+        "metric": "MCells/s, composite workload (every extension on)",
+        "value": round(wl.cells * args.steps / wall / 1e6, 1),
+        "unit": "MCells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 fields, f64 DFT accumulators",
+        "data": "synthetic",
+        "config": {"workload": f"port_spheres {wl.shape[0]}x{wl.shape[1]}x{wl.shape[2]}: CPML on 5 faces, Mur +z, "
+                               f"50-ohm port, {len(wl.spheres)} Drude/Lorentz spheres (ADE, physical), 64-frequency "
+                               f"port DFT, NF2FF box {wl.nf2ff_box} every {args.nf2ff_stride} steps",
+                   "nf2ff_far_field_s": round(t_ff, 3)},
+        "dmax": [round(f.dmax, 4) for f in ffs],
+        "s11_finite": bool(np.isfinite(s11).all()),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_learned(args):
     """Learned-stop vs plain microstrip sweeps.  Training data: full runs of
     --learned-train structures (seed 5) recorded to --learned-max-steps;
@@ -417,6 +476,8 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="run the microstrip parameter sweep instead")
     ap.add_argument("--sweep-count", type=int, default=256)
     ap.add_argument("--sweep-max-steps", type=int, default=20000)
+    ap.add_argument("--composite", action="store_true", help="run the 384x384x256 composite workload instead")
+    ap.add_argument("--nf2ff-stride", type=int, default=4)
     ap.add_argument("--learned", action="store_true", help="learned-stop vs plain sweep (trains the extrapolator)")
     ap.add_argument("--learned-train", type=int, default=512)
     ap.add_argument("--learned-test", type=int, default=64)
@@ -456,6 +517,9 @@ def main():
         return
     if args.learned:
         bench_learned(args)
+        return
+    if args.composite:
+        bench_composite(args)
         return
 
     import torch
