@@ -284,6 +284,26 @@ typedef enum { OE_ADE_LORENTZ = 0, OE_ADE_DRUDE = 1, OE_ADE_DEBYE = 2 } oe_ade_k
 oe_status oe_ext_ade_create(oe_engine* eng, int kind, const double params[3],
                             const uint32_t* positions, size_t n, int mode, oe_extension** out);
 
+/* Upml (src/extensions/pml.rs:209-667) with PmlBoundaries {layers,
+ * grading_order, reflection_coeff}: up to six non-overlapping regions (x
+ * faces over the whole y-z plane, y faces without the x regions, z faces
+ * without both), per cell and component the coefficients vv, vvfo, vvfn
+ * (E) and ii, iifo, iifn (H) of Taflove eq. 7.88/7.90 as the reference forms
+ * them, and a voltage and a current flux.  Hooks: before the H update
+ * f = ii H - iifo flux, H <- flux, flux <- f; after it flux <- H,
+ * H <- f + iifn H; the same with the v-coefficients around the E update
+ * (pre/post_current_update, pre/post_voltage_update, :500-657).  This is the
+ * reference's formulation exactly, including its instability (its own tests
+ * are disabled for it); CPML is the working absorber. */
+oe_status oe_ext_upml_create(oe_engine* eng, const size_t layers[6], double grading_order,
+                             double reflection_coeff, oe_extension** out);
+
+/* UPML state: region index, array 0..7 (vv, vvfo, vvfn, ii, iifo, iifn,
+ * voltage flux, current flux), component 0..2.  *n receives the element
+ * count, start/size the region box (any may be NULL). */
+oe_status oe_ext_upml_read(oe_extension* ext, int region, int which, int component, float* out,
+                           size_t capacity, size_t* n, size_t start[3], size_t size[3]);
+
 /* FieldProbe (src/processing/probes.rs:263-303): records Ex..Hz at a node
  * after every step.  Data are read with oe_ext_probe_read. */
 oe_status oe_ext_field_probe_create(oe_engine* eng, uint32_t i, uint32_t j, uint32_t k,

@@ -163,6 +163,8 @@ SIGNATURES = {
     "oe_extrap_forward": (_i, [_vp, _fp, _sz, _fp, _fp]),
     "oe_extrap_gradients": (_i, [_vp, _fp, _fp, _sz, _dp, _fp]),
     "oe_extrap_train_step": (_i, [_vp, _fp, _fp, _sz, C.c_float, _dp]),
+    "oe_ext_upml_create": (_i, [_vp, _szp, _d, _d, _pp]),
+    "oe_ext_upml_read": (_i, [_vp, _i, _i, _i, _fp, _sz, _szp, _szp, _szp]),
     "oe_ext_cpml_read_psi": (_i, [_vp, _i, _i, _i, _fp, _sz, _szp]),
     "oe_ext_ade_read_state": (_i, [_vp, _fp, _sz, _szp]),
     "oe_extension_reset": (_i, [_vp]),

@@ -702,6 +702,127 @@ oe_status oe_ext_mur_create(oe_engine* e, const int enabled[6], double phase_vel
     return OE_OK;
 }
 
+oe_status oe_ext_upml_create(oe_engine* e, const size_t layers[6], double m, double R, oe_extension** out) {
+    if (!e || !layers || !out) return fail(OE_ERR_CONFIG, "null argument");
+    if (!(R > 0.0 && R < 1.0)) return fail(OE_ERR_CONFIG, "reflection_coeff must be in (0, 1)");
+    if (!(m >= 0.0)) return fail(OE_ERR_CONFIG, "grading_order must be non-negative");
+    for (oe_extension* x : e->exts)
+        if (x->kind == EXT_UPML) return fail(OE_ERR_CONFIG, "the engine already has a UPML");
+    const size_t n[3] = {(size_t)e->g.nx, (size_t)e->g.ny, (size_t)e->g.nz};
+    const double dt = e->dt;
+    struct Reg { size_t st[3], sz[3]; int dir; bool is_min; };
+    std::vector<Reg> regs;
+    // Upml::build_regions (src/extensions/pml.rs:222-296)
+    auto add = [&](size_t a0, size_t a1, size_t a2, size_t b0, size_t b1, size_t b2, int dir, bool is_min) {
+        regs.push_back(Reg{{a0, a1, a2}, {b0, b1, b2}, dir, is_min});
+    };
+    const size_t nx = n[0], ny = n[1], nz = n[2];
+    if (layers[0] > 0 && layers[0] < nx) add(0, 0, 0, layers[0] + 1, ny, nz, 0, true);
+    if (layers[1] > 0 && layers[1] < nx) add(nx - layers[1] - 1, 0, 0, layers[1] + 1, ny, nz, 0, false);
+    const size_t xs = layers[0] > 0 ? layers[0] + 1 : 0, xe = layers[1] > 0 ? nx - layers[1] - 1 : nx;
+    if (layers[2] > 0 && layers[2] < ny && xs < xe) add(xs, 0, 0, xe - xs, layers[2] + 1, nz, 1, true);
+    if (layers[3] > 0 && layers[3] < ny && xs < xe) add(xs, ny - layers[3] - 1, 0, xe - xs, layers[3] + 1, nz, 1, false);
+    const size_t ys = layers[2] > 0 ? layers[2] + 1 : 0, ye = layers[3] > 0 ? ny - layers[3] - 1 : ny;
+    if (layers[4] > 0 && layers[4] < nz && xs < xe && ys < ye) add(xs, ys, 0, xe - xs, ye - ys, layers[4] + 1, 2, true);
+    if (layers[5] > 0 && layers[5] < nz && xs < xe && ys < ye)
+        add(xs, ys, nz - layers[5] - 1, xe - xs, ye - ys, layers[5] + 1, 2, false);
+    if (regs.empty()) return fail(OE_ERR_CONFIG, "no UPML region: every layer count is 0 or exceeds the grid");
+    UpmlDev u{};
+    u.n_reg = (int)regs.size();
+    long total = 0;
+    for (int r = 0; r < u.n_reg; ++r) {
+        for (int d = 0; d < 3; ++d) { u.start[r][d] = (int)regs[r].st[d]; u.size[r][d] = (int)regs[r].sz[d]; }
+        u.first[r] = total;
+        total += (long)(regs[r].sz[0] * regs[r].sz[1] * regs[r].sz[2]);
+    }
+    u.first[u.n_reg] = total;
+    // Upml::init_region_coefficients + calculate_kappa (src/extensions/pml.rs:299-461)
+    std::vector<float> a((size_t)8 * 3 * total, 0.f);
+    auto A = [&](int kind, int c, long q) -> float& { return a[((size_t)kind * 3 + c) * total + q]; };
+    for (int r = 0; r < u.n_reg; ++r) {
+        const Reg& g = regs[r];
+        const int dir = g.dir;
+        const size_t L = g.is_min ? layers[2 * dir] : layers[2 * dir + 1];
+        const double delta = e->d[dir], width = (double)L * delta;
+        const double smax = -(m + 1.0) * std::log(R) / (2.0 * Z0 * width);
+        long q = u.first[r];
+        for (size_t li = 0; li < g.sz[0]; ++li)
+            for (size_t lj = 0; lj < g.sz[1]; ++lj)
+                for (size_t lk = 0; lk < g.sz[2]; ++lk, ++q) {
+                    const size_t pos[3] = {li + g.st[0], lj + g.st[1], lk + g.st[2]};
+                    double kv[3], ki[3];
+                    for (int c = 0; c < 3; ++c) {
+                        double depth = 0.0;
+                        if (g.is_min) {
+                            if (pos[dir] <= L) depth = (double)(L - pos[dir]) * delta;
+                        } else {
+                            const size_t bs = n[dir] - L - 1;
+                            if (pos[dir] >= bs) depth = (double)(pos[dir] - bs) * delta;
+                        }
+                        if (depth <= 0.0) { kv[c] = 0.0; ki[c] = 0.0; continue; }
+                        kv[c] = smax * std::pow(depth / width, m);
+                        const double di = depth + (c == dir ? -0.5 : 0.5) * delta;
+                        double nd = di / width;
+                        nd = nd < 0.0 ? 0.0 : (nd > 1.0 ? 1.0 : nd);
+                        ki[c] = smax * std::pow(nd, m);
+                    }
+                    for (int c = 0; c < 3; ++c) {
+                        const int npp = (c + 2) % 3;
+                        if (kv[0] + kv[1] + kv[2] > 0.0) {
+                            const double den = 2.0 * EPS0 + kv[npp] * dt;
+                            A(0, c, q) = (float)((2.0 * EPS0 - kv[npp] * dt) / den);
+                            A(1, c, q) = (float)((2.0 * EPS0 - kv[c] * dt) / den);
+                            A(2, c, q) = (float)((2.0 * EPS0 + kv[c] * dt) / den);
+                        } else {
+                            A(0, c, q) = 1.f; A(1, c, q) = 0.f; A(2, c, q) = 1.f;
+                        }
+                        if (ki[0] + ki[1] + ki[2] > 0.0) {
+                            const double den = 2.0 * EPS0 + ki[npp] * dt;
+                            A(3, c, q) = (float)((2.0 * EPS0 - ki[npp] * dt) / den);
+                            A(4, c, q) = (float)((2.0 * EPS0 - ki[c] * dt) / den);
+                            A(5, c, q) = (float)((2.0 * EPS0 + ki[c] * dt) / den);
+                        } else {
+                            A(3, c, q) = 1.f; A(4, c, q) = 0.f; A(5, c, q) = 1.f;
+                        }
+                    }
+                }
+    }
+    HIP_OK(hipSetDevice(e->device));
+    std::unique_ptr<oe_extension> x(new oe_extension());
+    x->kind = EXT_UPML;
+    x->eng = e;
+    HIP_OK(x->upml_mem.alloc(a.size() * sizeof(float)));
+    HIP_OK(hipMemcpy(x->upml_mem.p, a.data(), a.size() * sizeof(float), hipMemcpyHostToDevice));
+    u.a = x->upml_mem.as<float>();
+    x->upml = u;
+    e->exts.push_back(x.get());
+    *out = x.release();
+    return OE_OK;
+}
+
+oe_status oe_ext_upml_read(oe_extension* x, int region, int which, int component, float* out, size_t capacity,
+                           size_t* n, size_t start[3], size_t size[3]) {
+    if (!x || x->kind != EXT_UPML) return fail(OE_ERR_CONFIG, "not a UPML extension");
+    if (!x->eng) return fail(OE_ERR_CONFIG, "the extension's engine was destroyed");
+    const UpmlDev& u = x->upml;
+    if (region < 0 || region >= u.n_reg || which < 0 || which > 7 || component < 0 || component > 2)
+        return fail(OE_ERR_CONFIG, "region, array or component out of range");
+    const long total = u.first[u.n_reg];
+    const size_t cnt = (size_t)(u.first[region + 1] - u.first[region]);
+    if (n) *n = cnt;
+    for (int d = 0; d < 3; ++d) {
+        if (start) start[d] = (size_t)u.start[region][d];
+        if (size) size[d] = (size_t)u.size[region][d];
+    }
+    if (out && capacity) {
+        HIP_OK(hipSetDevice(x->eng->device));
+        HIP_OK(hipStreamSynchronize(x->eng->stream));
+        HIP_OK(hipMemcpy(out, u.a + ((size_t)which * 3 + component) * total + u.first[region],
+                         std::min(capacity, cnt) * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return OE_OK;
+}
+
 oe_status oe_ext_ade_create(oe_engine* e, int kind, const double params[3], const uint32_t* positions, size_t n,
                             int mode, oe_extension** out) {
     if (!e || !params || !out || (n && !positions)) return fail(OE_ERR_CONFIG, "null argument");
@@ -1235,6 +1356,10 @@ oe_status oe_extension_reset(oe_extension* x) {
         HIP_OK(hipMemsetAsync(x->ade.J, 0, 9 * cnt * sizeof(float), e->stream));
     }
     if (x->kind == EXT_PROBE) HIP_OK(hipMemsetAsync(x->probe_cnt.p, 0, sizeof(uint64_t), e->stream));
+    if (x->kind == EXT_UPML) {  // Upml::reset (src/extensions/pml.rs:660-667): the two flux arrays
+        const long total = x->upml.first[x->upml.n_reg];
+        HIP_OK(hipMemsetAsync(x->upml.a + (size_t)6 * 3 * total, 0, (size_t)2 * 3 * total * sizeof(float), e->stream));
+    }
     if (x->kind == EXT_DFT) {
         void* acc = x->dft.prec == OE_DFT_FP64 ? (void*)x->dft.acc64 : (void*)x->dft.acc32;
         HIP_OK(hipMemsetAsync(acc, 0, (size_t)x->dft.n_f * x->dft.n_sig * 16, e->stream));
@@ -1271,6 +1396,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
 
     // Sort extensions by hook.
     const CpmlArgs* cp = nullptr;
+    oe_extension* upml = nullptr;
     std::vector<oe_extension*> murs, ades, probes, dfts;
     for (size_t q = 0; q < b->n_extensions; ++q) {
         oe_extension* x = b->extensions[q];
@@ -1281,6 +1407,10 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             case EXT_CPML:
                 if (cp) return fail(OE_ERR_CONFIG, "more than one CPML extension in a batch");
                 cp = &x->cpml;
+                break;
+            case EXT_UPML:
+                if (upml) return fail(OE_ERR_CONFIG, "more than one UPML extension in a batch");
+                upml = x;
                 break;
             case EXT_MUR: murs.push_back(x); break;
             case EXT_ADE: ades.push_back(x); break;
@@ -1372,9 +1502,17 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
     for (uint64_t st = 0; st < num_steps; ++st) {
         const uint64_t cur = e->timestep;
         e->prof_step = e->profiling > 0 && (st % (uint64_t)e->profiling) == 0;
+        if (upml) {
+            Timed tm(e, "upml");
+            HIP_OK(launch_upml_pre(e->F, e->g, upml->upml, false, e->stream));
+        }
         for (const Launch& L : plan) {
             Timed tm(e, (L.axes & (AX_X | AX_Y)) ? "update_h:slabs" : (cp ? "update_h:interior" : "update_h:grid"));
             HIP_OK(launch_update_h(L.a, cp, L.axes, e->stream));
+        }
+        if (upml) {
+            Timed tm(e, "upml");
+            HIP_OK(launch_upml_post(e->F, e->g, upml->upml, false, e->stream));
         }
         for (auto* x : murs) {
             Timed tm(e, "mur_pre");
@@ -1385,9 +1523,17 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
                 Timed tm(e, "ade_pre");
                 HIP_OK(launch_ade_pre(e->F, e->g.N, x->ade, e->stream));
             }
+        if (upml) {
+            Timed tm(e, "upml");
+            HIP_OK(launch_upml_pre(e->F, e->g, upml->upml, true, e->stream));
+        }
         for (const Launch& L : plan) {
             Timed tm(e, (L.axes & (AX_X | AX_Y)) ? "update_e:slabs" : (cp ? "update_e:interior" : "update_e:grid"));
             HIP_OK(launch_update_e(L.a, cp, L.axes, e->stream));
+        }
+        if (upml) {
+            Timed tm(e, "upml");
+            HIP_OK(launch_upml_post(e->F, e->g, upml->upml, true, e->stream));
         }
         for (auto* x : ades) {
             Timed tm(e, "ade_post");

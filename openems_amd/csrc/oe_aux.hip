@@ -401,6 +401,62 @@ __global__ __launch_bounds__(EN_THREADS) void k_sweep_energy(const SweepStruct* 
     }
 }
 
+// ----------------------------------------------------------------- UPML
+// Upml::pre/post_voltage_update and pre/post_current_update
+// (src/extensions/pml.rs:500-657), one thread per region cell, the three
+// components in order:
+//   pre:  f = c F - cfo flux;  F <- flux;  flux <- f
+//   post: f = flux;  flux <- F;  F <- f + cfn F
+__device__ __forceinline__ bool upml_cell(const Grid& g, const UpmlDev& u, long q, size_t* p) {
+    int r = 0;
+#pragma unroll
+    for (int t = 1; t < 6; ++t)
+        if (t < u.n_reg && q >= u.first[t]) r = t;
+    const long l = q - u.first[r];
+    const int sz1 = u.size[r][1], sz2 = u.size[r][2];
+    const int lk = (int)(l % sz2);
+    const long t = l / sz2;
+    const int lj = (int)(t % sz1), li = (int)(t / sz1);
+    *p = ((size_t)(li + u.start[r][0]) * g.ny + (lj + u.start[r][1])) * g.nz + (lk + u.start[r][2]);
+    return true;
+}
+
+__global__ void k_upml_pre(float* __restrict__ F, Grid g, UpmlDev u, int e) {
+    const long total = u.first[u.n_reg];
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
+        size_t p;
+        upml_cell(g, u, q, &p);
+        const int kc = e ? 0 : 3, kfo = e ? 1 : 4, kfl = e ? 6 : 7;
+        for (int c = 0; c < 3; ++c) {
+            float* f = F + (size_t)(e ? c : 3 + c) * g.N + p;
+            float* fl = u.a + ((size_t)kfl * 3 + c) * total + q;
+            const float cc = u.a[((size_t)kc * 3 + c) * total + q];
+            const float cfo = u.a[((size_t)kfo * 3 + c) * total + q];
+            const float old = *f, fo = *fl;
+            const float fh = cc * old - cfo * fo;
+            *f = fo;
+            *fl = fh;
+        }
+    }
+}
+
+__global__ void k_upml_post(float* __restrict__ F, Grid g, UpmlDev u, int e) {
+    const long total = u.first[u.n_reg];
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
+        size_t p;
+        upml_cell(g, u, q, &p);
+        const int kfn = e ? 2 : 5, kfl = e ? 6 : 7;
+        for (int c = 0; c < 3; ++c) {
+            float* f = F + (size_t)(e ? c : 3 + c) * g.N + p;
+            float* fl = u.a + ((size_t)kfl * 3 + c) * total + q;
+            const float cfn = u.a[((size_t)kfn * 3 + c) * total + q];
+            const float fh = *fl, fn = *f;
+            *fl = fn;
+            *f = fh + cfn * fn;
+        }
+    }
+}
+
 int grid_for(size_t n, int threads, int cap) {
     size_t b = (n + threads - 1) / threads;
     if (b < 1) b = 1;
@@ -475,6 +531,20 @@ hipError_t launch_sweep_energy(const SweepStruct* st, const int* ids, int n, dou
     for (int q = 0; q < n; ++q)
         hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(EN_THREADS), 0, s, partials + (size_t)q * 2 * EN_BLOCKS,
                            EN_BLOCKS, out + 2 * q);
+    return hipGetLastError();
+}
+
+hipError_t launch_upml_pre(float* F, const Grid& g, const UpmlDev& u, bool e, hipStream_t s) {
+    const long total = u.first[u.n_reg];
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_upml_pre, dim3(grid_for((size_t)total, 256, 16384)), dim3(256), 0, s, F, g, u, e ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_upml_post(float* F, const Grid& g, const UpmlDev& u, bool e, hipStream_t s) {
+    const long total = u.first[u.n_reg];
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_upml_post, dim3(grid_for((size_t)total, 256, 16384)), dim3(256), 0, s, F, g, u, e ? 1 : 0);
     return hipGetLastError();
 }
 

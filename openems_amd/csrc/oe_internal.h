@@ -178,6 +178,20 @@ struct AdeDev {
 hipError_t launch_ade_pre(const float* F, size_t N, const AdeDev& a, hipStream_t s);
 hipError_t launch_ade_post(float* F, const float* Cf, size_t N, const AdeDev& a, hipStream_t s);
 
+// UPML (src/extensions/pml.rs): up to 6 regions whose cells are numbered
+// consecutively (region r holds cells first[r] .. first[r+1]); arrays are
+// [8][3][total]: vv, vvfo, vvfn, ii, iifo, iifn, voltage flux, current flux.
+struct UpmlDev {
+    int n_reg;
+    int start[6][3];
+    int size[6][3];
+    long first[7];
+    float* a;
+};
+// e: the E (voltage) hooks, else the H (current) hooks.
+hipError_t launch_upml_pre(float* F, const Grid& g, const UpmlDev& u, bool e, hipStream_t s);
+hipError_t launch_upml_post(float* F, const Grid& g, const UpmlDev& u, bool e, hipStream_t s);
+
 // Running DFT of weighted field signals (one thread per signal).
 struct DftDev {
     int n_sig, n_f, prec;

@@ -59,7 +59,7 @@ struct oe_operator {
 };
 
 // ------------------------------------------------------------ extensions
-enum ExtKind { EXT_CPML = 1, EXT_MUR = 2, EXT_ADE = 3, EXT_PROBE = 4, EXT_DFT = 5 };
+enum ExtKind { EXT_CPML = 1, EXT_MUR = 2, EXT_ADE = 3, EXT_PROBE = 4, EXT_DFT = 5, EXT_UPML = 6 };
 
 struct oe_extension {
     int kind = 0;
@@ -73,6 +73,9 @@ struct oe_extension {
     std::vector<MurDev> mur_faces;
     DevBuf mur_mem;
     DevBuf mur_dev;     // device copy of mur_faces
+    // UPML
+    oe::UpmlDev upml{};
+    DevBuf upml_mem;
     // ADE
     AdeDev ade{};
     DevBuf ade_mem;

@@ -181,6 +181,20 @@ class Extension:
         check(lib().oe_ext_cpml_read_psi(self._h, face, f, w, out.ctypes.data_as(C.POINTER(C.c_float)), n.value, C.byref(n)))
         return out
 
+    # UPML
+    UPML_ARRAYS = ("vv", "vvfo", "vvfn", "ii", "iifo", "iifn", "volt_flux", "curr_flux")
+
+    def upml_array(self, region: int, name: str, component: int) -> np.ndarray:
+        w = self.UPML_ARRAYS.index(name)
+        n = C.c_size_t()
+        st = (C.c_size_t * 3)()
+        sz = (C.c_size_t * 3)()
+        check(lib().oe_ext_upml_read(self._h, region, w, component, None, 0, C.byref(n), st, sz))
+        out = np.empty(n.value, np.float32)
+        check(lib().oe_ext_upml_read(self._h, region, w, component, out.ctypes.data_as(C.POINTER(C.c_float)),
+                                     n.value, C.byref(n), st, sz))
+        return out.reshape(tuple(sz))
+
     # ADE
     def state(self) -> np.ndarray:
         n = C.c_size_t()
@@ -491,6 +505,14 @@ class HipEngine:
         h = C.c_void_p()
         check(lib().oe_ext_cpml_create(self._h, C.byref(c), C.byref(h)))
         return Extension(self, h.value, "cpml")
+
+    def upml(self, layers: Sequence[int] = (8,) * 6, grading_order: float = 3.0,
+             reflection_coeff: float = 1e-6) -> Extension:
+        """Upml with PmlBoundaries (src/extensions/pml.rs), the reference's formulation."""
+        arr = (C.c_size_t * 6)(*map(int, layers))
+        h = C.c_void_p()
+        check(lib().oe_ext_upml_create(self._h, arr, float(grading_order), float(reflection_coeff), C.byref(h)))
+        return Extension(self, h.value, "upml")
 
     def mur(self, enabled: Sequence[bool] = (True,) * 6, phase_velocity: float = 0.0) -> Extension:
         en = (C.c_int * 6)(*[int(bool(x)) for x in enabled])

@@ -89,6 +89,15 @@ struct Ade {
     std::vector<T> J[3], Jold[3], Eprev[3];
 };
 
+// UPML region (src/extensions/pml.rs:131-178): coefficient and flux arrays
+// over the region box, z fastest; [3] = component.
+template <class T>
+struct UpmlRegion {
+    size_t start[3], size[3];
+    std::vector<T> vv[3], vvfo[3], vvfn[3], ii[3], iifo[3], iifn[3], vflux[3], cflux[3];
+    size_t at(size_t i, size_t j, size_t k) const { return (i * size[1] + j) * size[2] + k; }
+};
+
 template <class T>
 struct Probe {
     uint32_t i, j, k;
@@ -125,6 +134,7 @@ struct Sim {
     bool cpml_on = false;
     CpmlFace<T> cf[6];
     std::vector<MurFace<T>> mur;
+    std::vector<UpmlRegion<T>> upml;
     std::vector<Ade<T>> ade;
     std::vector<Probe<T>> probes;
     std::vector<MatOp> ops;
@@ -697,19 +707,149 @@ double energy_of(const Sim<T>& s, const std::vector<T>* F) {
 }
 
 // Excitation table entry (ScheduledExcitation, src/fdtd/batch.rs:71-83)
+// ---------------------------------------------------------------- UPML
+// Upml::calculate_kappa (src/extensions/pml.rs:408-461): grading from the
+// outer boundary of the region's own direction; the current value is taken
+// half a cell inside (component == direction) or outside.
+inline void upml_kappa(size_t n_dir, int component, int direction, size_t pos, bool is_min, size_t layers,
+                       double delta, double width, double sigma_max, double m, double* kv, double* ki) {
+    double depth = 0.0;
+    if (is_min) {
+        if (pos <= layers) depth = (double)(layers - pos) * delta;
+    } else {
+        const size_t bs = n_dir - layers - 1;
+        if (pos >= bs) depth = (double)(pos - bs) * delta;
+    }
+    if (depth <= 0.0) { *kv = 0.0; *ki = 0.0; return; }
+    *kv = sigma_max * std::pow(depth / width, m);
+    const double di = depth + (component == direction ? -0.5 : 0.5) * delta;
+    double nd = di / width;
+    nd = nd < 0.0 ? 0.0 : (nd > 1.0 ? 1.0 : nd);
+    *ki = sigma_max * std::pow(nd, m);
+}
+
+// Upml::build_regions + init_region_coefficients (src/extensions/pml.rs:222-404)
+template <class T>
+void add_upml(Sim<T>& s, const size_t layers[6], double m, double R) {
+    const size_t n[3] = {s.nx, s.ny, s.nz};
+    const double del[3] = {s.dx, s.dy, s.dz};
+    auto region = [&](size_t st0, size_t st1, size_t st2, size_t sz0, size_t sz1, size_t sz2, int dir, bool is_min) {
+        UpmlRegion<T> r;
+        r.start[0] = st0; r.start[1] = st1; r.start[2] = st2;
+        r.size[0] = sz0; r.size[1] = sz1; r.size[2] = sz2;
+        const size_t cnt = sz0 * sz1 * sz2;
+        for (int c = 0; c < 3; ++c) {
+            for (auto* v : {&r.vv[c], &r.vvfo[c], &r.vvfn[c], &r.ii[c], &r.iifo[c], &r.iifn[c], &r.vflux[c], &r.cflux[c]})
+                v->assign(cnt, T(0));
+        }
+        const size_t L = is_min ? layers[2 * dir] : layers[2 * dir + 1];
+        const double delta = del[dir], width = (double)L * delta;
+        const double smax = -(m + 1.0) * std::log(R) / (2.0 * Z0 * width);
+        for (size_t li = 0; li < sz0; ++li)
+            for (size_t lj = 0; lj < sz1; ++lj)
+                for (size_t lk = 0; lk < sz2; ++lk) {
+                    const size_t pos[3] = {li + st0, lj + st1, lk + st2};
+                    double kv[3], ki[3];
+                    for (int c = 0; c < 3; ++c)
+                        upml_kappa(n[dir], c, dir, pos[dir], is_min, L, delta, width, smax, m, &kv[c], &ki[c]);
+                    const size_t q = r.at(li, lj, lk);
+                    for (int c = 0; c < 3; ++c) {
+                        const int npp = (c + 2) % 3;
+                        if (kv[0] + kv[1] + kv[2] > 0.0) {  // eq. 7.88
+                            const double den = 2.0 * EPS0 + kv[npp] * s.dt;
+                            r.vv[c][q] = (T)((2.0 * EPS0 - kv[npp] * s.dt) / den);
+                            r.vvfn[c][q] = (T)((2.0 * EPS0 + kv[c] * s.dt) / den);
+                            r.vvfo[c][q] = (T)((2.0 * EPS0 - kv[c] * s.dt) / den);
+                        } else {
+                            r.vv[c][q] = T(1); r.vvfn[c][q] = T(1); r.vvfo[c][q] = T(0);
+                        }
+                        if (ki[0] + ki[1] + ki[2] > 0.0) {  // eq. 7.90 (with eps0, as the reference)
+                            const double den = 2.0 * EPS0 + ki[npp] * s.dt;
+                            r.ii[c][q] = (T)((2.0 * EPS0 - ki[npp] * s.dt) / den);
+                            r.iifn[c][q] = (T)((2.0 * EPS0 + ki[c] * s.dt) / den);
+                            r.iifo[c][q] = (T)((2.0 * EPS0 - ki[c] * s.dt) / den);
+                        } else {
+                            r.ii[c][q] = T(1); r.iifn[c][q] = T(1); r.iifo[c][q] = T(0);
+                        }
+                    }
+                }
+        s.upml.push_back(std::move(r));
+    };
+    const size_t nx = n[0], ny = n[1], nz = n[2];
+    if (layers[0] > 0 && layers[0] < nx) region(0, 0, 0, layers[0] + 1, ny, nz, 0, true);
+    if (layers[1] > 0 && layers[1] < nx) region(nx - layers[1] - 1, 0, 0, layers[1] + 1, ny, nz, 0, false);
+    const size_t xs = layers[0] > 0 ? layers[0] + 1 : 0, xe = layers[1] > 0 ? nx - layers[1] - 1 : nx;
+    if (layers[2] > 0 && layers[2] < ny && xs < xe) region(xs, 0, 0, xe - xs, layers[2] + 1, nz, 1, true);
+    if (layers[3] > 0 && layers[3] < ny && xs < xe) region(xs, ny - layers[3] - 1, 0, xe - xs, layers[3] + 1, nz, 1, false);
+    const size_t ys = layers[2] > 0 ? layers[2] + 1 : 0, ye = layers[3] > 0 ? ny - layers[3] - 1 : ny;
+    if (layers[4] > 0 && layers[4] < nz && xs < xe && ys < ye)
+        region(xs, ys, 0, xe - xs, ye - ys, layers[4] + 1, 2, true);
+    if (layers[5] > 0 && layers[5] < nz && xs < xe && ys < ye)
+        region(xs, ys, nz - layers[5] - 1, xe - xs, ye - ys, layers[5] + 1, 2, false);
+}
+
+// pre_voltage_update / pre_current_update (src/extensions/pml.rs:500-537, 584-619):
+// f_help = c * F - cfo * flux; F <- flux; flux <- f_help
+template <class T>
+void upml_pre(Sim<T>& s, bool e) {
+    for (auto& r : s.upml)
+        for (size_t li = 0; li < r.size[0]; ++li)
+            for (size_t lj = 0; lj < r.size[1]; ++lj)
+                for (size_t lk = 0; lk < r.size[2]; ++lk) {
+                    const size_t q = r.at(li, lj, lk);
+                    const size_t p = s.at(li + r.start[0], lj + r.start[1], lk + r.start[2]);
+                    for (int c = 0; c < 3; ++c) {
+                        T* F = e ? s.E[c].data() : s.H[c].data();
+                        std::vector<T>& fl = e ? r.vflux[c] : r.cflux[c];
+                        const T cc = e ? r.vv[c][q] : r.ii[c][q];
+                        const T cfo = e ? r.vvfo[c][q] : r.iifo[c][q];
+                        const T old = F[p], fo = fl[q];
+                        const T fh = cc * old - cfo * fo;
+                        F[p] = fo;
+                        fl[q] = fh;
+                    }
+                }
+}
+
+// post_voltage_update / post_current_update (src/extensions/pml.rs:540-580, 622-657):
+// F_new = f_help + cfn * flux_new; flux <- flux_new (the engine's update of F)
+template <class T>
+void upml_post(Sim<T>& s, bool e) {
+    for (auto& r : s.upml)
+        for (size_t li = 0; li < r.size[0]; ++li)
+            for (size_t lj = 0; lj < r.size[1]; ++lj)
+                for (size_t lk = 0; lk < r.size[2]; ++lk) {
+                    const size_t q = r.at(li, lj, lk);
+                    const size_t p = s.at(li + r.start[0], lj + r.start[1], lk + r.start[2]);
+                    for (int c = 0; c < 3; ++c) {
+                        T* F = e ? s.E[c].data() : s.H[c].data();
+                        std::vector<T>& fl = e ? r.vflux[c] : r.cflux[c];
+                        const T cfn = e ? r.vvfn[c][q] : r.iifn[c][q];
+                        const T fh = fl[q], fn = F[p];
+                        fl[q] = fn;
+                        F[p] = fh + cfn * fn;
+                    }
+                }
+}
+
 struct Exc {
     uint32_t i, j, k, dir, soft;
 };
 
 // run_batch (src/fdtd/basic_engine.rs:56-163) with the hook order
-//   H, post_h(CPML) | pre_e(Mur, ADE) , E, post_e(CPML, ADE, Mur) | excitations
+//   pre_h(UPML), H, post_h(UPML, CPML) | pre_e(Mur, ADE, UPML), E,
+//   post_e(UPML, CPML, ADE, Mur) | excitations
 template <class T>
 void step(Sim<T>& s) {
+    if (!s.upml.empty()) upml_pre(s, false);
     update_h(s);
+    if (!s.upml.empty()) upml_post(s, false);
     if (s.cpml_on) cpml_post_h(s);
     mur_pre(s);
     ade_pre(s);
+    if (!s.upml.empty()) upml_pre(s, true);
     update_e(s);
+    if (!s.upml.empty()) upml_post(s, true);
     if (s.cpml_on) cpml_post_e(s);
     ade_post(s);
     mur_post_apply(s);
@@ -884,6 +1024,39 @@ size_t oro_cpml_psi(void* hv, int face, int field, int which, double* out) {
     return n;
 }
 
+void oro_add_upml(void* hv, const size_t layers[6], double m, double R) {
+    oro_handle* h = (oro_handle*)hv;
+    DISPATCH(h, add_upml(s, layers, m, R));
+}
+
+// UPML region q: which 0..5 = vv, vvfo, vvfn, ii, iifo, iifn; 6 = volt flux,
+// 7 = current flux; component c.  Returns the element count; start/size
+// receive the region box.
+size_t oro_upml_array(void* hv, size_t q, int which, int c, double* out, size_t* start, size_t* size) {
+    oro_handle* h = (oro_handle*)hv;
+    size_t n = 0;
+    DISPATCH(h, {
+        if (q < s.upml.size()) {
+            auto& r = s.upml[q];
+            const auto* arrs = which == 0 ? &r.vv[c] : which == 1 ? &r.vvfo[c] : which == 2 ? &r.vvfn[c]
+                             : which == 3 ? &r.ii[c] : which == 4 ? &r.iifo[c] : which == 5 ? &r.iifn[c]
+                             : which == 6 ? &r.vflux[c] : &r.cflux[c];
+            n = arrs->size();
+            if (out) for (size_t k = 0; k < n; ++k) out[k] = (double)(*arrs)[k];
+            if (start) for (int d = 0; d < 3; ++d) start[d] = r.start[d];
+            if (size) for (int d = 0; d < 3; ++d) size[d] = r.size[d];
+        }
+    });
+    return n;
+}
+
+size_t oro_upml_regions(void* hv) {
+    oro_handle* h = (oro_handle*)hv;
+    size_t n = 0;
+    DISPATCH(h, n = s.upml.size());
+    return n;
+}
+
 void oro_add_mur(void* hv, const int enabled[6], double phase_velocity) {
     oro_handle* h = (oro_handle*)hv;
     DISPATCH(h, add_mur(s, enabled, phase_velocity));
@@ -977,7 +1150,7 @@ double oro_energy(void* hv, int which) {
 void oro_yee_steps(void* hv, uint64_t n) {
     oro_handle* h = (oro_handle*)hv;
     DISPATCH(h, {
-        for (uint64_t t = 0; t < n; ++t) { update_h(s); if (s.cpml_on) cpml_post_h(s); update_e(s); if (s.cpml_on) cpml_post_e(s); }
+        for (uint64_t t = 0; t < n; ++t) step(s);
     });
 }
 

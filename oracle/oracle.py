@@ -93,6 +93,11 @@ def lib():
         L.oro_cpml_psi.restype = sz
         L.oro_cpml_psi.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
         L.oro_add_mur.argtypes = [vp, C.POINTER(C.c_int), d]
+        L.oro_add_upml.argtypes = [vp, C.POINTER(sz), d, d]
+        L.oro_upml_regions.restype = sz
+        L.oro_upml_regions.argtypes = [vp]
+        L.oro_upml_array.restype = sz
+        L.oro_upml_array.argtypes = [vp, sz, C.c_int, C.c_int, dp, C.POINTER(sz), C.POINTER(sz)]
         L.oro_add_ade.argtypes = [vp, C.c_int, dp, C.POINTER(u32), sz, C.c_int]
         L.oro_ade_state.restype = sz
         L.oro_ade_state.argtypes = [vp, sz, dp]
@@ -203,6 +208,31 @@ class OracleSim:
         out = np.empty(n, np.float64)
         lib().oro_cpml_psi(self._h, face, f, w, _dptr(out))
         return out
+
+    UPML_ARRAYS = ("vv", "vvfo", "vvfn", "ii", "iifo", "iifn", "volt_flux", "curr_flux")
+
+    def add_upml(self, layers: Sequence[int] = (8,) * 6, grading_order: float = 3.0, reflection_coeff: float = 1e-6):
+        arr = (C.c_size_t * 6)(*layers)
+        lib().oro_add_upml(self._h, arr, grading_order, reflection_coeff)
+
+    def upml_regions(self):
+        """[(start, size)] of the UPML regions in build order."""
+        out = []
+        for q in range(lib().oro_upml_regions(self._h)):
+            st = (C.c_size_t * 3)()
+            sz = (C.c_size_t * 3)()
+            lib().oro_upml_array(self._h, q, 0, 0, None, st, sz)
+            out.append((tuple(st), tuple(sz)))
+        return out
+
+    def upml_array(self, region: int, name: str, component: int) -> np.ndarray:
+        w = self.UPML_ARRAYS.index(name)
+        st = (C.c_size_t * 3)()
+        sz = (C.c_size_t * 3)()
+        n = lib().oro_upml_array(self._h, region, w, component, None, st, sz)
+        out = np.empty(n, np.float64)
+        lib().oro_upml_array(self._h, region, w, component, _dptr(out), st, sz)
+        return out.reshape(tuple(sz))
 
     def add_mur(self, enabled: Sequence[bool] = (True,) * 6, phase_velocity: float = 0.0):
         arr = (C.c_int * 6)(*[int(bool(x)) for x in enabled])

@@ -28,6 +28,7 @@ class Scenario:
     excitations: list = field(default_factory=list)  # (pos, dir, soft, fn(t))
     cpml: Optional[dict] = None  # kwargs of CpmlBoundaries
     mur: Optional[Sequence[bool]] = None
+    upml: Optional[dict] = None  # layers, grading_order, reflection_coeff
     ades: list = field(default_factory=list)  # (kind, params, positions, mode)
     probes: list = field(default_factory=list)
     tiles: Optional[dict] = None
@@ -69,6 +70,11 @@ def run_pair(sc: Scenario, prec: int = 32, batches: Optional[List[int]] = None):
     if sc.mur is not None:
         exts.append(eng.mur(sc.mur))
         sim.add_mur(sc.mur)
+    upml_ext = None
+    if sc.upml is not None:
+        upml_ext = eng.upml(**sc.upml)
+        exts.append(upml_ext)
+        sim.add_upml(**sc.upml)
     probe_ext = []
     for p in sc.probes:
         x = eng.field_probe(p)
@@ -95,6 +101,7 @@ def run_pair(sc: Scenario, prec: int = 32, batches: Optional[List[int]] = None):
     out["oracle_probes"] = [sim.probe_data(q) for q in range(len(sc.probes))]
     out["gpu_ade"] = [x.state() for x in ade_ext]
     out["oracle_ade"] = [sim.ade_state(q) for q in range(len(sc.ades))]
+    out["upml"] = upml_ext
     return out
 
 
