@@ -51,6 +51,19 @@ class Operator:
         check(lib().oe_operator_create(nx, ny, nz, d, C.byref(h)))
         return cls(h.value, (nx, ny, nz), tuple(map(float, delta)))
 
+    @classmethod
+    def from_lines(cls, x, y, z) -> "Operator":
+        """Grid::cartesian + Operator::new (src/geometry/mod.rs:69-71,
+        src/fdtd/operator.rs:59-77): mesh lines in metres, cells = lines - 1
+        per axis; dt and the coefficients use the smallest spacing per axis."""
+        ls = [np.ascontiguousarray(v, dtype=np.float64).reshape(-1) for v in (x, y, z)]
+        dp = C.POINTER(C.c_double)
+        h = C.c_void_p()
+        check(lib().oe_operator_create_lines(ls[0].ctypes.data_as(dp), ls[0].size, ls[1].ctypes.data_as(dp),
+                                             ls[1].size, ls[2].ctypes.data_as(dp), ls[2].size, C.byref(h)))
+        delta = tuple(float(np.abs(np.diff(l)).min()) if l.size > 1 else 0.0 for l in ls)
+        return cls(h.value, tuple(max(l.size - 1, 1) for l in ls), delta)
+
     def __del__(self):
         if getattr(self, "_h", None) and self._h.value:
             lib().oe_operator_destroy(self._h)
