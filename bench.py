@@ -48,28 +48,35 @@ def slab_cells(shape, layers):
     return tot
 
 
-def kernel_bytes(shape, layers):
-    """Algorithmic HBM bytes of one H (or E) half-step over the whole grid: 12
-    float reads + 3 writes per cell (own 3 components, the other field's 3
-    components, 6 coefficients), plus 2 psi reads + 2 psi writes per slab cell
+CELL_BYTES = {
+    # per cell and half-step: the own 3 components read and written, the other
+    # field's 3 components read, and the coefficients
+    "arrays": 60,   # 6 coefficient floats (the reference's 12-array layout)
+    "table": 38,    # one 2-byte material-table row index
+}
+
+
+def kernel_bytes(shape, layers, coef="arrays"):
+    """Algorithmic HBM bytes of one H (or E) half-step over the whole grid:
+    CELL_BYTES[coef] per cell plus 2 psi reads + 2 psi writes per slab cell
     per direction."""
     n = shape[0] * shape[1] * shape[2]
-    return 60 * n + 16 * slab_cells(shape, layers)
+    return CELL_BYTES[coef] * n + 16 * slab_cells(shape, layers)
 
 
-def launch_bytes(name, shape, layers):
+def launch_bytes(name, shape, layers, coef="arrays"):
     """Algorithmic bytes of one launch of the plan: 'grid' (no CPML), or
     'interior' (x/y slabs excluded; z-face psi at the row ends) and 'slabs'
     (the x/y face slabs with all faces) when a CPML is attached."""
     nx, ny, nz = shape
     part = name.split(":")[1]
     if part == "grid" or not layers:
-        return kernel_bytes(shape, 0)
+        return kernel_bytes(shape, 0, coef)
     ix, jy = nx - 2 * layers, ny - 2 * layers
-    interior = 60 * ix * jy * nz + 16 * ix * jy * 2 * layers
+    interior = CELL_BYTES[coef] * ix * jy * nz + 16 * ix * jy * 2 * layers
     if part == "interior":
         return interior
-    return kernel_bytes(shape, layers) - interior
+    return kernel_bytes(shape, layers, coef) - interior
 
 
 def summarize_rocprof(paths):
@@ -471,6 +478,8 @@ def main():
                     help="HIP events around every launch of every n-th timed step (1 = all)")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--tiles", type=str, default="", help="bx,by,ci,vec override")
+    ap.add_argument("--coef", choices=["auto", "table", "arrays"], default="auto",
+                    help="coefficient layout of the Yee updates (auto: the material table when the grid fits it)")
     ap.add_argument("--summarize-rocprof", nargs="+", metavar="DB",
                     help="print a markdown summary of rocprofv3 result databases and exit")
     ap.add_argument("--sweep", action="store_true", help="run the microstrip parameter sweep instead")
@@ -534,8 +543,12 @@ def main():
     if args.tiles:
         bx, by, ci, vec = (int(x) for x in args.tiles.split(","))
         eng.set_tiles(bx, by, ci, vec)
+    rows = eng.material_table(0 if args.coef == "arrays" else -1)
+    coef = "table" if rows[0] and rows[1] else "arrays"
+    if args.coef == "table" and coef != "table":
+        raise SystemExit(f"material table unavailable for this grid (rows {rows})")
     dt = op.dt
-    log(f"[bench] tiles {eng.tiles()}; warmup {args.warmup} steps")
+    log(f"[bench] tiles {eng.tiles()}; coefficients: {coef} (rows E/H {rows}); warmup {args.warmup} steps")
     if args.warmup:
         run_steps(eng, exts, wl, dt, 0, args.warmup)
 
@@ -558,11 +571,12 @@ def main():
     yee = [k for k in kern if k.startswith("update_")]
     dom = max(yee, key=lambda k: kern[k][0] * kern[k][1])
     avg_ms = kern[dom][1]
-    kb = launch_bytes(dom, wl.shape, wl.cpml_layers)
+    kb = launch_bytes(dom, wl.shape, wl.cpml_layers, coef)
+    kb_ref = launch_bytes(dom, wl.shape, wl.cpml_layers, "arrays")
     achieved = kb / (avg_ms * 1e-3) / 1e9
     for k, (n, ms) in kern.items():
         log(f"[bench] {k}: {n} launches, mean {ms * 1e3:.1f} us")
-    dom_key = f"{dom}@{wl.name}{wl.shape[0]}"
+    dom_key = f"{dom}@{wl.name}{wl.shape[0]}" + ("" if coef == "arrays" else "/table")
     traffic = measured_traffic(dom_key)
     E, H = eng.read_fields()
     import numpy as np
@@ -576,7 +590,7 @@ def main():
         log(f"[bench] CPU baseline {cpu['value']:.1f} MCells/s")
 
     mcells = wl.cells * args.steps / wall / 1e6
-    step_bytes = 2 * kernel_bytes(wl.shape, wl.cpml_layers)
+    step_bytes = 2 * kernel_bytes(wl.shape, wl.cpml_layers, coef)
     out = {
         "metric": METRIC,
         "value": round(mcells, 1),
@@ -596,6 +610,7 @@ def main():
                         f"boxes, soft Ez Gaussian fc=15 GHz, energy sampled every 100 steps",
             "grid": list(wl.shape),
             "tiles": eng.tiles(),
+            "coefficients": coef,
         },
         "hbm_frac_step": round(step_bytes * args.steps / wall / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {
@@ -609,6 +624,10 @@ def main():
             "traffic": traffic,
             "bytes_per_launch": kb,
             "mean_launch_us": round(avg_ms * 1e3, 2),
+            # the same launch priced at the reference's 12-array layout (60 B per
+            # cell): the bandwidth that layout would need for this launch time
+            "bytes_per_launch_array_layout": kb_ref,
+            "array_layout_equivalent_gbs": round(kb_ref / (avg_ms * 1e-3) / 1e9, 1),
         },
         "fields_finite": finite,
         "cpu_baseline": cpu,

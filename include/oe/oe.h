@@ -122,6 +122,21 @@ oe_status oe_engine_write_fields(oe_engine* eng, const float* const fields[6]);
  * oe_engine_create_from_arrays). */
 oe_status oe_engine_read_coefficients(oe_engine* eng, int which, float* out);
 
+/* Coefficient layout of the Yee updates; no counterpart in the reference,
+ * whose engines read the 12 arrays cloned from the Operator
+ * (src/fdtd/basic_engine.rs:42-54).  By default (unless the environment sets
+ * OE_COEF_TABLE=0) an engine derives from its coefficient arrays a material
+ * table: a 2-byte row index per cell and rows holding each distinct set of
+ * (a_x, b_x, a_y, b_y, a_z, b_z), separately for the H half-step (Da, Db)
+ * and the E half-step (Ca, Cb).  The rows are the arrays' own float values,
+ * so results are unchanged; a half-step then reads 2 bytes of coefficients
+ * per cell instead of 24.  A half whose grid holds more than 1024 distinct
+ * sets keeps reading the arrays.  enable: 1 builds the table if absent, 0
+ * drops it (every half reads the arrays), -1 only queries.  rows[0], rows[1]
+ * (may be NULL) receive the E and H row counts, 0 for a half that reads the
+ * arrays. */
+oe_status oe_engine_material_table(oe_engine* eng, int enable, int rows[2]);
+
 /* EngineImpl::reset (src/fdtd/engine_impl.rs:70): zero fields and the step
  * counter (extension state is reset with oe_extension_reset). */
 oe_status oe_engine_reset(oe_engine* eng);

@@ -97,6 +97,14 @@ YeeArgs launch_args(const oe_engine* e, const Tiles& t, const std::vector<Box>& 
     a.g = e->g;
     a.F = e->F;
     a.Cf = e->Cf;
+    if (e->mat) {
+        a.mat_e = e->mat;
+        a.mat_h = e->mat + e->g.N;
+        a.tab_e = e->mtab;
+        a.tab_h = e->mtab + (size_t)MAX_MAT_ROWS * 6;
+        a.n_mat_e = e->n_mat[0];
+        a.n_mat_h = e->n_mat[1];
+    }
     a.t = t;
     const int nzv = (e->g.nz + t.vec - 1) / t.vec;
     a.n_kb = (nzv + t.bx - 1) / t.bx;
@@ -283,6 +291,40 @@ oe_status engine_alloc(oe_engine* e, size_t nx, size_t ny, size_t nz, int device
     return OE_OK;
 }
 
+void drop_material_table(oe_engine* e) {
+    if (e->mat) (void)hipFree(e->mat);
+    if (e->mtab) (void)hipFree(e->mtab);
+    e->mat = nullptr;
+    e->mtab = nullptr;
+    e->n_mat[0] = e->n_mat[1] = 0;
+}
+
+// Material table of both halves from the coefficient block (oe_coef.hip).
+// A half with more than MAX_MAT_ROWS distinct coefficient sets keeps reading
+// the arrays; with neither half tabulated the table is dropped.
+oe_status build_material_tables(oe_engine* e) {
+    drop_material_table(e);
+    const size_t N = e->g.N;
+    HIP_OK(hipMalloc(&e->mat, 2 * N * sizeof(uint16_t)));
+    HIP_OK(hipMalloc(&e->mtab, 2 * (size_t)MAX_MAT_ROWS * 6 * sizeof(float)));
+    int n[2] = {0, 0};
+    for (int h = 0; h < 2; ++h)
+        HIP_OK(build_material_table(e->Cf, N, h == 1, MAX_MAT_ROWS, e->mat + (size_t)h * N,
+                                    e->mtab + (size_t)h * MAX_MAT_ROWS * 6, &n[h], e->stream));
+    if (n[0] == 0 && n[1] == 0) {
+        drop_material_table(e);
+        return OE_OK;
+    }
+    e->n_mat[0] = n[0];
+    e->n_mat[1] = n[1];
+    return OE_OK;
+}
+
+bool material_table_default() {
+    const char* v = getenv("OE_COEF_TABLE");
+    return !(v && atoi(v) == 0);
+}
+
 void engine_free(oe_engine* e) {
     if (!e) return;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
@@ -297,6 +339,8 @@ void engine_free(oe_engine* e) {
     for (auto ev : e->event_pool) (void)hipEventDestroy(ev);
     if (e->F) (void)hipFree(e->F);
     if (e->Cf) (void)hipFree(e->Cf);
+    if (e->mat) (void)hipFree(e->mat);
+    if (e->mtab) (void)hipFree(e->mtab);
     if (e->partials) (void)hipFree(e->partials);
     if (e->energy_dev) (void)hipFree(e->energy_dev);
     if (e->energy_host) (void)hipHostFree(e->energy_host);
@@ -397,6 +441,7 @@ oe_status oe_engine_create(const oe_operator* op, int device, oe_engine** out) {
         e->dt = op->dt;
         st = materialise(e, op);
     }
+    if (st == OE_OK && material_table_default()) st = build_material_tables(e);
     if (st == OE_OK && hipStreamSynchronize(e->stream) != hipSuccess) st = fail(OE_ERR_DEVICE, "engine setup failed");
     if (st != OE_OK) { engine_free(e); return st; }
     *out = e;
@@ -416,6 +461,7 @@ oe_status oe_engine_create_from_arrays(size_t nx, size_t ny, size_t nz, const do
             if (hipMemcpy(e->Cf + (size_t)w * e->g.N, coeff[w], e->g.N * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
                 st = fail(OE_ERR_DEVICE, "coefficient upload failed");
     }
+    if (st == OE_OK && material_table_default()) st = build_material_tables(e);
     if (st != OE_OK) { engine_free(e); return st; }
     *out = e;
     return OE_OK;
@@ -442,6 +488,24 @@ oe_status oe_engine_write_fields(oe_engine* e, const float* const fields[6]) {
         if (fields[d])
             HIP_OK(hipMemcpyAsync(e->F + (size_t)d * e->g.N, fields[d], e->g.N * sizeof(float), hipMemcpyHostToDevice, e->stream));
     HIP_OK(hipStreamSynchronize(e->stream));
+    return OE_OK;
+}
+
+oe_status oe_engine_material_table(oe_engine* e, int enable, int rows[2]) {
+    if (!e) return fail(OE_ERR_CONFIG, "null argument");
+    if (enable != -1 && enable != 0 && enable != 1) return fail(OE_ERR_CONFIG, "enable must be -1, 0 or 1");
+    HIP_OK(hipSetDevice(e->device));
+    if (enable == 0 && e->mat) {
+        HIP_OK(hipStreamSynchronize(e->stream));
+        drop_material_table(e);
+    } else if (enable == 1 && !e->mat) {
+        OE_TRY(build_material_tables(e));
+        HIP_OK(hipStreamSynchronize(e->stream));
+    }
+    if (rows) {
+        rows[0] = e->n_mat[0];
+        rows[1] = e->n_mat[1];
+    }
     return OE_OK;
 }
 

@@ -83,6 +83,14 @@ struct YeeArgs {
     Grid g;
     float* F;        // fields
     const float* Cf; // coefficients
+    // Material table (oe_coef.hip): per-cell row indices and rows of
+    // (a_x, b_x, a_y, b_y, a_z, b_z) for the H (Da, Db) and E (Ca, Cb)
+    // half-steps; a half whose row count is 0 reads the coefficient arrays.
+    const uint16_t* mat_h;
+    const uint16_t* mat_e;
+    const float* tab_h;
+    const float* tab_e;
+    int n_mat_h, n_mat_e;
     Tiles t;
     int n_kb;              // blocks along k (every box spans all k)
     int nbox;
@@ -93,6 +101,14 @@ struct YeeArgs {
 
 // CPML code compiled into a launch: bit 0 x faces, bit 1 y faces, bit 2 z faces.
 enum { AX_X = 1, AX_Y = 2, AX_Z = 4 };
+
+// Largest material table a half-step keeps in LDS (rows of 24 bytes).
+constexpr int MAX_MAT_ROWS = 1024;
+// Builds the material table of the H (h) or E half from the coefficient
+// block (oe_coef.hip); *n_rows = 0 when the grid holds more than cap
+// distinct coefficient sets.
+hipError_t build_material_table(const float* Cf, size_t N, bool h, int cap, uint16_t* mat, float* tab,
+                                int* n_rows, hipStream_t s);
 
 // Kernel launchers (oe_yee.hip).
 hipError_t launch_update_h(const YeeArgs& a, const CpmlArgs* c, int axes, hipStream_t s);

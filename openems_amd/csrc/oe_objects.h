@@ -113,6 +113,10 @@ struct oe_engine {
     double dt = 0;
     float* F = nullptr;   // 6N
     float* Cf = nullptr;  // 12N
+    // material table: row indices [E: N][H: N] and rows [E: cap x 6][H: cap x 6]
+    uint16_t* mat = nullptr;
+    float* mtab = nullptr;
+    int n_mat[2] = {0, 0};  // rows of the E and H tables; 0: that half reads Cf
     uint64_t timestep = 0;
     Tiles tiles{};
     int xcd_remap = 1;

@@ -115,6 +115,29 @@ __device__ __forceinline__ Vec<V> zerov() {
     return r;
 }
 
+// Material table (oe_coef.hip): V row indices of a lane (nontemporal, one
+// 8-byte load for V = 4) and the rows' (a, b) pairs of x, y, z from LDS.
+typedef unsigned int u2v __attribute__((ext_vector_type(2)));
+
+template <int V>
+__device__ __forceinline__ void coef_rows(const uint16_t* __restrict__ mat, const float2* tab, Vec<V>& a0,
+                                          Vec<V>& a1, Vec<V>& a2, Vec<V>& b0, Vec<V>& b1, Vec<V>& b2) {
+    unsigned m[V];
+    if constexpr (V == 4) {
+        const u2v t = __builtin_nontemporal_load(reinterpret_cast<const u2v*>(mat));
+        m[0] = t.x & 0xffffu; m[1] = t.x >> 16; m[2] = t.y & 0xffffu; m[3] = t.y >> 16;
+    } else {
+        m[0] = *mat;
+    }
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        const float2 x = tab[3 * m[e]], y = tab[3 * m[e] + 1], z = tab[3 * m[e] + 2];
+        a0.v[e] = x.x; b0.v[e] = x.y;
+        a1.v[e] = y.x; b1.v[e] = y.y;
+        a2.v[e] = z.x; b2.v[e] = z.y;
+    }
+}
+
 struct Tile {
     int k0, j, i0, i1, jend;
 };
@@ -229,9 +252,11 @@ __device__ __forceinline__ void psi_store(const Psi<V>& s) {
 }
 
 // AX: CPML axes compiled in (AX_X | AX_Y | AX_Z); PHYS: apply the field
-// correction (else psi only, as the reference).
-template <int V, int AX, bool PHYS>
+// correction (else psi only, as the reference); TAB: coefficients from the
+// material table (mat, tab in LDS) instead of the arrays in Cf.
+template <int V, int AX, bool PHYS, bool TAB>
 __device__ __forceinline__ void h_body(const Grid& g, float* __restrict__ F, const float* __restrict__ Cf,
+                                       const uint16_t* __restrict__ mat, const float2* tab,
                                        const CpmlArgs& c, const Tile& t) {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     if (t.k0 >= nz || t.j >= t.jend) return;
@@ -283,8 +308,13 @@ __device__ __forceinline__ void h_body(const Grid& g, float* __restrict__ F, con
             ex_kl = Ex[p + V];
         }
         Vec<V> hx = ldn<V, OE_NT_OWN>(Hx + p), hy = ldn<V, OE_NT_OWN>(Hy + p), hz = ldn<V, OE_NT_OWN>(Hz + p);
-        const Vec<V> a0 = ldn<V, OE_NT_COEF>(da0 + p), a1 = ldn<V, OE_NT_COEF>(da1 + p), a2 = ldn<V, OE_NT_COEF>(da2 + p);
-        const Vec<V> b0 = ldn<V, OE_NT_COEF>(db0 + p), b1 = ldn<V, OE_NT_COEF>(db1 + p), b2 = ldn<V, OE_NT_COEF>(db2 + p);
+        Vec<V> a0, a1, a2, b0, b1, b2;
+        if constexpr (TAB) {
+            coef_rows<V>(mat + p, tab, a0, a1, a2, b0, b1, b2);
+        } else {
+            a0 = ldn<V, OE_NT_COEF>(da0 + p); a1 = ldn<V, OE_NT_COEF>(da1 + p); a2 = ldn<V, OE_NT_COEF>(da2 + p);
+            b0 = ldn<V, OE_NT_COEF>(db0 + p); b1 = ldn<V, OE_NT_COEF>(db1 + p); b2 = ldn<V, OE_NT_COEF>(db2 + p);
+        }
 
         Psi<V> px, py, pz;
         if constexpr ((AX & AX_X) != 0) {
@@ -342,8 +372,9 @@ __device__ __forceinline__ void h_body(const Grid& g, float* __restrict__ F, con
     }
 }
 
-template <int V, int AX, bool PHYS>
+template <int V, int AX, bool PHYS, bool TAB>
 __device__ __forceinline__ void e_body(const Grid& g, float* __restrict__ F, const float* __restrict__ Cf,
+                                       const uint16_t* __restrict__ mat, const float2* tab,
                                        const CpmlArgs& c, const Tile& t) {
     const int ny = g.ny, nz = g.nz;
     if (t.k0 >= nz || t.j >= t.jend) return;
@@ -393,8 +424,12 @@ __device__ __forceinline__ void e_body(const Grid& g, float* __restrict__ F, con
         Vec<V> ex, ey, ez, c0, c1, c2, s0, s1, s2;
         if (row) {
             ex = ldn<V, OE_NT_OWN>(Ex + p); ey = ldn<V, OE_NT_OWN>(Ey + p); ez = ldn<V, OE_NT_OWN>(Ez + p);
-            s0 = ldn<V, OE_NT_COEF>(ca0 + p); s1 = ldn<V, OE_NT_COEF>(ca1 + p); s2 = ldn<V, OE_NT_COEF>(ca2 + p);
-            c0 = ldn<V, OE_NT_COEF>(cb0 + p); c1 = ldn<V, OE_NT_COEF>(cb1 + p); c2 = ldn<V, OE_NT_COEF>(cb2 + p);
+            if constexpr (TAB) {
+                coef_rows<V>(mat + p, tab, s0, s1, s2, c0, c1, c2);
+            } else {
+                s0 = ldn<V, OE_NT_COEF>(ca0 + p); s1 = ldn<V, OE_NT_COEF>(ca1 + p); s2 = ldn<V, OE_NT_COEF>(ca2 + p);
+                c0 = ldn<V, OE_NT_COEF>(cb0 + p); c1 = ldn<V, OE_NT_COEF>(cb1 + p); c2 = ldn<V, OE_NT_COEF>(cb2 + p);
+            }
         }
 
         Psi<V> px, py, pz;
@@ -461,15 +496,29 @@ __device__ __forceinline__ void e_body(const Grid& g, float* __restrict__ F, con
     }
 }
 
-// One grid: the tile comes from the launch's box list.
-template <int V, int AX, bool PHYS>
+// One grid: the tile comes from the launch's box list.  With TAB the
+// work-group first copies its half's material rows into LDS (dynamic LDS of
+// 24 bytes per row).
+template <int V, int AX, bool PHYS, bool TAB>
 __global__ __launch_bounds__(256) void k_update_h(YeeArgs a, CpmlArgs c) {
-    h_body<V, AX, PHYS>(a.g, a.F, a.Cf, c, decode<V>(a));
+    extern __shared__ float2 lds_rows[];
+    if constexpr (TAB) {
+        const float2* src = reinterpret_cast<const float2*>(a.tab_h);
+        for (int q = threadIdx.x; q < 3 * a.n_mat_h; q += blockDim.x) lds_rows[q] = src[q];
+        __syncthreads();
+    }
+    h_body<V, AX, PHYS, TAB>(a.g, a.F, a.Cf, a.mat_h, lds_rows, c, decode<V>(a));
 }
 
-template <int V, int AX, bool PHYS>
+template <int V, int AX, bool PHYS, bool TAB>
 __global__ __launch_bounds__(256) void k_update_e(YeeArgs a, CpmlArgs c) {
-    e_body<V, AX, PHYS>(a.g, a.F, a.Cf, c, decode<V>(a));
+    extern __shared__ float2 lds_rows[];
+    if constexpr (TAB) {
+        const float2* src = reinterpret_cast<const float2*>(a.tab_e);
+        for (int q = threadIdx.x; q < 3 * a.n_mat_e; q += blockDim.x) lds_rows[q] = src[q];
+        __syncthreads();
+    }
+    e_body<V, AX, PHYS, TAB>(a.g, a.F, a.Cf, a.mat_e, lds_rows, c, decode<V>(a));
 }
 
 // Many grids in one launch (a parameter sweep): work-group -> box of some
@@ -505,7 +554,7 @@ __global__ __launch_bounds__(256) void k_update_h_sweep(SweepArgs a) {
     int sid;
     decode_sweep<V>(a, t, sid);
     const SweepStruct& S = a.st[sid];
-    h_body<V, AX, PHYS>(S.g, S.F, S.Cf, S.c, t);
+    h_body<V, AX, PHYS, false>(S.g, S.F, S.Cf, nullptr, nullptr, S.c, t);
 }
 
 template <int V, int AX, bool PHYS>
@@ -514,24 +563,32 @@ __global__ __launch_bounds__(256) void k_update_e_sweep(SweepArgs a) {
     int sid;
     decode_sweep<V>(a, t, sid);
     const SweepStruct& S = a.st[sid];
-    e_body<V, AX, PHYS>(S.g, S.F, S.Cf, S.c, t);
+    e_body<V, AX, PHYS, false>(S.g, S.F, S.Cf, nullptr, nullptr, S.c, t);
 }
 
 template <int V, int AX, bool PHYS>
 hipError_t launch_h_t(const YeeArgs& a, const CpmlArgs& c, hipStream_t s) {
-    hipLaunchKernelGGL((k_update_h<V, AX, PHYS>), dim3(a.first[a.nbox]), dim3(256), 0, s, a, c);
+    if (a.n_mat_h > 0)
+        hipLaunchKernelGGL((k_update_h<V, AX, PHYS, true>), dim3(a.first[a.nbox]), dim3(256),
+                           3 * a.n_mat_h * sizeof(float2), s, a, c);
+    else
+        hipLaunchKernelGGL((k_update_h<V, AX, PHYS, false>), dim3(a.first[a.nbox]), dim3(256), 0, s, a, c);
     return hipGetLastError();
 }
 
 template <int V, int AX, bool PHYS>
 hipError_t launch_e_t(const YeeArgs& a, const CpmlArgs& c, hipStream_t s) {
-    hipLaunchKernelGGL((k_update_e<V, AX, PHYS>), dim3(a.first[a.nbox]), dim3(256), 0, s, a, c);
+    if (a.n_mat_e > 0)
+        hipLaunchKernelGGL((k_update_e<V, AX, PHYS, true>), dim3(a.first[a.nbox]), dim3(256),
+                           3 * a.n_mat_e * sizeof(float2), s, a, c);
+    else
+        hipLaunchKernelGGL((k_update_e<V, AX, PHYS, false>), dim3(a.first[a.nbox]), dim3(256), 0, s, a, c);
     return hipGetLastError();
 }
 
 // Instantiated variants: plain (no CPML), z faces only (interior of a CPML
-// grid), x and z faces (x slabs), y and z faces (y slabs) and all faces
-// (corner columns), each in compat and physical mode.
+// grid) and all faces (the x/y face slabs), each in compat and physical
+// mode, each reading the material table or the coefficient arrays.
 template <bool H, int V>
 hipError_t dispatch(const YeeArgs& a, const CpmlArgs* c, int axes, hipStream_t s) {
     const CpmlArgs none{};
@@ -540,15 +597,7 @@ hipError_t dispatch(const YeeArgs& a, const CpmlArgs* c, int axes, hipStream_t s
         if (c->physical) return H ? launch_h_t<V, AX_Z, true>(a, *c, s) : launch_e_t<V, AX_Z, true>(a, *c, s);
         return H ? launch_h_t<V, AX_Z, false>(a, *c, s) : launch_e_t<V, AX_Z, false>(a, *c, s);
     }
-    constexpr int XZ = AX_X | AX_Z, YZ = AX_Y | AX_Z, ALL = AX_X | AX_Y | AX_Z;
-    if (axes == XZ) {
-        if (c->physical) return H ? launch_h_t<V, XZ, true>(a, *c, s) : launch_e_t<V, XZ, true>(a, *c, s);
-        return H ? launch_h_t<V, XZ, false>(a, *c, s) : launch_e_t<V, XZ, false>(a, *c, s);
-    }
-    if (axes == YZ) {
-        if (c->physical) return H ? launch_h_t<V, YZ, true>(a, *c, s) : launch_e_t<V, YZ, true>(a, *c, s);
-        return H ? launch_h_t<V, YZ, false>(a, *c, s) : launch_e_t<V, YZ, false>(a, *c, s);
-    }
+    constexpr int ALL = AX_X | AX_Y | AX_Z;
     if (c->physical) return H ? launch_h_t<V, ALL, true>(a, *c, s) : launch_e_t<V, ALL, true>(a, *c, s);
     return H ? launch_h_t<V, ALL, false>(a, *c, s) : launch_e_t<V, ALL, false>(a, *c, s);
 }

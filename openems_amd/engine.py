@@ -446,6 +446,24 @@ class HipEngine:
         self.dt = op.dt
         self.delta = op.delta
 
+    @classmethod
+    def from_arrays(cls, shape, delta, dt: float, coeff: np.ndarray, device: int = 0) -> "HipEngine":
+        """EngineImpl::new on an Operator's cloned arrays (src/fdtd/basic_engine.rs:42-54):
+        coeff is [12, nx, ny, nz] float32 (Ca, Cb, Da, Db of x, y, z; Db negative)."""
+        nx, ny, nz = map(int, shape)
+        a = np.ascontiguousarray(coeff, dtype=np.float32).reshape(12, nx * ny * nz)
+        ptrs = (C.POINTER(C.c_float) * 12)(*[a[w].ctypes.data_as(C.POINTER(C.c_float)) for w in range(12)])
+        d = (C.c_double * 3)(*([float(delta)] * 3 if np.isscalar(delta) else map(float, delta)))
+        h = C.c_void_p()
+        check(lib().oe_engine_create_from_arrays(nx, ny, nz, d, float(dt), ptrs, device, C.byref(h)))
+        self = cls.__new__(cls)
+        self._h = h
+        self.shape = (nx, ny, nz)
+        self.N = nx * ny * nz
+        self.dt = float(dt)
+        self.delta = tuple(d)
+        return self
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             lib().oe_engine_destroy(self._h)
@@ -499,6 +517,13 @@ class HipEngine:
         n, ms = C.c_uint64(), C.c_double()
         check(lib().oe_engine_kernel_time(self._h, name.encode(), C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+    def material_table(self, enable: int = -1) -> Tuple[int, int]:
+        """Material-table rows of the (E, H) half-steps; 0 for a half that
+        reads the 12 coefficient arrays.  enable 1 builds, 0 drops, -1 queries."""
+        r = (C.c_int * 2)()
+        check(lib().oe_engine_material_table(self._h, int(enable), r))
+        return r[0], r[1]
 
     def set_tiles(self, bx=0, by=0, ci=0, vec=0):
         check(lib().oe_engine_set_tiles(self._h, bx, by, ci, vec))

@@ -967,6 +967,16 @@ void oro_get_coeff_f32(void* hv, int which, float* out) {
     std::memcpy(out, v.data(), s.N * sizeof(float));
 }
 
+// Coefficient arrays handed over whole, as EngineImpl::new clones them from
+// the Operator (src/fdtd/basic_engine.rs:42-54); FP32 oracle only.
+void oro_set_coeff_f32(void* hv, int which, const float* in) {
+    oro_handle* h = (oro_handle*)hv;
+    if (h->prec != 32) return;
+    auto& s = *h->f;
+    auto& v = which < 3 ? s.ca[which] : which < 6 ? s.cb[which - 3] : which < 9 ? s.da[which - 6] : s.db[which - 9];
+    std::memcpy(v.data(), in, s.N * sizeof(float));
+}
+
 // fields: [Ex,Ey,Ez,Hx,Hy,Hz] each N
 void oro_get_fields(void* hv, double* out) {
     oro_handle* h = (oro_handle*)hv;

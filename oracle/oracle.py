@@ -85,6 +85,7 @@ def lib():
         L.oro_set_pec.argtypes = [vp, sz, sz, sz, sz, sz, sz]
         L.oro_get_coeff.argtypes = [vp, C.c_int, dp]
         L.oro_get_coeff_f32.argtypes = [vp, C.c_int, C.POINTER(C.c_float)]
+        L.oro_set_coeff_f32.argtypes = [vp, C.c_int, C.POINTER(C.c_float)]
         L.oro_get_fields.argtypes = [vp, dp]
         L.oro_get_fields_f32.argtypes = [vp, C.POINTER(C.c_float)]
         L.oro_set_fields.argtypes = [vp, dp]
@@ -175,6 +176,13 @@ class OracleSim:
             for w in range(12):
                 lib().oro_get_coeff(self._h, w, _dptr(out[w]))
         return out.reshape((12,) + self.shape)
+
+    def set_coefficients(self, coeff: np.ndarray):
+        """Replace the 12 coefficient arrays ([12, nx, ny, nz] float32; FP32 oracle)."""
+        assert self.prec == 32
+        a = np.ascontiguousarray(coeff, dtype=np.float32).reshape(12, self.N)
+        for w in range(12):
+            lib().oro_set_coeff_f32(self._h, w, a[w].ctypes.data_as(C.POINTER(C.c_float)))
 
     def fields(self) -> np.ndarray:
         """[6, nx, ny, nz]: Ex, Ey, Ez, Hx, Hy, Hz (float32 or float64)."""
