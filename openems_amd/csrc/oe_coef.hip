@@ -11,13 +11,14 @@
 // the coefficient arrays themselves, so it holds exactly their float values
 // and the updates stay bit-identical:
 //
-//   claim   one pass over the cells: hash the cell's 6-float tuple (bit
-//           patterns) into a 4096-slot open-addressing table; the first cell
-//           with a new tuple claims a free slot (compare-and-swap of its cell
-//           index), every other cell compares its tuple exactly with the
-//           owner's.  More than `cap` distinct tuples, or a probe sequence
-//           longer than 64 slots, sets the overflow flag and the engine keeps
-//           the 12-array path.
+//   claim   one pass over the cells: a cell whose tuple differs from the
+//           previous cell's hashes its 6-float tuple (bit patterns) into a
+//           4096-slot open-addressing table; the first cell with a new tuple
+//           claims a free slot (compare-and-swap of its cell index), every
+//           other cell compares its tuple exactly with the owner's.  More
+//           than `cap` distinct tuples, or a probe sequence longer than 64
+//           slots, sets the overflow flag and the engine keeps the 12-array
+//           path.
 //   number  the host numbers the claimed slots; a gather builds the table
 //           rows from the owners' coefficients.
 //   assign  a second pass writes every cell's table row (uint16).
@@ -70,6 +71,10 @@ __global__ void k_tab_claim(Rows s, size_t N, uint32_t* __restrict__ slot, int* 
         if (__hip_atomic_load(overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
         uint32_t w[6];
         tuple_of(s, c, w);
+        // The first cell of every distinct tuple differs from its predecessor,
+        // so cells equal to the previous cell need not probe (runs along k
+        // are long; this keeps millions of lanes off one hot slot).
+        if (c > 0 && same(s, (uint32_t)(c - 1), w)) continue;
         uint32_t h = hash6(w) & (SLOTS - 1);
         bool done = false;
         for (int probe = 0; probe < MAX_PROBE && !done; ++probe) {

@@ -122,6 +122,13 @@ struct SweepStruct {
     float* F;
     const float* Cf;
     CpmlArgs c;
+    // the structure's material table (see YeeArgs); used by launches whose
+    // every structure has one
+    const uint16_t* mat_h;
+    const uint16_t* mat_e;
+    const float* tab_h;
+    const float* tab_e;
+    int n_mat_h, n_mat_e;
 };
 struct SweepBox {
     int sid;
@@ -136,8 +143,10 @@ struct SweepArgs {
     const int* wg_box;
     int xcd_remap;
 };
-// h: H (else E) half-step over n_wg work-groups; axes 0, AX_Z or all faces.
-hipError_t launch_sweep(bool h, const SweepArgs& a, int n_wg, int axes, int vec, hipStream_t s);
+// h: H (else E) half-step over n_wg work-groups; axes 0, AX_Z, AX_X|AX_Z,
+// AX_Y|AX_Z or all faces.  rows > 0: every structure reads its material
+// table, rows being the largest table of the launch's half (LDS size).
+hipError_t launch_sweep(bool h, const SweepArgs& a, int n_wg, int axes, int vec, int rows, hipStream_t s);
 
 // Auxiliary kernels (oe_aux.hip).
 hipError_t launch_fill(float* p, size_t n, float v, hipStream_t s);

@@ -53,6 +53,9 @@ struct oe_sweep {
     std::vector<Item> items;
     int ci = 8;            // planes per work-group along i
     int xcd_remap = 1;
+    // largest material table of the run per half (0: some structure reads
+    // its coefficient arrays, so every structure does)
+    int rows_h = 0, rows_e = 0;
     double tune_ms[4] = {0, 0, 0, 0};
     int tune_ci[4] = {2, 4, 8, 16};
     // device state of a run
@@ -176,7 +179,7 @@ hipError_t yee_half(oe_sweep* sw, bool h) {
         a.box = G.d_box;
         a.wg_box = G.d_wg;
         a.xcd_remap = sw->xcd_remap;
-        hipError_t e = launch_sweep(h, a, G.n_wg, G.axes, G.vec, sw->stream);
+        hipError_t e = launch_sweep(h, a, G.n_wg, G.axes, G.vec, h ? sw->rows_h : sw->rows_e, sw->stream);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -352,8 +355,26 @@ oe_status oe_sweep_run(oe_sweep* sw, const oe_sweep_config* cfg, oe_sweep_entry*
         st[s].F = it.eng->F;
         st[s].Cf = it.eng->Cf;
         if (it.cpml) st[s].c = it.cpml->cpml;
+        if (it.eng->mat) {
+            st[s].mat_e = it.eng->mat;
+            st[s].mat_h = it.eng->mat + it.eng->g.N;
+            st[s].tab_e = it.eng->mtab;
+            st[s].tab_h = it.eng->mtab + (size_t)MAX_MAT_ROWS * 6;
+            st[s].n_mat_e = it.eng->n_mat[0];
+            st[s].n_mat_h = it.eng->n_mat[1];
+        }
         for (const auto& v : it.samples) total += v.size();
     }
+    sw->rows_e = sw->rows_h = 0;
+    bool all_e = true, all_h = true;
+    for (int s = 0; s < n; ++s) {
+        all_e = all_e && st[s].n_mat_e > 0;
+        all_h = all_h && st[s].n_mat_h > 0;
+        sw->rows_e = std::max(sw->rows_e, st[s].n_mat_e);
+        sw->rows_h = std::max(sw->rows_h, st[s].n_mat_h);
+    }
+    if (!all_e) sw->rows_e = 0;
+    if (!all_h) sw->rows_h = 0;
     HIP_OK(sw->structs.alloc(n * sizeof(SweepStruct)));
     HIP_OK(hipMemcpyAsync(sw->structs.p, st.data(), n * sizeof(SweepStruct), hipMemcpyHostToDevice, sw->stream));
     HIP_OK(sw->exc_samples.alloc(std::max<size_t>(total, 1) * sizeof(float)));

@@ -548,22 +548,37 @@ __device__ __forceinline__ bool decode_sweep(const SweepArgs& a, Tile& r, int& s
     return true;
 }
 
-template <int V, int AX, bool PHYS>
+// A work-group belongs to one structure, so with TAB it copies that
+// structure's rows into LDS (every lane reaches the barrier: surplus lanes
+// only get an empty row range).
+template <int V, int AX, bool PHYS, bool TAB>
 __global__ __launch_bounds__(256) void k_update_h_sweep(SweepArgs a) {
+    extern __shared__ float2 lds_rows[];
     Tile t;
     int sid;
     decode_sweep<V>(a, t, sid);
     const SweepStruct& S = a.st[sid];
-    h_body<V, AX, PHYS, false>(S.g, S.F, S.Cf, nullptr, nullptr, S.c, t);
+    if constexpr (TAB) {
+        const float2* src = reinterpret_cast<const float2*>(S.tab_h);
+        for (int q = threadIdx.x; q < 3 * S.n_mat_h; q += blockDim.x) lds_rows[q] = src[q];
+        __syncthreads();
+    }
+    h_body<V, AX, PHYS, TAB>(S.g, S.F, S.Cf, S.mat_h, lds_rows, S.c, t);
 }
 
-template <int V, int AX, bool PHYS>
+template <int V, int AX, bool PHYS, bool TAB>
 __global__ __launch_bounds__(256) void k_update_e_sweep(SweepArgs a) {
+    extern __shared__ float2 lds_rows[];
     Tile t;
     int sid;
     decode_sweep<V>(a, t, sid);
     const SweepStruct& S = a.st[sid];
-    e_body<V, AX, PHYS, false>(S.g, S.F, S.Cf, nullptr, nullptr, S.c, t);
+    if constexpr (TAB) {
+        const float2* src = reinterpret_cast<const float2*>(S.tab_e);
+        for (int q = threadIdx.x; q < 3 * S.n_mat_e; q += blockDim.x) lds_rows[q] = src[q];
+        __syncthreads();
+    }
+    e_body<V, AX, PHYS, TAB>(S.g, S.F, S.Cf, S.mat_e, lds_rows, S.c, t);
 }
 
 template <int V, int AX, bool PHYS>
@@ -613,29 +628,36 @@ hipError_t launch_update_e(const YeeArgs& a, const CpmlArgs* c, int axes, hipStr
 }
 
 // Sweep launches: physical CPML only (every structure of a sweep carries the
-// Roden-Gedney correction); axes 0 (no CPML), AX_Z or all faces.
+// Roden-Gedney correction); axes 0 (no CPML), AX_Z, the x or y slab pair
+// with z, or all faces; rows > 0 selects the material-table variant.
 template <bool H, int V, int AX>
-static hipError_t sweep_t(const SweepArgs& a, int n_wg, hipStream_t s) {
-    if (H) hipLaunchKernelGGL((k_update_h_sweep<V, AX, AX != 0>), dim3(n_wg), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_update_e_sweep<V, AX, AX != 0>), dim3(n_wg), dim3(256), 0, s, a);
+static hipError_t sweep_t(const SweepArgs& a, int n_wg, int rows, hipStream_t s) {
+    const size_t lds = (size_t)rows * 3 * sizeof(float2);
+    if (rows > 0) {
+        if (H) hipLaunchKernelGGL((k_update_h_sweep<V, AX, AX != 0, true>), dim3(n_wg), dim3(256), lds, s, a);
+        else hipLaunchKernelGGL((k_update_e_sweep<V, AX, AX != 0, true>), dim3(n_wg), dim3(256), lds, s, a);
+    } else {
+        if (H) hipLaunchKernelGGL((k_update_h_sweep<V, AX, AX != 0, false>), dim3(n_wg), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_update_e_sweep<V, AX, AX != 0, false>), dim3(n_wg), dim3(256), 0, s, a);
+    }
     return hipGetLastError();
 }
 
 template <bool H, int V>
-static hipError_t sweep_dispatch(const SweepArgs& a, int n_wg, int axes, hipStream_t s) {
+static hipError_t sweep_dispatch(const SweepArgs& a, int n_wg, int axes, int rows, hipStream_t s) {
     switch (axes) {
-        case 0: return sweep_t<H, V, 0>(a, n_wg, s);
-        case AX_Z: return sweep_t<H, V, AX_Z>(a, n_wg, s);
-        case AX_X | AX_Z: return sweep_t<H, V, AX_X | AX_Z>(a, n_wg, s);
-        case AX_Y | AX_Z: return sweep_t<H, V, AX_Y | AX_Z>(a, n_wg, s);
-        default: return sweep_t<H, V, AX_X | AX_Y | AX_Z>(a, n_wg, s);
+        case 0: return sweep_t<H, V, 0>(a, n_wg, rows, s);
+        case AX_Z: return sweep_t<H, V, AX_Z>(a, n_wg, rows, s);
+        case AX_X | AX_Z: return sweep_t<H, V, AX_X | AX_Z>(a, n_wg, rows, s);
+        case AX_Y | AX_Z: return sweep_t<H, V, AX_Y | AX_Z>(a, n_wg, rows, s);
+        default: return sweep_t<H, V, AX_X | AX_Y | AX_Z>(a, n_wg, rows, s);
     }
 }
 
-hipError_t launch_sweep(bool h, const SweepArgs& a, int n_wg, int axes, int vec, hipStream_t s) {
+hipError_t launch_sweep(bool h, const SweepArgs& a, int n_wg, int axes, int vec, int rows, hipStream_t s) {
     if (n_wg <= 0) return hipSuccess;
-    if (h) return vec == 4 ? sweep_dispatch<true, 4>(a, n_wg, axes, s) : sweep_dispatch<true, 1>(a, n_wg, axes, s);
-    return vec == 4 ? sweep_dispatch<false, 4>(a, n_wg, axes, s) : sweep_dispatch<false, 1>(a, n_wg, axes, s);
+    if (h) return vec == 4 ? sweep_dispatch<true, 4>(a, n_wg, axes, rows, s) : sweep_dispatch<true, 1>(a, n_wg, axes, rows, s);
+    return vec == 4 ? sweep_dispatch<false, 4>(a, n_wg, axes, rows, s) : sweep_dispatch<false, 1>(a, n_wg, axes, rows, s);
 }
 
 }  // namespace oe
