@@ -64,12 +64,26 @@ def kernel_bytes(shape, layers, coef="arrays"):
     return CELL_BYTES[coef] * n + 16 * slab_cells(shape, layers)
 
 
+FUSED_CELL_BYTES = {
+    # per cell and fused step: E and H read and written, and the coefficients
+    # of both halves
+    "arrays": 96,   # 12 coefficient floats
+    "table": 52,    # two 2-byte row indices
+}
+
+
 def launch_bytes(name, shape, layers, coef="arrays"):
     """Algorithmic bytes of one launch of the plan: 'grid' (no CPML), or
     'interior' (x/y slabs excluded; z-face psi at the row ends) and 'slabs'
-    (the x/y face slabs with all faces) when a CPML is attached."""
+    (the x/y face slabs with all faces) when a CPML is attached.  A fused
+    launch ('fused:interior', 'fused:grid') covers both half-steps of its
+    box: FUSED_CELL_BYTES per cell, 32 B per z-slab cell (H and E psi)."""
     nx, ny, nz = shape
-    part = name.split(":")[1]
+    kind, part = name.split(":")
+    if kind == "fused":
+        ix, jy = (nx - 2 * layers, ny - 2 * layers) if part == "interior" else (nx, ny)
+        zl = 2 * layers if layers and 2 * layers < nz else 0
+        return FUSED_CELL_BYTES[coef] * ix * jy * nz + 32 * ix * jy * zl
     if part == "grid" or not layers:
         return kernel_bytes(shape, 0, coef)
     ix, jy = nx - 2 * layers, ny - 2 * layers
@@ -478,6 +492,8 @@ def main():
                     help="HIP events around every launch of every n-th timed step (1 = all)")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--tiles", type=str, default="", help="bx,by,ci,vec override")
+    ap.add_argument("--fused", action="store_true",
+                    help="one fused pass per timestep over the CPML interior (slower than two half-steps on MI355X)")
     ap.add_argument("--coef", choices=["auto", "table", "arrays"], default="auto",
                     help="coefficient layout of the Yee updates (auto: the material table when the grid fits it)")
     ap.add_argument("--summarize-rocprof", nargs="+", metavar="DB",
@@ -544,6 +560,7 @@ def main():
         bx, by, ci, vec = (int(x) for x in args.tiles.split(","))
         eng.set_tiles(bx, by, ci, vec)
     rows = eng.material_table(0 if args.coef == "arrays" else -1)
+    eng.fused_steps(1 if args.fused else 0)
     coef = "table" if rows[0] and rows[1] else "arrays"
     if args.coef == "table" and coef != "table":
         raise SystemExit(f"material table unavailable for this grid (rows {rows})")
@@ -564,15 +581,19 @@ def main():
 
     kern = {}
     for name in ("update_h:grid", "update_e:grid", "update_h:interior", "update_e:interior", "update_h:slabs",
-                 "update_e:slabs", "point_ops", "energy"):
+                 "update_e:slabs", "fused:interior", "fused:grid", "point_ops", "energy"):
         n, ms = eng.kernel_time(name)
         if n:
             kern[name] = (n, ms / n)
-    yee = [k for k in kern if k.startswith("update_")]
+    yee = [k for k in kern if k.startswith("update_") or k.startswith("fused:")]
     dom = max(yee, key=lambda k: kern[k][0] * kern[k][1])
     avg_ms = kern[dom][1]
     kb = launch_bytes(dom, wl.shape, wl.cpml_layers, coef)
-    kb_ref = launch_bytes(dom, wl.shape, wl.cpml_layers, "arrays")
+    # the reference's layout and scheme: 12 coefficient arrays, two in-place
+    # half-steps (a fused launch counts as both halves of its box)
+    kb_ref = launch_bytes(dom.replace("fused:", "update_h:"), wl.shape, wl.cpml_layers, "arrays")
+    if dom.startswith("fused:"):
+        kb_ref *= 2
     achieved = kb / (avg_ms * 1e-3) / 1e9
     for k, (n, ms) in kern.items():
         log(f"[bench] {k}: {n} launches, mean {ms * 1e3:.1f} us")
@@ -591,6 +612,11 @@ def main():
 
     mcells = wl.cells * args.steps / wall / 1e6
     step_bytes = 2 * kernel_bytes(wl.shape, wl.cpml_layers, coef)
+    if "fused:interior" in kern:
+        step_bytes = (launch_bytes("fused:interior", wl.shape, wl.cpml_layers, coef)
+                      + 2 * launch_bytes("update_h:slabs", wl.shape, wl.cpml_layers, coef))
+    elif "fused:grid" in kern:
+        step_bytes = launch_bytes("fused:grid", wl.shape, wl.cpml_layers, coef)
     out = {
         "metric": METRIC,
         "value": round(mcells, 1),
@@ -611,12 +637,14 @@ def main():
             "grid": list(wl.shape),
             "tiles": eng.tiles(),
             "coefficients": coef,
+            "step_scheme": "fused" if any(k.startswith("fused:") for k in kern) else "two half-steps",
         },
         "hbm_frac_step": round(step_bytes * args.steps / wall / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {
             "bound": "hbm",
             "kernel": {"update_h:interior": "k_update_h<4,AX_Z,phys> (interior box, z-face CPML)",
-                       "update_e:interior": "k_update_e<4,AX_Z,phys> (interior box, z-face CPML)"}.get(dom, dom),
+                       "update_e:interior": "k_update_e<4,AX_Z,phys> (interior box, z-face CPML)",
+                       "fused:interior": "k_fused<AX_Z,phys> (H and E of the interior box in one pass)"}.get(dom, dom),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -137,6 +137,23 @@ oe_status oe_engine_read_coefficients(oe_engine* eng, int which, float* out);
  * arrays. */
 oe_status oe_engine_material_table(oe_engine* eng, int enable, int rows[2]);
 
+/* Step scheme of oe_engine_run_batch; no counterpart in the reference, whose
+ * engines run the H and E half-steps as two sweeps
+ * (src/fdtd/basic_engine.rs:92-95).  Mode 0 (the default) runs the two
+ * half-steps in place.  With mode 1 a batch whose extensions add nothing
+ * between the half-steps (CPML, probes, DFT and ports only; no Mur, ADE or
+ * UPML) on a grid with nz a multiple of 4 between 128 and 256 runs one fused
+ * pass per timestep over the interior of the CPML x/y slabs: the pass reads
+ * the old fields from one buffer and writes the new ones to a second, so the
+ * engine holds two field blocks once a fused batch has run.  Every cell's
+ * arithmetic is unchanged, so results are identical.  Mode 2 fuses from
+ * nz = 4 on.  On MI355X the fused pass moves fewer bytes but runs slower
+ * than the two half-steps with CPML (its per-plane barrier and register
+ * count limit the loads in flight), hence the default.  The environment
+ * variable OE_FUSED sets the initial mode.  -1 only queries; *current (may
+ * be NULL) receives the mode in force. */
+oe_status oe_engine_fused_steps(oe_engine* eng, int mode, int* current);
+
 /* EngineImpl::reset (src/fdtd/engine_impl.rs:70): zero fields and the step
  * counter (extension state is reset with oe_extension_reset). */
 oe_status oe_engine_reset(oe_engine* eng);

@@ -95,7 +95,8 @@ Box make_box(const Tiles& t, int i0, int i1, int j0, int j1) {
 YeeArgs launch_args(const oe_engine* e, const Tiles& t, const std::vector<Box>& boxes) {
     YeeArgs a{};
     a.g = e->g;
-    a.F = e->F;
+    a.Fin = e->F;
+    a.Fout = e->F;
     a.Cf = e->Cf;
     if (e->mat) {
         a.mat_e = e->mat;
@@ -288,6 +289,7 @@ oe_status engine_alloc(oe_engine* e, size_t nx, size_t ny, size_t nz, int device
     HIP_OK(hipHostMalloc(&e->energy_host, 2 * e->energy_dev_cap * sizeof(double), hipHostMallocDefault));
     e->tiles = auto_tiles(e->g, 0);
     if (const char* v = getenv("OE_XCD_REMAP")) e->xcd_remap = atoi(v) != 0;
+    if (const char* v = getenv("OE_FUSED")) e->fused_mode = std::max(0, std::min(2, atoi(v)));
     return OE_OK;
 }
 
@@ -338,6 +340,7 @@ void engine_free(oe_engine* e) {
     }
     for (auto ev : e->event_pool) (void)hipEventDestroy(ev);
     if (e->F) (void)hipFree(e->F);
+    if (e->F2) (void)hipFree(e->F2);
     if (e->Cf) (void)hipFree(e->Cf);
     if (e->mat) (void)hipFree(e->mat);
     if (e->mtab) (void)hipFree(e->mtab);
@@ -488,6 +491,14 @@ oe_status oe_engine_write_fields(oe_engine* e, const float* const fields[6]) {
         if (fields[d])
             HIP_OK(hipMemcpyAsync(e->F + (size_t)d * e->g.N, fields[d], e->g.N * sizeof(float), hipMemcpyHostToDevice, e->stream));
     HIP_OK(hipStreamSynchronize(e->stream));
+    return OE_OK;
+}
+
+oe_status oe_engine_fused_steps(oe_engine* e, int mode, int* current) {
+    if (!e) return fail(OE_ERR_CONFIG, "null argument");
+    if (mode < -1 || mode > 2) return fail(OE_ERR_CONFIG, "mode must be -1, 0, 1 or 2");
+    if (mode >= 0) e->fused_mode = mode;
+    if (current) *current = e->fused_mode;
     return OE_OK;
 }
 
@@ -695,6 +706,12 @@ oe_status oe_ext_cpml_create(oe_engine* e, const oe_cpml_config* cfg, oe_extensi
         x->psi_count[f] = cnt;
         total += 4 * cnt;
     }
+    size_t zalt_off[2] = {0, 0};
+    for (int f = 4; f < 6; ++f) {  // second z-face H psi buffers (fused steps)
+        if (!(c.on_mask & (1 << f))) continue;
+        zalt_off[f - 4] = total;
+        total += 2 * x->psi_count[f];
+    }
     const size_t dummy_off = total;  // 64 zeroed floats read by cells outside every slab
     total += 64;
     HIP_OK(x->cpml_mem.alloc(total * sizeof(float)));
@@ -711,6 +728,12 @@ oe_status oe_ext_cpml_create(oe_engine* e, const oe_cpml_config* cfg, oe_extensi
         c.pe_neg[f] = base + cnt;
         c.ph_pos[f] = base + 2 * cnt;
         c.ph_neg[f] = base + 3 * cnt;
+        c.phw_pos[f] = c.ph_pos[f];
+        c.phw_neg[f] = c.ph_neg[f];
+        if (f >= 4) {
+            x->zalt_pos[f - 4] = x->cpml_mem.as<float>() + zalt_off[f - 4];
+            x->zalt_neg[f - 4] = x->cpml_mem.as<float>() + zalt_off[f - 4] + cnt;
+        }
     }
     HIP_OK(x->cpml_tab.alloc(tab.size() * sizeof(float)));
     HIP_OK(hipMemcpyAsync(x->cpml_tab.p, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, e->stream));
@@ -1460,6 +1483,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
 
     // Sort extensions by hook.
     const CpmlArgs* cp = nullptr;
+    oe_extension* cpx = nullptr;
     oe_extension* upml = nullptr;
     std::vector<oe_extension*> murs, ades, probes, dfts;
     for (size_t q = 0; q < b->n_extensions; ++q) {
@@ -1471,6 +1495,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             case EXT_CPML:
                 if (cp) return fail(OE_ERR_CONFIG, "more than one CPML extension in a batch");
                 cp = &x->cpml;
+                cpx = x;
                 break;
             case EXT_UPML:
                 if (upml) return fail(OE_ERR_CONFIG, "more than one UPML extension in a batch");
@@ -1533,6 +1558,43 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
     Tiles tl = e->tiles;
     if (cp && !cp->zvec && tl.vec == 4) tl = auto_tiles(e->g, 1);  // z slabs not 4-aligned: scalar lanes
     const std::vector<Launch> plan = plan_launches(e, tl, cp);
+    // Fused steps (one pass per step over the plan's interior box) when no
+    // extension hooks in between the half-steps and the grid suits the
+    // fused kernel; the x/y CPML slabs keep their two launches around it.
+    const bool fused = e->fused_mode && !upml && murs.empty() && ades.empty() && tl.vec == 4 &&
+                       fused_supported(e->g) && (e->fused_mode == 2 || e->g.nz >= 128) && (!cp || cp->zvec);
+    FusedArgs fa{};
+    std::vector<Launch> slabs;
+    if (fused) {
+        if (!e->F2) HIP_OK(hipMalloc(&e->F2, 6 * e->g.N * sizeof(float)));
+        const Launch* inner = nullptr;
+        for (const Launch& L : plan) {
+            if (L.axes & (AX_X | AX_Y)) slabs.push_back(L);
+            else inner = &L;
+        }
+        fa.g = e->g;
+        fa.Cf = e->Cf;
+        if (e->mat && e->n_mat[0] > 0 && e->n_mat[1] > 0) {
+            fa.mat_e = e->mat;
+            fa.mat_h = e->mat + e->g.N;
+            fa.tab_e = e->mtab;
+            fa.tab_h = e->mtab + (size_t)MAX_MAT_ROWS * 6;
+            fa.n_mat_e = e->n_mat[0];
+            fa.n_mat_h = e->n_mat[1];
+        }
+        if (inner) {
+            const Box& B = inner->a.box[0];
+            fa.i0 = B.i0; fa.i1 = B.i1; fa.j0 = B.j0; fa.j1 = B.j1;
+        }
+        fa.n_jb = inner ? (fa.j1 - fa.j0 + FUSED_ROWS - 1) / FUSED_ROWS : 0;
+        // about 2048 work-groups, 4 to 32 planes each
+        const int ni = fa.i1 - fa.i0;
+        int ci = fa.n_jb ? (int)(((long)ni * fa.n_jb + 2047) / 2048) : 1;
+        if (const char* v = getenv("OE_FUSED_CI")) ci = atoi(v);
+        fa.ci = std::max(1, std::min(std::max(ci, 4), 32));
+        fa.n_ib = inner ? (ni + fa.ci - 1) / fa.ci : 0;
+        fa.xcd_remap = e->xcd_remap;
+    }
     double peak = b->persistent_peak ? e->peak_persistent : 0.0;
     std::vector<uint64_t> sample_steps;  // timesteps whose energy is pending in energy_dev
     size_t n_samples_total = 0;
@@ -1566,14 +1628,59 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
     for (uint64_t st = 0; st < num_steps; ++st) {
         const uint64_t cur = e->timestep;
         e->prof_step = e->profiling > 0 && (st % (uint64_t)e->profiling) == 0;
+        if (fused) {
+            // slab H (old -> new buffer), the fused interior, slab E; then the
+            // new buffer becomes the current one
+            CpmlArgs cf{};
+            if (cp) {
+                cf = *cp;
+                for (int f = 4; f < 6; ++f)
+                    if (cf.on_mask & (1 << f)) {
+                        cf.phw_pos[f] = cpx->zalt_pos[f - 4];
+                        cf.phw_neg[f] = cpx->zalt_neg[f - 4];
+                    }
+            }
+            float* Fo = e->F;
+            float* Fn = e->F2;
+            for (const Launch& L : slabs) {
+                YeeArgs a = L.a;
+                a.Fin = Fo;
+                a.Fout = Fn;
+                Timed tm(e, "update_h:slabs");
+                HIP_OK(launch_update_h(a, &cf, L.axes, e->stream));
+            }
+            {
+                fa.Fo = Fo;
+                fa.Fn = Fn;
+                Timed tm(e, cp ? "fused:interior" : "fused:grid");
+                HIP_OK(launch_fused(fa, cp ? &cf : nullptr, e->stream));
+            }
+            for (const Launch& L : slabs) {
+                YeeArgs a = L.a;
+                a.Fin = Fo;
+                a.Fout = Fn;
+                Timed tm(e, "update_e:slabs");
+                HIP_OK(launch_update_e(a, &cf, L.axes, e->stream));
+            }
+            std::swap(e->F, e->F2);
+            if (cpx)
+                for (int f = 4; f < 6; ++f)
+                    if (cpx->cpml.on_mask & (1 << f)) {
+                        std::swap(cpx->cpml.ph_pos[f], cpx->zalt_pos[f - 4]);
+                        std::swap(cpx->cpml.ph_neg[f], cpx->zalt_neg[f - 4]);
+                        cpx->cpml.phw_pos[f] = cpx->cpml.ph_pos[f];
+                        cpx->cpml.phw_neg[f] = cpx->cpml.ph_neg[f];
+                    }
+        }
         if (upml) {
             Timed tm(e, "upml");
             HIP_OK(launch_upml_pre(e->F, e->g, upml->upml, false, e->stream));
         }
-        for (const Launch& L : plan) {
-            Timed tm(e, (L.axes & (AX_X | AX_Y)) ? "update_h:slabs" : (cp ? "update_h:interior" : "update_h:grid"));
-            HIP_OK(launch_update_h(L.a, cp, L.axes, e->stream));
-        }
+        if (!fused)
+            for (const Launch& L : plan) {
+                Timed tm(e, (L.axes & (AX_X | AX_Y)) ? "update_h:slabs" : (cp ? "update_h:interior" : "update_h:grid"));
+                HIP_OK(launch_update_h(L.a, cp, L.axes, e->stream));
+            }
         if (upml) {
             Timed tm(e, "upml");
             HIP_OK(launch_upml_post(e->F, e->g, upml->upml, false, e->stream));
@@ -1591,10 +1698,11 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             Timed tm(e, "upml");
             HIP_OK(launch_upml_pre(e->F, e->g, upml->upml, true, e->stream));
         }
-        for (const Launch& L : plan) {
-            Timed tm(e, (L.axes & (AX_X | AX_Y)) ? "update_e:slabs" : (cp ? "update_e:interior" : "update_e:grid"));
-            HIP_OK(launch_update_e(L.a, cp, L.axes, e->stream));
-        }
+        if (!fused)
+            for (const Launch& L : plan) {
+                Timed tm(e, (L.axes & (AX_X | AX_Y)) ? "update_e:slabs" : (cp ? "update_e:interior" : "update_e:grid"));
+                HIP_OK(launch_update_e(L.a, cp, L.axes, e->stream));
+            }
         if (upml) {
             Timed tm(e, "upml");
             HIP_OK(launch_upml_post(e->F, e->g, upml->upml, true, e->stream));

@@ -67,6 +67,11 @@ struct CpmlArgs {
     float* pe_neg[6];
     float* ph_pos[6];
     float* ph_neg[6];
+    // where the H updates store the new H psi: the read arrays (in place)
+    // except in a ping-pong step, whose z faces read the old and write the
+    // new buffer
+    float* phw_pos[6];
+    float* phw_neg[6];
     const float* tab;  // [6][CPML_TABLES][Lmax]
     const float* dummy;  // 16 zeroed floats: load target for cells outside every slab
 };
@@ -81,7 +86,12 @@ struct Box {
 
 struct YeeArgs {
     Grid g;
-    float* F;        // fields
+    // Fields read (Fin) and written (Fout); equal for an in-place update.
+    // The H half reads E and the old H from Fin and writes H to Fout; the E
+    // half reads the new H from Fout and the old E from Fin and, when the
+    // two differ, writes every E cell (unchanged ones copied) to Fout.
+    const float* Fin;
+    float* Fout;
     const float* Cf; // coefficients
     // Material table (oe_coef.hip): per-cell row indices and rows of
     // (a_x, b_x, a_y, b_y, a_z, b_z) for the H (Da, Db) and E (Ca, Cb)
@@ -109,6 +119,35 @@ constexpr int MAX_MAT_ROWS = 1024;
 // distinct coefficient sets.
 hipError_t build_material_table(const float* Cf, size_t N, bool h, int cap, uint16_t* mat, float* tab,
                                 int* n_rows, hipStream_t s);
+
+// Fused step (oe_yee.hip): one pass over a box computes H^{n+1/2} and
+// E^{n+1} of every cell, reading the old fields from Fo and writing the new
+// ones to Fn.  A work-group holds FUSED_ROWS rows of full k-rows (one wave
+// per row, nz <= 256) plus one wave that recomputes the H of the row below
+// the block; the H of the plane before its i-chunk is recomputed too.  Rows
+// and planes outside the box (CPML x/y slabs) must already hold their new H
+// in Fn.  The z-face H psi is read from c.ph_* and written to c.phw_*.
+constexpr int FUSED_ROWS = 4;
+struct FusedArgs {
+    Grid g;
+    const float* Fo;
+    float* Fn;
+    const float* Cf;
+    const uint16_t* mat_h;
+    const uint16_t* mat_e;
+    const float* tab_h;
+    const float* tab_e;
+    int n_mat_h, n_mat_e;  // both > 0, or both 0 (coefficient arrays)
+    int i0, i1, j0, j1;    // box
+    int ci;                // planes per work-group
+    int n_jb, n_ib;
+    int xcd_remap;
+};
+// Whether the fused step handles this grid (V = 4 lanes, every k-row inside
+// one wave).  Below nz = 128 most lanes of a row idle, so by default the
+// engine fuses only from there on.
+bool fused_supported(const Grid& g);
+hipError_t launch_fused(const FusedArgs& a, const CpmlArgs* c, hipStream_t s);
 
 // Kernel launchers (oe_yee.hip).
 hipError_t launch_update_h(const YeeArgs& a, const CpmlArgs* c, int axes, hipStream_t s);

@@ -69,6 +69,10 @@ struct oe_extension {
     DevBuf cpml_mem;    // psi arrays
     DevBuf cpml_tab;
     size_t psi_count[6] = {0, 0, 0, 0, 0, 0};
+    // second buffers of the z-face H psi (faces 4, 5) for fused steps, which
+    // read the old psi and write the new one; swapped with cpml.ph_* per step
+    float* zalt_pos[2] = {nullptr, nullptr};
+    float* zalt_neg[2] = {nullptr, nullptr};
     // Mur
     std::vector<MurDev> mur_faces;
     DevBuf mur_mem;
@@ -111,7 +115,9 @@ struct oe_engine {
     Grid g{};
     double d[3] = {0, 0, 0};
     double dt = 0;
-    float* F = nullptr;   // 6N
+    float* F = nullptr;   // 6N, the current fields
+    float* F2 = nullptr;  // 6N, the other buffer of fused steps (allocated on first use)
+    int fused_mode = 0;   // 0: two half-steps; 1, 2: fused steps where the batch allows them
     float* Cf = nullptr;  // 12N
     // material table: row indices [E: N][H: N] and rows [E: cap x 6][H: cap x 6]
     uint16_t* mat = nullptr;

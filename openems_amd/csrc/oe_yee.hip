@@ -190,6 +190,8 @@ struct Psi {
     int f;
     const float* p;  // psi_pos address (or dummy)
     const float* n;  // psi_neg address (or dummy)
+    float* wp;       // where the new psi_pos / psi_neg go (same offsets)
+    float* wn;
     Vec<V> pp, pn;   // loaded values
     Vec<V> b, a, km; // per-cell table values
 };
@@ -200,15 +202,20 @@ struct Psi {
 // the argument arrays, which would spill them to scratch).
 template <int V, bool ZAXIS>
 __device__ __forceinline__ void psi_load(Psi<V>& s, const CpmlArgs& c, int d, int f, int l, size_t q,
-                                         float* const* pos, float* const* neg, int tb, int ta, int tk) {
+                                         float* const* pos, float* const* neg, float* const* wpos,
+                                         float* const* wneg, int tb, int ta, int tk) {
     s.f = f;
     const bool on = f >= 0;
     const bool lo = f == 2 * d;
     const int fs = on ? f : 2 * d;
     float* P = lo ? pos[2 * d] : pos[2 * d + 1];
     float* Q = lo ? neg[2 * d] : neg[2 * d + 1];
+    float* WP = lo ? wpos[2 * d] : wpos[2 * d + 1];
+    float* WQ = lo ? wneg[2 * d] : wneg[2 * d + 1];
     s.p = on ? P + q : c.dummy;
     s.n = on ? Q + q : c.dummy;
+    s.wp = on ? WP + q : nullptr;
+    s.wn = on ? WQ + q : nullptr;
     s.pp = ldv<V>(s.p);
     s.pn = ldv<V>(s.n);
     const float* T = c.tab + (size_t)fs * CPML_TABLES * c.Lmax;
@@ -246,8 +253,8 @@ __device__ __forceinline__ void psi_apply(Psi<V>& s, const float* dpos, const fl
 template <int V>
 __device__ __forceinline__ void psi_store(const Psi<V>& s) {
     if (s.f >= 0) {
-        stv<V>(const_cast<float*>(s.p), s.pp);
-        stv<V>(const_cast<float*>(s.n), s.pn);
+        stv<V>(s.wp, s.pp);
+        stv<V>(s.wn, s.pn);
     }
 }
 
@@ -255,19 +262,24 @@ __device__ __forceinline__ void psi_store(const Psi<V>& s) {
 // correction (else psi only, as the reference); TAB: coefficients from the
 // material table (mat, tab in LDS) instead of the arrays in Cf.
 template <int V, int AX, bool PHYS, bool TAB>
-__device__ __forceinline__ void h_body(const Grid& g, float* __restrict__ F, const float* __restrict__ Cf,
+__device__ __forceinline__ void h_body(const Grid& g, const float* Fin, float* Fout, const float* __restrict__ Cf,
                                        const uint16_t* __restrict__ mat, const float2* tab,
                                        const CpmlArgs& c, const Tile& t) {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     if (t.k0 >= nz || t.j >= t.jend) return;
     const size_t N = g.N;
     const size_t plane = (size_t)ny * nz;
-    const float* __restrict__ Ex = F;
-    const float* __restrict__ Ey = F + N;
-    const float* __restrict__ Ez = F + 2 * N;
-    float* __restrict__ Hx = F + 3 * N;
-    float* __restrict__ Hy = F + 4 * N;
-    float* __restrict__ Hz = F + 5 * N;
+    const float* __restrict__ Ex = Fin;
+    const float* __restrict__ Ey = Fin + N;
+    const float* __restrict__ Ez = Fin + 2 * N;
+    // the old H is read from Fin and the new H stored to Fout (the same
+    // arrays for an in-place update: no __restrict__ across the pair)
+    const float* Hx = Fin + 3 * N;
+    const float* Hy = Fin + 4 * N;
+    const float* Hz = Fin + 5 * N;
+    float* Hxw = Fout + 3 * N;
+    float* Hyw = Fout + 4 * N;
+    float* Hzw = Fout + 5 * N;
     const float* __restrict__ da0 = Cf + 6 * N;
     const float* __restrict__ da1 = Cf + 7 * N;
     const float* __restrict__ da2 = Cf + 8 * N;
@@ -320,15 +332,15 @@ __device__ __forceinline__ void h_body(const Grid& g, float* __restrict__ F, con
         if constexpr ((AX & AX_X) != 0) {
             int lx;
             const int fx = face_of(c, 0, i, lx);
-            psi_load<V, false>(px, c, 0, fx, lx, ((size_t)lx * ny + j) * nz + k0, c.ph_pos, c.ph_neg, TAB_BH, TAB_AH, TAB_KH);
+            psi_load<V, false>(px, c, 0, fx, lx, ((size_t)lx * ny + j) * nz + k0, c.ph_pos, c.ph_neg, c.phw_pos, c.phw_neg, TAB_BH, TAB_AH, TAB_KH);
         }
         if constexpr ((AX & AX_Y) != 0) {
             const int Ly = fy == 2 ? c.L[2] : (fy == 3 ? c.L[3] : 0);
-            psi_load<V, false>(py, c, 1, fy, ly, ((size_t)i * Ly + ly) * nz + k0, c.ph_pos, c.ph_neg, TAB_BH, TAB_AH, TAB_KH);
+            psi_load<V, false>(py, c, 1, fy, ly, ((size_t)i * Ly + ly) * nz + k0, c.ph_pos, c.ph_neg, c.phw_pos, c.phw_neg, TAB_BH, TAB_AH, TAB_KH);
         }
         if constexpr ((AX & AX_Z) != 0) {
             const int Lz = fz == 4 ? c.L[4] : (fz == 5 ? c.L[5] : 0);
-            psi_load<V, true>(pz, c, 2, fz, lz, ((size_t)i * ny + j) * Lz + lz, c.ph_pos, c.ph_neg, TAB_BH, TAB_AH, TAB_KH);
+            psi_load<V, true>(pz, c, 2, fz, lz, ((size_t)i * ny + j) * Lz + lz, c.ph_pos, c.ph_neg, c.phw_pos, c.phw_neg, TAB_BH, TAB_AH, TAB_KH);
         }
 
         float dEz_x[V], dEy_x[V], dEx_y[V], dEz_y[V], dEy_z[V], dEx_z[V];
@@ -364,28 +376,33 @@ __device__ __forceinline__ void h_body(const Grid& g, float* __restrict__ F, con
         if constexpr ((AX & AX_Y) != 0) psi_store<V>(py);
         if constexpr ((AX & AX_Z) != 0) psi_store<V>(pz);
 
-        stn<V, OE_NT_STORE>(Hx + p, hx);
-        stn<V, OE_NT_STORE>(Hy + p, hy);
-        stn<V, OE_NT_STORE>(Hz + p, hz);
+        stn<V, OE_NT_STORE>(Hxw + p, hx);
+        stn<V, OE_NT_STORE>(Hyw + p, hy);
+        stn<V, OE_NT_STORE>(Hzw + p, hz);
         ey_c = ey_n;
         ez_c = ez_n;
     }
 }
 
 template <int V, int AX, bool PHYS, bool TAB>
-__device__ __forceinline__ void e_body(const Grid& g, float* __restrict__ F, const float* __restrict__ Cf,
+__device__ __forceinline__ void e_body(const Grid& g, const float* Fin, float* Fout, const float* __restrict__ Cf,
                                        const uint16_t* __restrict__ mat, const float2* tab,
                                        const CpmlArgs& c, const Tile& t) {
     const int ny = g.ny, nz = g.nz;
     if (t.k0 >= nz || t.j >= t.jend) return;
     const size_t N = g.N;
     const size_t plane = (size_t)ny * nz;
-    float* __restrict__ Ex = F;
-    float* __restrict__ Ey = F + N;
-    float* __restrict__ Ez = F + 2 * N;
-    const float* __restrict__ Hx = F + 3 * N;
-    const float* __restrict__ Hy = F + 4 * N;
-    const float* __restrict__ Hz = F + 5 * N;
+    // old E from Fin, new E to Fout; the new H (already in Fout) is read
+    const bool copy_all = Fin != Fout;
+    const float* Ex = Fin;
+    const float* Ey = Fin + N;
+    const float* Ez = Fin + 2 * N;
+    float* Exw = Fout;
+    float* Eyw = Fout + N;
+    float* Ezw = Fout + 2 * N;
+    const float* __restrict__ Hx = Fout + 3 * N;
+    const float* __restrict__ Hy = Fout + 4 * N;
+    const float* __restrict__ Hz = Fout + 5 * N;
     const float* __restrict__ ca0 = Cf;
     const float* __restrict__ ca1 = Cf + N;
     const float* __restrict__ ca2 = Cf + 2 * N;
@@ -436,15 +453,15 @@ __device__ __forceinline__ void e_body(const Grid& g, float* __restrict__ F, con
         if constexpr ((AX & AX_X) != 0) {
             int lx;
             const int fx = face_of(c, 0, i, lx);
-            psi_load<V, false>(px, c, 0, fx, lx, ((size_t)lx * ny + j) * nz + k0, c.pe_pos, c.pe_neg, TAB_BE, TAB_AE, TAB_KE);
+            psi_load<V, false>(px, c, 0, fx, lx, ((size_t)lx * ny + j) * nz + k0, c.pe_pos, c.pe_neg, c.pe_pos, c.pe_neg, TAB_BE, TAB_AE, TAB_KE);
         }
         if constexpr ((AX & AX_Y) != 0) {
             const int Ly = fy == 2 ? c.L[2] : (fy == 3 ? c.L[3] : 0);
-            psi_load<V, false>(py, c, 1, fy, ly, ((size_t)i * Ly + ly) * nz + k0, c.pe_pos, c.pe_neg, TAB_BE, TAB_AE, TAB_KE);
+            psi_load<V, false>(py, c, 1, fy, ly, ((size_t)i * Ly + ly) * nz + k0, c.pe_pos, c.pe_neg, c.pe_pos, c.pe_neg, TAB_BE, TAB_AE, TAB_KE);
         }
         if constexpr ((AX & AX_Z) != 0) {
             const int Lz = fz == 4 ? c.L[4] : (fz == 5 ? c.L[5] : 0);
-            psi_load<V, true>(pz, c, 2, fz, lz, ((size_t)i * ny + j) * Lz + lz, c.pe_pos, c.pe_neg, TAB_BE, TAB_AE, TAB_KE);
+            psi_load<V, true>(pz, c, 2, fz, lz, ((size_t)i * ny + j) * Lz + lz, c.pe_pos, c.pe_neg, c.pe_pos, c.pe_neg, TAB_BE, TAB_AE, TAB_KE);
         }
 
         float dHz_y[V], dHy_z[V], dHx_z[V], dHz_x[V], dHy_x[V], dHx_y[V];
@@ -487,9 +504,13 @@ __device__ __forceinline__ void e_body(const Grid& g, float* __restrict__ F, con
         if constexpr ((AX & AX_Z) != 0) psi_store<V>(pz);
 
         if (row) {
-            stn<V, OE_NT_STORE>(Ex + p, ex);
-            stn<V, OE_NT_STORE>(Ey + p, ey);
-            stn<V, OE_NT_STORE>(Ez + p, ez);
+            stn<V, OE_NT_STORE>(Exw + p, ex);
+            stn<V, OE_NT_STORE>(Eyw + p, ey);
+            stn<V, OE_NT_STORE>(Ezw + p, ez);
+        } else if (copy_all) {  // index-0 planes and rows keep their value
+            stn<V, OE_NT_STORE>(Exw + p, ldn<V, OE_NT_OWN>(Ex + p));
+            stn<V, OE_NT_STORE>(Eyw + p, ldn<V, OE_NT_OWN>(Ey + p));
+            stn<V, OE_NT_STORE>(Ezw + p, ldn<V, OE_NT_OWN>(Ez + p));
         }
         hy_p = hy_c;
         hz_p = hz_c;
@@ -507,7 +528,7 @@ __global__ __launch_bounds__(256) void k_update_h(YeeArgs a, CpmlArgs c) {
         for (int q = threadIdx.x; q < 3 * a.n_mat_h; q += blockDim.x) lds_rows[q] = src[q];
         __syncthreads();
     }
-    h_body<V, AX, PHYS, TAB>(a.g, a.F, a.Cf, a.mat_h, lds_rows, c, decode<V>(a));
+    h_body<V, AX, PHYS, TAB>(a.g, a.Fin, a.Fout, a.Cf, a.mat_h, lds_rows, c, decode<V>(a));
 }
 
 template <int V, int AX, bool PHYS, bool TAB>
@@ -518,7 +539,7 @@ __global__ __launch_bounds__(256) void k_update_e(YeeArgs a, CpmlArgs c) {
         for (int q = threadIdx.x; q < 3 * a.n_mat_e; q += blockDim.x) lds_rows[q] = src[q];
         __syncthreads();
     }
-    e_body<V, AX, PHYS, TAB>(a.g, a.F, a.Cf, a.mat_e, lds_rows, c, decode<V>(a));
+    e_body<V, AX, PHYS, TAB>(a.g, a.Fin, a.Fout, a.Cf, a.mat_e, lds_rows, c, decode<V>(a));
 }
 
 // Many grids in one launch (a parameter sweep): work-group -> box of some
@@ -563,7 +584,7 @@ __global__ __launch_bounds__(256) void k_update_h_sweep(SweepArgs a) {
         for (int q = threadIdx.x; q < 3 * S.n_mat_h; q += blockDim.x) lds_rows[q] = src[q];
         __syncthreads();
     }
-    h_body<V, AX, PHYS, TAB>(S.g, S.F, S.Cf, S.mat_h, lds_rows, S.c, t);
+    h_body<V, AX, PHYS, TAB>(S.g, S.F, S.F, S.Cf, S.mat_h, lds_rows, S.c, t);
 }
 
 template <int V, int AX, bool PHYS, bool TAB>
@@ -578,7 +599,7 @@ __global__ __launch_bounds__(256) void k_update_e_sweep(SweepArgs a) {
         for (int q = threadIdx.x; q < 3 * S.n_mat_e; q += blockDim.x) lds_rows[q] = src[q];
         __syncthreads();
     }
-    e_body<V, AX, PHYS, TAB>(S.g, S.F, S.Cf, S.mat_e, lds_rows, S.c, t);
+    e_body<V, AX, PHYS, TAB>(S.g, S.F, S.F, S.Cf, S.mat_e, lds_rows, S.c, t);
 }
 
 template <int V, int AX, bool PHYS>
@@ -617,6 +638,262 @@ hipError_t dispatch(const YeeArgs& a, const CpmlArgs* c, int axes, hipStream_t s
     return H ? launch_h_t<V, ALL, false>(a, *c, s) : launch_e_t<V, ALL, false>(a, *c, s);
 }
 
+// ------------------------------------------------------------ fused step
+// One pass per timestep over a box: H^{n+1/2} and E^{n+1} of a block of
+// FUSED_ROWS k-rows (one wave each, V = 4 cells per lane) marching along i,
+// reading the old fields (Fo) and writing the new ones (Fn).  The E update
+// of plane i needs the new H of
+//   (i, j, k-1)   the lane to the left (cross-lane shift) or its own vector,
+//   (i, j-1, k)   the wave below, through LDS; for the block's first row the
+//                 extra wave 0, which recomputes the H of the row below the
+//                 block from the old fields (or reads it from Fn where that
+//                 row is a CPML slab row, updated before this launch),
+//   (i-1, j, k)   carried from the previous plane; before the first plane
+//                 of the chunk it is recomputed (or read from Fn likewise).
+// The arithmetic of every cell is that of h_body / e_body, so the result is
+// bit-identical to the two in-place half-steps.  The z-face H psi is read
+// from c.ph_* and the new values stored to c.phw_* (a second buffer), so the
+// recomputations read old psi while owners write new psi.
+template <int AX, bool PHYS, bool TAB>
+__global__ __launch_bounds__((FUSED_ROWS + 1) * 64) void k_fused(FusedArgs a, CpmlArgs c) {
+    constexpr int V = 4;
+    constexpr bool ZC = (AX & AX_Z) != 0;
+    extern __shared__ float4 lds4[];
+    float4* xch = lds4;  // [2 parities][FUSED_ROWS writer waves][Hx, Hz][64 lanes]
+    float2* tabh = reinterpret_cast<float2*>(lds4 + 2 * FUSED_ROWS * 2 * 64);
+    float2* tabe = tabh + 3 * a.n_mat_h;
+    const int tid = threadIdx.x, w = tid >> 6, kx = tid & 63;
+    if constexpr (TAB) {
+        const float2* sh = reinterpret_cast<const float2*>(a.tab_h);
+        const float2* se = reinterpret_cast<const float2*>(a.tab_e);
+        for (int q = tid; q < 3 * a.n_mat_h; q += blockDim.x) tabh[q] = sh[q];
+        for (int q = tid; q < 3 * a.n_mat_e; q += blockDim.x) tabe[q] = se[q];
+    }
+    __syncthreads();
+
+    int bid = blockIdx.x;
+    if (a.xcd_remap) {
+        const int per = gridDim.x >> 3;
+        if (bid < (per << 3)) bid = (bid & 7) * per + (bid >> 3);
+    }
+    const int jb = bid % a.n_jb;
+    const int ia = a.i0 + (bid / a.n_jb) * a.ci;
+    const int ie = min(ia + a.ci, a.i1);
+    const int j = a.j0 + jb * FUSED_ROWS - 1 + w;  // wave 0: the row below the block
+    const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
+    const size_t N = a.g.N;
+    const size_t plane = (size_t)ny * nz;
+    const int k0 = kx * V;
+    const bool lane_on = k0 < nz;
+    const bool owner = w > 0 && j < a.j1;
+    const bool halo_comp = w == 0 && j >= a.j0;
+    const bool halo_read = w == 0 && j >= 0 && j < a.j0;
+    const bool comp = lane_on && (owner || halo_comp);
+
+    const float* __restrict__ Ex = a.Fo;
+    const float* __restrict__ Ey = a.Fo + N;
+    const float* __restrict__ Ez = a.Fo + 2 * N;
+    const float* __restrict__ Hxo = a.Fo + 3 * N;
+    const float* __restrict__ Hyo = a.Fo + 4 * N;
+    const float* __restrict__ Hzo = a.Fo + 5 * N;
+    float* __restrict__ Exn = a.Fn;
+    float* __restrict__ Eyn = a.Fn + N;
+    float* __restrict__ Ezn = a.Fn + 2 * N;
+    float* __restrict__ Hxn = a.Fn + 3 * N;
+    float* __restrict__ Hyn = a.Fn + 4 * N;
+    float* __restrict__ Hzn = a.Fn + 5 * N;
+
+    int lz = 0, fz = -1;
+    if constexpr (ZC) fz = face_of(c, 2, k0, lz);
+    const int Lz = fz == 4 ? c.L[4] : (fz == 5 ? c.L[5] : 0);
+    const bool hj = j + 1 < ny;
+    const bool hk_last = k0 + V < nz;
+    bool upd_all[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) upd_all[e] = true;
+
+    // H^{n+1/2} of (i, j, k0..k0+3) from the old fields (h_body's arithmetic);
+    // ey_c/ez_c hold the old Ey/Ez of plane i, ey_n/ez_n those of plane i+1.
+    auto h_step = [&](int i, size_t p, const Vec<V>& ey_c, const Vec<V>& ez_c, const Vec<V>& ey_n,
+                      const Vec<V>& ez_n, Vec<V>& ex_c, Vec<V>& hx, Vec<V>& hy, Vec<V>& hz, Psi<V>& pz) {
+        const bool hi = i + 1 < nx;
+        ex_c = ldv<V>(Ex + p);
+        Vec<V> ez_j = ez_c, ex_j = ex_c;
+        if (hj) {
+            ez_j = ldv<V>(Ez + p + nz);
+            ex_j = ldv<V>(Ex + p + nz);
+        }
+        float ey_kl = 0.f, ex_kl = 0.f;
+        if (hk_last) {
+            ey_kl = Ey[p + V];
+            ex_kl = Ex[p + V];
+        }
+        hx = ldn<V, OE_NT_OWN>(Hxo + p);
+        hy = ldn<V, OE_NT_OWN>(Hyo + p);
+        hz = ldn<V, OE_NT_OWN>(Hzo + p);
+        Vec<V> a0, a1, a2, b0, b1, b2;
+        if constexpr (TAB) {
+            coef_rows<V>(a.mat_h + p, tabh, a0, a1, a2, b0, b1, b2);
+        } else {
+            a0 = ldn<V, OE_NT_COEF>(a.Cf + 6 * N + p); a1 = ldn<V, OE_NT_COEF>(a.Cf + 7 * N + p);
+            a2 = ldn<V, OE_NT_COEF>(a.Cf + 8 * N + p); b0 = ldn<V, OE_NT_COEF>(a.Cf + 9 * N + p);
+            b1 = ldn<V, OE_NT_COEF>(a.Cf + 10 * N + p); b2 = ldn<V, OE_NT_COEF>(a.Cf + 11 * N + p);
+        }
+        if constexpr (ZC)
+            psi_load<V, true>(pz, c, 2, fz, lz, ((size_t)i * ny + j) * Lz + lz, c.ph_pos, c.ph_neg, c.phw_pos,
+                              c.phw_neg, TAB_BH, TAB_AH, TAB_KH);
+        float dEy_z[V], dEx_z[V];
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            const bool hk = (e < V - 1) || hk_last;
+            const float eyk = (e < V - 1) ? ey_c.v[(e < V - 1) ? e + 1 : e] : ey_kl;
+            const float exk = (e < V - 1) ? ex_c.v[(e < V - 1) ? e + 1 : e] : ex_kl;
+            const float dez_dy = hj ? ez_j.v[e] - ez_c.v[e] : 0.f;
+            const float dey_dz = hk ? eyk - ey_c.v[e] : 0.f;
+            const float cx = dez_dy - dey_dz;
+            hx.v[e] = a0.v[e] * hx.v[e] + b0.v[e] * cx;
+            const float dex_dz = hk ? exk - ex_c.v[e] : 0.f;
+            const float dez_dx = hi ? ez_n.v[e] - ez_c.v[e] : 0.f;
+            const float cy = dex_dz - dez_dx;
+            hy.v[e] = a1.v[e] * hy.v[e] + b1.v[e] * cy;
+            const float dey_dx = hi ? ey_n.v[e] - ey_c.v[e] : 0.f;
+            const float dex_dy = hj ? ex_j.v[e] - ex_c.v[e] : 0.f;
+            const float cz = dey_dx - dex_dy;
+            hz.v[e] = a2.v[e] * hz.v[e] + b2.v[e] * cz;
+            dEy_z[e] = dey_dz;
+            dEx_z[e] = dex_dz;
+        }
+        // z: psi_pos <- dEy/dz (Hx), psi_neg <- dEx/dz (Hy)
+        if constexpr (ZC) psi_apply<V, PHYS>(pz, dEy_z, dEx_z, hx, b0, hy, b1, upd_all);
+    };
+
+    const size_t p0 = (size_t)ia * plane + (size_t)(j < 0 ? 0 : j) * nz + k0;
+    Vec<V> ey_c = zerov<V>(), ez_c = zerov<V>();
+    if (comp) {
+        ey_c = ldv<V>(Ey + p0);
+        ez_c = ldv<V>(Ez + p0);
+    }
+    // H^{n+1/2} of plane ia - 1 (its Hy, Hz enter the x-differences of plane ia)
+    Vec<V> hy_p = zerov<V>(), hz_p = zerov<V>();
+    if (owner && lane_on && ia > 0) {
+        const size_t pp = p0 - plane;
+        if (ia - 1 < a.i0) {
+            hy_p = ldv<V>(Hyn + pp);
+            hz_p = ldv<V>(Hzn + pp);
+        } else {
+            Vec<V> ex_t, hx_t;
+            Psi<V> pz_t;
+            h_step(ia - 1, pp, ldv<V>(Ey + pp), ldv<V>(Ez + pp), ey_c, ez_c, ex_t, hx_t, hy_p, hz_p, pz_t);
+        }
+    }
+
+    for (int i = ia; i < ie; ++i) {
+        const int par = (i - ia) & 1;
+        const size_t p = p0 + (size_t)(i - ia) * plane;
+        Vec<V> hx = zerov<V>(), hy = zerov<V>(), hz = zerov<V>(), ex_c = zerov<V>();
+        Vec<V> ey_n = ey_c, ez_n = ez_c;
+        Psi<V> pz;
+        // the E half's coefficient rows and psi, fetched after the barrier
+        // (fetching them with the H half's loads raised the register count
+        // to 218 and was slower)
+        Vec<V> s0, s1, s2, c0, c1, c2;
+        Psi<V> pe;
+        auto e_loads = [&]() {
+            if constexpr (TAB) {
+                coef_rows<V>(a.mat_e + p, tabe, s0, s1, s2, c0, c1, c2);
+            } else {
+                s0 = ldn<V, OE_NT_COEF>(a.Cf + p); s1 = ldn<V, OE_NT_COEF>(a.Cf + N + p);
+                s2 = ldn<V, OE_NT_COEF>(a.Cf + 2 * N + p); c0 = ldn<V, OE_NT_COEF>(a.Cf + 3 * N + p);
+                c1 = ldn<V, OE_NT_COEF>(a.Cf + 4 * N + p); c2 = ldn<V, OE_NT_COEF>(a.Cf + 5 * N + p);
+            }
+            if constexpr (ZC)
+                psi_load<V, true>(pe, c, 2, fz, lz, ((size_t)i * ny + j) * Lz + lz, c.pe_pos, c.pe_neg, c.pe_pos,
+                                  c.pe_neg, TAB_BE, TAB_AE, TAB_KE);
+        };
+        if (comp) {
+            if (i + 1 < nx) {
+                ey_n = ldv<V>(Ey + p + plane);
+                ez_n = ldv<V>(Ez + p + plane);
+            }
+            h_step(i, p, ey_c, ez_c, ey_n, ez_n, ex_c, hx, hy, hz, pz);
+            if (owner) {
+                stn<V, OE_NT_STORE>(Hxn + p, hx);
+                stn<V, OE_NT_STORE>(Hyn + p, hy);
+                stn<V, OE_NT_STORE>(Hzn + p, hz);
+                if constexpr (ZC) psi_store<V>(pz);
+            }
+        } else if (halo_read && lane_on) {
+            hx = ldv<V>(Hxn + p);
+            hz = ldv<V>(Hzn + p);
+        }
+        if (w < FUSED_ROWS) {
+            xch[((par * FUSED_ROWS + w) * 2 + 0) * 64 + kx] = make_float4(hx.v[0], hx.v[1], hx.v[2], hx.v[3]);
+            xch[((par * FUSED_ROWS + w) * 2 + 1) * 64 + kx] = make_float4(hz.v[0], hz.v[1], hz.v[2], hz.v[3]);
+        }
+        const float hy_kf = __shfl_up(hy.v[V - 1], 1, 64);
+        const float hx_kf = __shfl_up(hx.v[V - 1], 1, 64);
+        __syncthreads();
+
+        if (owner && lane_on) {
+            // E^{n+1} (e_body's arithmetic)
+            const bool gi = i > 0, gj = j > 0, gk_first = k0 > 0;
+            const bool row = gi && gj;
+            Vec<V> hx_j = hx, hz_j = hz;
+            if (gj) {
+                const float4 tx = xch[((par * FUSED_ROWS + (w - 1)) * 2 + 0) * 64 + kx];
+                const float4 tz = xch[((par * FUSED_ROWS + (w - 1)) * 2 + 1) * 64 + kx];
+                hx_j.v[0] = tx.x; hx_j.v[1] = tx.y; hx_j.v[2] = tx.z; hx_j.v[3] = tx.w;
+                hz_j.v[0] = tz.x; hz_j.v[1] = tz.y; hz_j.v[2] = tz.z; hz_j.v[3] = tz.w;
+            }
+            e_loads();
+            Vec<V> ex = ex_c, ey = ey_c, ez = ez_c;
+            float dHy_z[V], dHx_z[V];
+            bool upd[V];
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                const bool gk = (e > 0) || gk_first;
+                upd[e] = row && gk;
+                const float hyk = (e > 0) ? hy.v[(e > 0) ? e - 1 : e] : hy_kf;
+                const float hxk = (e > 0) ? hx.v[(e > 0) ? e - 1 : e] : hx_kf;
+                const float dHz_y = gj ? hz.v[e] - hz_j.v[e] : 0.f;
+                dHy_z[e] = gk ? hy.v[e] - hyk : 0.f;
+                dHx_z[e] = gk ? hx.v[e] - hxk : 0.f;
+                const float dHz_x = gi ? hz.v[e] - hz_p.v[e] : 0.f;
+                const float dHy_x = gi ? hy.v[e] - hy_p.v[e] : 0.f;
+                const float dHx_y = gj ? hx.v[e] - hx_j.v[e] : 0.f;
+                if (upd[e]) {
+                    const float cx = dHz_y - dHy_z[e];
+                    ex.v[e] = s0.v[e] * ex.v[e] + c0.v[e] * cx;
+                    const float cy = dHx_z[e] - dHz_x;
+                    ey.v[e] = s1.v[e] * ey.v[e] + c1.v[e] * cy;
+                    const float cz = dHy_x - dHx_y;
+                    ez.v[e] = s2.v[e] * ez.v[e] + c2.v[e] * cz;
+                }
+            }
+            // z: psi_pos <- dHy/dz (Ex), psi_neg <- dHx/dz (Ey)
+            if constexpr (ZC) {
+                psi_apply<V, PHYS>(pe, dHy_z, dHx_z, ex, c0, ey, c1, upd);
+                psi_store<V>(pe);
+            }
+            stn<V, OE_NT_STORE>(Exn + p, ex);
+            stn<V, OE_NT_STORE>(Eyn + p, ey);
+            stn<V, OE_NT_STORE>(Ezn + p, ez);
+            hy_p = hy;
+            hz_p = hz;
+        }
+        ey_c = ey_n;
+        ez_c = ez_n;
+    }
+}
+
+template <int AX, bool PHYS, bool TAB>
+hipError_t launch_fused_t(const FusedArgs& a, const CpmlArgs& c, hipStream_t s) {
+    const size_t lds = 2 * FUSED_ROWS * 2 * 64 * sizeof(float4) +
+                       (TAB ? 3 * (size_t)(a.n_mat_h + a.n_mat_e) * sizeof(float2) : 0);
+    hipLaunchKernelGGL((k_fused<AX, PHYS, TAB>), dim3(a.n_jb * a.n_ib), dim3((FUSED_ROWS + 1) * 64), lds, s, a, c);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t launch_update_h(const YeeArgs& a, const CpmlArgs* c, int axes, hipStream_t s) {
@@ -625,6 +902,18 @@ hipError_t launch_update_h(const YeeArgs& a, const CpmlArgs* c, int axes, hipStr
 
 hipError_t launch_update_e(const YeeArgs& a, const CpmlArgs* c, int axes, hipStream_t s) {
     return a.t.vec == 4 ? dispatch<false, 4>(a, c, axes, s) : dispatch<false, 1>(a, c, axes, s);
+}
+
+bool fused_supported(const Grid& g) { return g.nz % 4 == 0 && g.nz >= 4 && g.nz <= 256; }
+
+hipError_t launch_fused(const FusedArgs& a, const CpmlArgs* c, hipStream_t s) {
+    if (a.n_jb * a.n_ib <= 0) return hipSuccess;
+    const bool tab = a.n_mat_h > 0 && a.n_mat_e > 0;
+    const CpmlArgs none{};
+    if (!c) return tab ? launch_fused_t<0, false, true>(a, none, s) : launch_fused_t<0, false, false>(a, none, s);
+    if (c->physical)
+        return tab ? launch_fused_t<AX_Z, true, true>(a, *c, s) : launch_fused_t<AX_Z, true, false>(a, *c, s);
+    return tab ? launch_fused_t<AX_Z, false, true>(a, *c, s) : launch_fused_t<AX_Z, false, false>(a, *c, s);
 }
 
 // Sweep launches: physical CPML only (every structure of a sweep carries the

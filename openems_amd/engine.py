@@ -525,6 +525,14 @@ class HipEngine:
         check(lib().oe_engine_material_table(self._h, int(enable), r))
         return r[0], r[1]
 
+    def fused_steps(self, mode: int = -1) -> int:
+        """0: two in-place half-steps per timestep (default); 1: one fused
+        pass per timestep where the batch allows it (nz from 128 to 256);
+        2: also for nz below 128; -1 only queries."""
+        cur = C.c_int()
+        check(lib().oe_engine_fused_steps(self._h, int(mode), C.byref(cur)))
+        return cur.value
+
     def set_tiles(self, bx=0, by=0, ci=0, vec=0):
         check(lib().oe_engine_set_tiles(self._h, bx, by, ci, vec))
 

@@ -32,6 +32,8 @@ class Scenario:
     ades: list = field(default_factory=list)  # (kind, params, positions, mode)
     probes: list = field(default_factory=list)
     tiles: Optional[dict] = None
+    fused: Optional[int] = None  # engine step scheme (HipEngine.fused_steps); None: the default
+    table: Optional[int] = None  # material table on (1) or off (0); None: the default
 
 
 def say(msg):
@@ -54,6 +56,10 @@ def run_pair(sc: Scenario, prec: int = 32, batches: Optional[List[int]] = None):
     eng = HipEngine(op)
     if sc.tiles:
         eng.set_tiles(**sc.tiles)
+    if sc.fused is not None:
+        eng.fused_steps(sc.fused)
+    if sc.table is not None:
+        eng.material_table(sc.table)
     dt = op.dt
     assert dt == sim.dt
     exts = []
