@@ -218,71 +218,88 @@ struct ReduceArgs {
     double* NL;            // [n_ang][6] complex
 };
 
-// A work-group takes 64 consecutive angles of the u-sorted order (one per
-// lane, so the lanes of a wave mostly share their G rows and the row loads
-// are broadcasts); its RW waves split the iq range and their partial sums
-// are added in a fixed order through LDS.
-constexpr int RW = 4;  // 8 measured slower (108 vs 84 us per face launch)
+// QL lanes share one angle and split the q sum between them (lane l takes
+// q = l, l + QL, ...), so that consecutive lanes read consecutive G entries;
+// the angles of a work-group are consecutive in the u-sorted order, so a
+// wave's QL-lane groups mostly read the same G row.  The QL partial sums are
+// added by a fixed xor butterfly (deterministic); the angle's first lane adds
+// the result, rotated by the face-position phase, into NL.
+constexpr int QL = 16;
+constexpr int REDUCE_THREADS = 256;
+constexpr int REDUCE_ANGLES = REDUCE_THREADS / QL;
 
-__global__ __launch_bounds__(64 * RW) void k_nf2ff_reduce(ReduceArgs r) {
-    __shared__ double part[RW - 1][8][64];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int pos = r.b0 + blockIdx.x * 64 + lane;
+__global__ __launch_bounds__(REDUCE_THREADS) void k_nf2ff_reduce(ReduceArgs r) {
+    const int ql = threadIdx.x & (QL - 1);
+    const int pos = r.b0 + blockIdx.x * REDUCE_ANGLES + (int)(threadIdx.x / QL);
     const bool act = pos < r.b1;
     const int a = r.order[act ? pos : r.b0];
-    const double sg = r.usgn[a];
+    const float sgf = r.usgn[a];
     const float* gc = r.G + (size_t)(2 * (r.uidx[a] - r.u0)) * r.ldg;
     const float* gs = gc + r.ldg;
     const double kk = r.k * r.kapq[a];
-    const int per = (r.nq + RW - 1) / RW;
-    const int i0 = wave * per, i1 = min(r.nq, i0 + per);
     double sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
     double pr = 1, pi = 0, str = 1, sti = 0;
-    if (r.uniform && i0 < i1) {
-        sincos(kk * r.xq[i0], &pi, &pr);
-        sincos(kk * r.dxq, &sti, &str);
+    if (r.uniform && ql < r.nq) {
+        sincos(kk * r.xq[ql], &pi, &pr);
+        sincos(kk * (QL * r.dxq), &sti, &str);
     }
-    for (int iq = i0; iq < i1; ++iq) {
-        double sn, cs;
-        if (r.uniform) {
-            cs = pr;
-            sn = pi;
-            const double t = pr * str - pi * sti;
-            pi = pr * sti + pi * str;
-            pr = t;
-        } else {
-            sincos(kk * r.xq[iq], &sn, &cs);
-        }
-        const double wr = r.wq[iq] * cs, wi = r.wq[iq] * sn;
-        const float4 cr4 = *reinterpret_cast<const float4*>(gc + 4 * iq);
-        const float4 ci4 = *reinterpret_cast<const float4*>(gc + 4 * r.nq + 4 * iq);
-        const float4 sr4 = *reinterpret_cast<const float4*>(gs + 4 * iq);
-        const float4 si4 = *reinterpret_cast<const float4*>(gs + 4 * r.nq + 4 * iq);
-        const float gcr[4] = {cr4.x, cr4.y, cr4.z, cr4.w}, gci[4] = {ci4.x, ci4.y, ci4.z, ci4.w};
-        const float gsr[4] = {sr4.x, sr4.y, sr4.z, sr4.w}, gsi[4] = {si4.x, si4.y, si4.z, si4.w};
+    // G = GC -/+ j GS in float32 (one rounding of the matrix-core output,
+    // 6e-8 relative), then the phase product accumulated in float64 with
+    // fused multiply-adds.  The loads of the lane's next q are issued before
+    // the current q is accumulated.
+    struct Row {
+        float4 cr, ci, sr, si;
+        double w;
+    };
+    auto fetch = [&](int iq) {
+        Row v;
+        v.cr = *reinterpret_cast<const float4*>(gc + 4 * iq);
+        v.ci = *reinterpret_cast<const float4*>(gc + 4 * r.nq + 4 * iq);
+        v.sr = *reinterpret_cast<const float4*>(gs + 4 * iq);
+        v.si = *reinterpret_cast<const float4*>(gs + 4 * r.nq + 4 * iq);
+        v.w = r.wq[iq];
+        return v;
+    };
+    auto accumulate = [&](const Row& v, double cs, double sn) {
+        const double wr = v.w * cs, wi = v.w * sn;
+        const float gcr[4] = {v.cr.x, v.cr.y, v.cr.z, v.cr.w}, gci[4] = {v.ci.x, v.ci.y, v.ci.z, v.ci.w};
+        const float gsr[4] = {v.sr.x, v.sr.y, v.sr.z, v.sr.w}, gsi[4] = {v.si.x, v.si.y, v.si.z, v.si.w};
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const double gr = (double)gcr[c] - sg * (double)gsi[c];
-            const double gi = (double)gci[c] + sg * (double)gsr[c];
-            sr[c] += wr * gr - wi * gi;
-            si[c] += wr * gi + wi * gr;
+            const double gr = (double)(gcr[c] - sgf * gsi[c]);
+            const double gi = (double)(gci[c] + sgf * gsr[c]);
+            sr[c] = fma(wr, gr, fma(-wi, gi, sr[c]));
+            si[c] = fma(wr, gi, fma(wi, gr, si[c]));
+        }
+    };
+    if (ql < r.nq) {
+        Row cur = fetch(ql);
+        for (int iq = ql; iq < r.nq; iq += QL) {
+            const int nx = iq + QL;
+            Row nxt = cur;
+            if (nx < r.nq) nxt = fetch(nx);
+            if (r.uniform) {
+                // phases by recurrence along the uniform grid
+                accumulate(cur, pr, pi);
+                const double t = pr * str - pi * sti;
+                pi = pr * sti + pi * str;
+                pr = t;
+            } else {
+                double sn, cs;
+                sincos(kk * r.xq[iq], &sn, &cs);
+                accumulate(cur, cs, sn);
+            }
+            cur = nxt;
         }
     }
-    if (wave > 0) {
+#pragma unroll
+    for (int o = QL / 2; o > 0; o >>= 1)
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            part[wave - 1][c][lane] = sr[c];
-            part[wave - 1][4 + c][lane] = si[c];
+            sr[c] += __shfl_xor(sr[c], o, QL);
+            si[c] += __shfl_xor(si[c], o, QL);
         }
-    }
-    __syncthreads();
-    if (wave != 0 || !act) return;
-    for (int w = 0; w < RW - 1; ++w)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            sr[c] += part[w][c][lane];
-            si[c] += part[w][4 + c][lane];
-        }
+    if (ql != 0 || !act) return;
     double sn, cs;
     sincos(r.k * r.kap0[a] * r.x0, &sn, &cs);
 #pragma unroll
@@ -615,7 +632,8 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
             ra.b1 = A.start[u1];
             ra.u0 = u0;
             const int na = ra.b1 - ra.b0;
-            hipLaunchKernelGGL(k_nf2ff_reduce, dim3((na + 63) / 64), dim3(64 * RW), 0, s, ra);
+            hipLaunchKernelGGL(k_nf2ff_reduce, dim3((na + REDUCE_ANGLES - 1) / REDUCE_ANGLES), dim3(REDUCE_THREADS), 0,
+                               s, ra);
             NF_OK(hipGetLastError());
         }
     }
