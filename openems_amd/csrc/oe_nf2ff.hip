@@ -246,7 +246,8 @@ __global__ __launch_bounds__(REDUCE_THREADS) void k_nf2ff_reduce(ReduceArgs r) {
     // G = GC -/+ j GS in float32 (one rounding of the matrix-core output,
     // 6e-8 relative), then the phase product accumulated in float64 with
     // fused multiply-adds.  The loads of the lane's next q are issued before
-    // the current q is accumulated.
+    // the current q is accumulated (two ahead: 138 VGPRs, 3 waves/SIMD,
+    // slower).
     struct Row {
         float4 cr, ci, sr, si;
         double w;
