@@ -21,6 +21,8 @@ Prints one JSON line:
 from __future__ import annotations
 
 import argparse
+import contextlib
+import gc
 import json
 import os
 import sys
@@ -35,6 +37,19 @@ HBM_PEAK_GBS = 8000.0
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+@contextlib.contextmanager
+def quiet_gc():
+    """Collect Python garbage before a timed region and pause the collector
+    inside it: a full collection over the interpreter's heap (torch leaves a
+    large one) takes tens of milliseconds and lands at random."""
+    gc.collect()
+    gc.disable()
+    try:
+        yield
+    finally:
+        gc.enable()
 
 
 def slab_cells(shape, layers):
@@ -312,10 +327,11 @@ def bench_sweep(args):
         keep.append((ms, eng, port))
     torch.cuda.synchronize()
     log(f"[bench] sweep: built in {time.perf_counter() - t_build:.1f} s; running")
-    t0 = time.perf_counter()
-    res = sw.run(args.sweep_max_steps, energy_decay_db=50.0, check_interval=100, use_graph=True, autotune=True)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    with quiet_gc():
+        t0 = time.perf_counter()
+        res = sw.run(args.sweep_max_steps, energy_decay_db=50.0, check_interval=100, use_graph=True, autotune=True)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
     steps = np.array([r.timesteps_executed for r in res])
     cells = np.array([ms.cells for ms, _, _ in keep])
     updates = float((steps * cells).sum())
@@ -374,10 +390,11 @@ def bench_composite(args):
         eng.run_batch(EngineBatch(num_steps=args.warmup, excitations=exc, extensions=list(exts.values())))
     torch.cuda.synchronize()
     exc = port_excitations(2, wl.port_region, smp[args.warmup:])
-    t0 = time.perf_counter()
-    eng.run_batch(EngineBatch(num_steps=args.steps, excitations=exc, extensions=list(exts.values())))
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    with quiet_gc():
+        t0 = time.perf_counter()
+        eng.run_batch(EngineBatch(num_steps=args.steps, excitations=exc, extensions=list(exts.values())))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
     th = np.linspace(0, np.pi, 181)
     ph = np.linspace(0, 2 * np.pi, 361)[:-1]
     t1 = time.perf_counter()
@@ -566,23 +583,30 @@ def main():
         raise SystemExit(f"material table unavailable for this grid (rows {rows})")
     dt = op.dt
     log(f"[bench] tiles {eng.tiles()}; coefficients: {coef} (rows E/H {rows}); warmup {args.warmup} steps")
+    # the warm-up runs with the timed region's launch timing, so that the
+    # timing events' first use falls outside the timed region
+    eng.set_profiling(args.prof_period)
     if args.warmup:
         run_steps(eng, exts, wl, dt, 0, args.warmup)
-
-    eng.set_profiling(args.prof_period)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    res = run_steps(eng, exts, wl, dt, args.warmup, args.steps)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    kernel_names = ("update_h:grid", "update_e:grid", "update_h:interior", "update_e:interior", "update_h:slabs",
+                    "update_e:slabs", "fused:interior", "fused:grid", "point_ops", "energy")
+    before = {k: eng.kernel_time(k) for k in kernel_names}
+    with quiet_gc():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = run_steps(eng, exts, wl, dt, args.warmup, args.steps)
+        t1 = time.perf_counter()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
     eng.set_profiling(0)
     assert res.timesteps_executed == args.steps
-    log(f"[bench] timed {args.steps} steps in {wall:.3f} s")
+    log(f"[bench] timed {args.steps} steps in {wall:.3f} s (batch {res.elapsed_seconds:.3f} s, "
+        f"call {t1 - t0:.3f} s, device sync {wall - (t1 - t0):.4f} s)")
 
     kern = {}
-    for name in ("update_h:grid", "update_e:grid", "update_h:interior", "update_e:interior", "update_h:slabs",
-                 "update_e:slabs", "fused:interior", "fused:grid", "point_ops", "energy"):
+    for name in kernel_names:  # launches timed inside the timed region only
         n, ms = eng.kernel_time(name)
+        n, ms = n - before[name][0], ms - before[name][1]
         if n:
             kern[name] = (n, ms / n)
     yee = [k for k in kern if k.startswith("update_") or k.startswith("fused:")]

@@ -154,6 +154,19 @@ oe_status oe_engine_material_table(oe_engine* eng, int enable, int rows[2]);
  * be NULL) receives the mode in force. */
 oe_status oe_engine_fused_steps(oe_engine* eng, int mode, int* current);
 
+/* Launch scheme of oe_engine_run_batch; no counterpart in the reference,
+ * whose GPU engine submits every step from the host
+ * (src/fdtd/gpu_engine.rs:1039-1271).  The kernels of a step read the step
+ * index from a device counter, so a run of `steps` consecutive steps after
+ * which the host has nothing to do (no energy sample, decay check or timed
+ * step falls inside) is captured once per batch into a HIP graph and
+ * replayed; other steps are launched one by one.  Results are identical
+ * either way.  steps = 0 launches every step directly; the default is 16
+ * (environment variable OE_GRAPH_STEPS); -1 only queries.  Batches with a
+ * DFT sampled every n > 1 steps, and fused steps, launch directly.
+ * *current (may be NULL) receives the value in force. */
+oe_status oe_engine_graph_steps(oe_engine* eng, int steps, int* current);
+
 /* EngineImpl::reset (src/fdtd/engine_impl.rs:70): zero fields and the step
  * counter (extension state is reset with oe_extension_reset). */
 oe_status oe_engine_reset(oe_engine* eng);

@@ -126,6 +126,7 @@ SIGNATURES = {
     "oe_engine_read_coefficients": (_i, [_vp, _i, _fp]),
     "oe_engine_material_table": (_i, [_vp, _i, C.POINTER(C.c_int)]),
     "oe_engine_fused_steps": (_i, [_vp, _i, C.POINTER(C.c_int)]),
+    "oe_engine_graph_steps": (_i, [_vp, _i, C.POINTER(C.c_int)]),
     "oe_engine_reset": (_i, [_vp]),
     "oe_engine_energy": (_i, [_vp, _dp, _dp]),
     "oe_engine_run_batch": (_i, [_vp, C.POINTER(BatchDesc), C.POINTER(BatchResultC)]),

@@ -290,6 +290,7 @@ oe_status engine_alloc(oe_engine* e, size_t nx, size_t ny, size_t nz, int device
     e->tiles = auto_tiles(e->g, 0);
     if (const char* v = getenv("OE_XCD_REMAP")) e->xcd_remap = atoi(v) != 0;
     if (const char* v = getenv("OE_FUSED")) e->fused_mode = std::max(0, std::min(2, atoi(v)));
+    if (const char* v = getenv("OE_GRAPH_STEPS")) e->graph_steps = std::max(0, atoi(v));
     return OE_OK;
 }
 
@@ -494,6 +495,14 @@ oe_status oe_engine_write_fields(oe_engine* e, const float* const fields[6]) {
     return OE_OK;
 }
 
+oe_status oe_engine_graph_steps(oe_engine* e, int steps, int* current) {
+    if (!e) return fail(OE_ERR_CONFIG, "null argument");
+    if (steps < -1 || steps > 4096) return fail(OE_ERR_CONFIG, "graph steps must be -1 (query) or 0 .. 4096");
+    if (steps >= 0) e->graph_steps = steps;
+    if (current) *current = e->graph_steps;
+    return OE_OK;
+}
+
 oe_status oe_engine_fused_steps(oe_engine* e, int mode, int* current) {
     if (!e) return fail(OE_ERR_CONFIG, "null argument");
     if (mode < -1 || mode > 2) return fail(OE_ERR_CONFIG, "mode must be -1, 0, 1 or 2");
@@ -553,6 +562,22 @@ oe_status oe_engine_energy(oe_engine* e, double* ee, double* he) {
 oe_status oe_engine_set_profiling(oe_engine* e, int on) {
     if (!e) return fail(OE_ERR_CONFIG, "null argument");
     e->profiling = on > 0 ? on : 0;
+    if (e->profiling && e->event_pool.size() < 64) {
+        // Create the event pool and time one empty interval now: the first
+        // timed events of a process can cost tens of milliseconds of runtime
+        // set-up, which must not land inside a measured batch.
+        HIP_OK(hipSetDevice(e->device));
+        while (e->event_pool.size() < 64) {
+            hipEvent_t ev = nullptr;
+            HIP_OK(hipEventCreate(&ev));
+            e->event_pool.push_back(ev);
+        }
+        HIP_OK(hipEventRecord(e->event_pool[0], e->stream));
+        HIP_OK(hipEventRecord(e->event_pool[1], e->stream));
+        HIP_OK(hipEventSynchronize(e->event_pool[1]));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, e->event_pool[0], e->event_pool[1]));
+    }
     return OE_OK;
 }
 
@@ -1553,7 +1578,15 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
     if (pp.size() * sizeof(PointProbe) > e->probe_table.bytes) HIP_OK(e->probe_table.alloc(pp.size() * sizeof(PointProbe) + 64));
     if (!pp.empty())
         HIP_OK(hipMemcpyAsync(e->probe_table.p, pp.data(), pp.size() * sizeof(PointProbe), hipMemcpyHostToDevice, e->stream));
-    const bool point_ops = !px.empty() || !pp.empty();
+    // Sources are added by the E kernels themselves when nothing acts between
+    // the E update and them (no ADE, Mur or UPML post-E hooks), when every
+    // source cell is one the E update writes (i, j >= 1) and the list is
+    // short; the point kernel then only records probes.
+    bool fold_sources = !px.empty() && px.size() <= (size_t)MAX_FOLD_SOURCES && murs.empty() && ades.empty() && !upml;
+    for (size_t q = 0; q < b->n_excitations && fold_sources; ++q)
+        if (b->excitations[q].i == 0 || b->excitations[q].j == 0) fold_sources = false;
+    if (const char* v = getenv("OE_FOLD_SOURCES")) fold_sources = fold_sources && atoi(v) != 0;
+    const bool point_ops = (!px.empty() && !fold_sources) || !pp.empty();
 
     Tiles tl = e->tiles;
     if (cp && !cp->zvec && tl.vec == 4) tl = auto_tiles(e->g, 1);  // z slabs not 4-aligned: scalar lanes
@@ -1561,7 +1594,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
     // Fused steps (one pass per step over the plan's interior box) when no
     // extension hooks in between the half-steps and the grid suits the
     // fused kernel; the x/y CPML slabs keep their two launches around it.
-    const bool fused = e->fused_mode && !upml && murs.empty() && ades.empty() && tl.vec == 4 &&
+    const bool fused = !fold_sources && e->fused_mode && !upml && murs.empty() && ades.empty() && tl.vec == 4 &&
                        fused_supported(e->g) && (e->fused_mode == 2 || e->g.nz >= 128) && (!cp || cp->zvec);
     FusedArgs fa{};
     std::vector<Launch> slabs;
@@ -1625,9 +1658,23 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
     uint64_t executed = 0;
     int reason = OE_TERM_STEPS_COMPLETED;
     double final_db = 0.0;
-    for (uint64_t st = 0; st < num_steps; ++st) {
-        const uint64_t cur = e->timestep;
-        e->prof_step = e->profiling > 0 && (st % (uint64_t)e->profiling) == 0;
+
+    // The step index seen by the kernels lives on the device (batch-relative),
+    // so a run of steps without host work in between (no energy sample, decay
+    // check or timed step) is replayed from one captured graph.  The DFTs
+    // then form their sample times from the batch's first step.
+    if (e->step_dev.bytes < sizeof(uint64_t)) HIP_OK(e->step_dev.alloc(sizeof(uint64_t)));
+    uint64_t* step_dev = e->step_dev.as<uint64_t>();
+    HIP_OK(hipMemsetAsync(step_dev, 0, sizeof(uint64_t), e->stream));
+    const uint64_t t_batch0 = e->timestep;
+    bool dev_dft = true;
+    std::vector<uint64_t> dft0(dfts.size());
+    for (size_t q = 0; q < dfts.size(); ++q) {
+        dev_dft = dev_dft && dfts[q]->dft_stride == 1;
+        dft0[q] = dfts[q]->dft_count;
+    }
+
+    auto enqueue_step = [&](uint64_t st, uint64_t cur) -> oe_status {
         if (fused) {
             // slab H (old -> new buffer), the fused interior, slab E; then the
             // new buffer becomes the current one
@@ -1701,7 +1748,17 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
         if (!fused)
             for (const Launch& L : plan) {
                 Timed tm(e, (L.axes & (AX_X | AX_Y)) ? "update_e:slabs" : (cp ? "update_e:interior" : "update_e:grid"));
-                HIP_OK(launch_update_e(L.a, cp, L.axes, e->stream));
+                if (fold_sources) {
+                    YeeArgs a = L.a;
+                    a.exc = e->exc_table.as<PointExc>();
+                    a.n_exc = (int)px.size();
+                    for (size_t q = 0; q < px.size(); ++q) a.exc_idx[q] = px[q].idx;
+                    a.exc_step = st;
+                    a.exc_step_dev = step_dev;
+                    HIP_OK(launch_update_e(a, cp, L.axes, e->stream));
+                } else {
+                    HIP_OK(launch_update_e(L.a, cp, L.axes, e->stream));
+                }
             }
         if (upml) {
             Timed tm(e, "upml");
@@ -1717,18 +1774,83 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
         }
         if (point_ops) {
             Timed tm(e, "point_ops");
-            HIP_OK(launch_point_ops(e->F, e->g.N, e->exc_table.as<PointExc>(), (int)px.size(), st,
-                                    e->probe_table.as<PointProbe>(), (int)pp.size(), e->stream));
+            HIP_OK(launch_point_ops(e->F, e->g.N, e->exc_table.as<PointExc>(), fold_sources ? 0 : (int)px.size(), st,
+                                    e->probe_table.as<PointProbe>(), (int)pp.size(), e->stream, step_dev));
         }
-        for (auto* x : dfts) {
-            if (x->dft_seen++ % x->dft_stride != 0) continue;
-            Timed tm(e, "dft");
-            HIP_OK(launch_dft(e->F, e->g.N, x->dft, x->dft_count, (double)(cur + 1) * e->dt, ((double)cur + 0.5) * e->dt,
-                              e->stream));
-            x->dft_count += 1;
+        for (size_t q = 0; q < dfts.size(); ++q) {
+            oe_extension* x = dfts[q];
+            if (dev_dft) {
+                Timed tm(e, "dft");
+                HIP_OK(launch_dft_dev(e->F, e->g.N, x->dft, step_dev, dft0[q], (double)t_batch0, e->dt, e->stream));
+            } else {
+                if (x->dft_seen++ % x->dft_stride != 0) continue;
+                Timed tm(e, "dft");
+                HIP_OK(launch_dft(e->F, e->g.N, x->dft, x->dft_count, (double)(cur + 1) * e->dt,
+                                  ((double)cur + 0.5) * e->dt, e->stream));
+                x->dft_count += 1;
+            }
         }
+        HIP_OK(launch_sweep_step(step_dev, e->stream));
+        return OE_OK;
+    };
+
+    const int LG = e->graph_steps;
+    const bool use_graph = LG > 0 && !fused && dev_dft;
+    struct GraphGuard {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = nullptr;
+        ~GraphGuard() {
+            if (x) (void)hipGraphExecDestroy(x);
+            if (g) (void)hipGraphDestroy(g);
+        }
+    } graph;
+    auto capture = [&]() -> oe_status {
+        const bool ps = e->prof_step;
+        e->prof_step = false;
+        HIP_OK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        oe_status cs = OE_OK;
+        for (int q = 0; q < LG && cs == OE_OK; ++q) cs = enqueue_step(0, 0);
+        const hipError_t ce = hipStreamEndCapture(e->stream, &graph.g);
+        e->prof_step = ps;
+        if (cs != OE_OK) return cs;
+        HIP_OK(ce);
+        HIP_OK(hipGraphInstantiate(&graph.x, graph.g, nullptr, nullptr, 0));
+        return OE_OK;
+    };
+
+    uint64_t st = 0;
+    while (st < num_steps) {
+        const uint64_t cur = e->timestep;
+        // steps from cur on after which the host has nothing to do
+        uint64_t nfree = num_steps - st;
+        if (b->sample_interval > 0)
+            nfree = std::min<uint64_t>(nfree, (b->sample_interval - cur % b->sample_interval) % b->sample_interval);
+        if (need_decay)
+            nfree = std::min<uint64_t>(nfree, cur == 0 ? check_interval : (check_interval - cur % check_interval) % check_interval);
+        if (e->profiling)
+            nfree = std::min<uint64_t>(nfree, ((uint64_t)e->profiling - st % (uint64_t)e->profiling) % (uint64_t)e->profiling);
+        if (use_graph && nfree >= (uint64_t)LG) {
+            if (!graph.x) OE_TRY(capture());
+            HIP_OK(hipGraphLaunch(graph.x, e->stream));
+            e->timestep += LG;
+            executed += LG;
+            st += LG;
+            for (auto* x : dfts) {
+                x->dft_seen += LG;
+                x->dft_count += LG;
+            }
+            continue;
+        }
+        e->prof_step = e->profiling > 0 && (st % (uint64_t)e->profiling) == 0;
+        OE_TRY(enqueue_step(st, cur));
+        if (dev_dft)
+            for (auto* x : dfts) {
+                x->dft_seen += 1;
+                x->dft_count += 1;
+            }
         e->timestep += 1;
         ++executed;
+        ++st;
 
         if (b->sample_interval > 0 && cur % b->sample_interval == 0) {
             if (sample_steps.size() >= e->energy_dev_cap) OE_TRY(drain());

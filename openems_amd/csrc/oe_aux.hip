@@ -35,7 +35,9 @@ __global__ void k_box_fill(float* __restrict__ base, int ny, int nz, int i0, int
 // in that order; then each probe lane records its node (FieldProbe::sample,
 // src/processing/probes.rs:289-303).
 __global__ void k_point_ops(float* __restrict__ F, size_t N, const PointExc* __restrict__ exc, int n_exc,
-                            uint64_t step, const PointProbe* __restrict__ probes, int n_probes) {
+                            uint64_t step, const PointProbe* __restrict__ probes, int n_probes,
+                            const uint64_t* __restrict__ step_dev) {
+    if (step_dev) step = *step_dev;
     if (threadIdx.x == 0) {
         for (int e = 0; e < n_exc; ++e) {
             const PointExc x = exc[e];
@@ -238,9 +240,20 @@ __global__ void k_ade_post(float* __restrict__ F, const float* __restrict__ Cf, 
 // frequencies is two loads and the accumulator update.
 constexpr int DFT_FCHUNK = 128;
 
+// With step_dev the step comes from the device counter (a captured graph):
+// sample = sample + st, tE = (t0 + st + 1) dt, tH = (t0 + st + 1/2) dt, the
+// same doubles the host forms from the absolute step.
 __global__ __launch_bounds__(256) void k_dft(const float* __restrict__ F, size_t N, DftDev d, uint64_t sample,
-                                            double tE, double tH) {
+                                            double tE, double tH, const uint64_t* __restrict__ step_dev, double t0,
+                                            double dt) {
     __shared__ double ph[2][DFT_FCHUNK][2];  // [E, H][f][cos, -sin]
+    if (step_dev) {
+        const uint64_t st = *step_dev;
+        sample += st;
+        const double cur = t0 + (double)st;
+        tE = (cur + 1.0) * dt;
+        tH = (cur + 0.5) * dt;
+    }
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = m < d.n_sig;
     double s = 0.0;
@@ -481,8 +494,8 @@ hipError_t launch_box_fill(float* base, const Grid& g, const int box[6], float v
 }
 
 hipError_t launch_point_ops(float* F, size_t N, const PointExc* exc, int n_exc, uint64_t step,
-                            const PointProbe* probes, int n_probes, hipStream_t s) {
-    hipLaunchKernelGGL(k_point_ops, dim3(1), dim3(64), 0, s, F, N, exc, n_exc, step, probes, n_probes);
+                            const PointProbe* probes, int n_probes, hipStream_t s, const uint64_t* step_dev) {
+    hipLaunchKernelGGL(k_point_ops, dim3(1), dim3(64), 0, s, F, N, exc, n_exc, step, probes, n_probes, step_dev);
     return hipGetLastError();
 }
 
@@ -551,7 +564,15 @@ hipError_t launch_upml_post(float* F, const Grid& g, const UpmlDev& u, bool e, h
 hipError_t launch_dft(const float* F, size_t N, const DftDev& d, uint64_t sample, double tE, double tH,
                       hipStream_t s) {
     if (d.n_sig <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_dft, dim3((d.n_sig + 255) / 256), dim3(256), 0, s, F, N, d, sample, tE, tH);
+    hipLaunchKernelGGL(k_dft, dim3((d.n_sig + 255) / 256), dim3(256), 0, s, F, N, d, sample, tE, tH,
+                       (const uint64_t*)nullptr, 0.0, 0.0);
+    return hipGetLastError();
+}
+
+hipError_t launch_dft_dev(const float* F, size_t N, const DftDev& d, const uint64_t* step_dev, uint64_t sample0,
+                          double t0, double dt, hipStream_t s) {
+    hipLaunchKernelGGL(k_dft, dim3((d.n_sig + 255) / 256), dim3(256), 0, s, F, N, d, sample0, 0.0, 0.0, step_dev, t0,
+                       dt);
     return hipGetLastError();
 }
 

@@ -84,6 +84,9 @@ struct Box {
     int n_jb, n_ib;  // j-blocks, i-chunks of this box
 };
 
+struct PointExc;
+constexpr int MAX_FOLD_SOURCES = 8;
+
 struct YeeArgs {
     Grid g;
     // Fields read (Fin) and written (Fout); equal for an in-place update.
@@ -107,6 +110,16 @@ struct YeeArgs {
     Box box[MAX_BOXES];
     int first[MAX_BOXES + 1];
     int xcd_remap;         // 1: make consecutive tiles share an XCD (L2)
+    // Point sources added by the E half-step after its update (when no
+    // extension acts between the E update and the sources): the lane holding
+    // a source's cell adds or sets samples[exc_step], in list order.  The
+    // cell indices travel in the kernel arguments, so the per-plane test is
+    // a comparison with uniform values.
+    const PointExc* exc;
+    int n_exc;
+    uint32_t exc_idx[MAX_FOLD_SOURCES];
+    uint64_t exc_step;
+    const uint64_t* exc_step_dev;  // non-null: the step comes from the device counter
 };
 
 // CPML code compiled into a launch: bit 0 x faces, bit 1 y faces, bit 2 z faces.
@@ -206,8 +219,10 @@ struct PointProbe {
     uint64_t capacity;
     uint64_t* count;  // device counter
 };
+// step_dev non-null: the step index is read from the device counter.
 hipError_t launch_point_ops(float* F, size_t N, const PointExc* exc, int n_exc, uint64_t step,
-                            const PointProbe* probes, int n_probes, hipStream_t s);
+                            const PointProbe* probes, int n_probes, hipStream_t s,
+                            const uint64_t* step_dev = nullptr);
 
 // Sum of squares of E (out[0]) and H (out[1]); partials has >= 2*blocks doubles.
 hipError_t launch_energy(const float* F, size_t N, double* partials, int blocks, double* out,
@@ -272,6 +287,9 @@ struct DftDev {
 };
 hipError_t launch_dft(const float* F, size_t N, const DftDev& d, uint64_t sample, double tE, double tH,
                       hipStream_t s);
+// Step from the device counter: sample = sample0 + st, times from t0 + st.
+hipError_t launch_dft_dev(const float* F, size_t N, const DftDev& d, const uint64_t* step_dev, uint64_t sample0,
+                          double t0, double dt, hipStream_t s);
 
 // Sweep step pieces (oe_aux.hip).  The step index is read from *step.
 struct SweepExc {

@@ -136,6 +136,8 @@ struct oe_engine {
     double* energy_host = nullptr;  // pinned
     // point ops
     DevBuf exc_samples;
+    DevBuf step_dev;       // batch-relative step read by the kernels of a step
+    int graph_steps = 16;  // steps per captured graph (0: no graphs)
     DevBuf exc_table;
     DevBuf probe_table;
     // profiling: 0 off, n > 0 times every launch of every n-th step

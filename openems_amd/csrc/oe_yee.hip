@@ -384,10 +384,44 @@ __device__ __forceinline__ void h_body(const Grid& g, const float* Fin, float* F
     }
 }
 
+// Point sources of the lane's V cells (BasicEngine::apply_excitations,
+// src/fdtd/basic_engine.rs:313-395): list order, soft adds, hard sets.
+struct Sources {
+    const PointExc* exc;
+    const uint32_t* idx;  // cell indices (kernel arguments)
+    int n;
+    uint64_t step;
+    const uint64_t* step_dev;  // non-null: read the step here (graph replay)
+};
+
+template <int V>
+__device__ __forceinline__ void apply_sources(const Sources& src, size_t p, Vec<V>& ex, Vec<V>& ey, Vec<V>& ez) {
+    for (int q = 0; q < src.n; ++q) {
+        const size_t idx = src.idx[q];
+        if (idx < p || idx >= p + V) continue;
+        const PointExc* x = src.exc + q;
+        const uint64_t step = src.step_dev ? *src.step_dev : src.step;
+        if (step >= x->n_samples) continue;
+        const float v = x->samples[step];
+        const bool soft = x->soft != 0;
+        const uint32_t comp = x->comp;
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+            if (idx != p + m) continue;
+            const float fx = soft ? ex.v[m] + v : v;
+            const float fy = soft ? ey.v[m] + v : v;
+            const float fz = soft ? ez.v[m] + v : v;
+            ex.v[m] = comp == 0 ? fx : ex.v[m];
+            ey.v[m] = comp == 1 ? fy : ey.v[m];
+            ez.v[m] = comp == 2 ? fz : ez.v[m];
+        }
+    }
+}
+
 template <int V, int AX, bool PHYS, bool TAB>
 __device__ __forceinline__ void e_body(const Grid& g, const float* Fin, float* Fout, const float* __restrict__ Cf,
                                        const uint16_t* __restrict__ mat, const float2* tab,
-                                       const CpmlArgs& c, const Tile& t) {
+                                       const CpmlArgs& c, const Tile& t, const Sources* src = nullptr) {
     const int ny = g.ny, nz = g.nz;
     if (t.k0 >= nz || t.j >= t.jend) return;
     const size_t N = g.N;
@@ -504,6 +538,7 @@ __device__ __forceinline__ void e_body(const Grid& g, const float* Fin, float* F
         if constexpr ((AX & AX_Z) != 0) psi_store<V>(pz);
 
         if (row) {
+            if (src) apply_sources<V>(*src, p, ex, ey, ez);
             stn<V, OE_NT_STORE>(Exw + p, ex);
             stn<V, OE_NT_STORE>(Eyw + p, ey);
             stn<V, OE_NT_STORE>(Ezw + p, ez);
@@ -539,7 +574,12 @@ __global__ __launch_bounds__(256) void k_update_e(YeeArgs a, CpmlArgs c) {
         for (int q = threadIdx.x; q < 3 * a.n_mat_e; q += blockDim.x) lds_rows[q] = src[q];
         __syncthreads();
     }
-    e_body<V, AX, PHYS, TAB>(a.g, a.Fin, a.Fout, a.Cf, a.mat_e, lds_rows, c, decode<V>(a));
+    if (a.n_exc) {
+        const Sources src{a.exc, a.exc_idx, a.n_exc, a.exc_step, a.exc_step_dev};
+        e_body<V, AX, PHYS, TAB>(a.g, a.Fin, a.Fout, a.Cf, a.mat_e, lds_rows, c, decode<V>(a), &src);
+    } else {
+        e_body<V, AX, PHYS, TAB>(a.g, a.Fin, a.Fout, a.Cf, a.mat_e, lds_rows, c, decode<V>(a));
+    }
 }
 
 // Many grids in one launch (a parameter sweep): work-group -> box of some

@@ -525,6 +525,13 @@ class HipEngine:
         check(lib().oe_engine_material_table(self._h, int(enable), r))
         return r[0], r[1]
 
+    def graph_steps(self, steps: int = -1) -> int:
+        """Steps per captured HIP graph (0: launch every step directly;
+        default 16); -1 only queries."""
+        cur = C.c_int()
+        check(lib().oe_engine_graph_steps(self._h, int(steps), C.byref(cur)))
+        return cur.value
+
     def fused_steps(self, mode: int = -1) -> int:
         """0: two in-place half-steps per timestep (default); 1: one fused
         pass per timestep where the batch allows it (nz from 128 to 256);
@@ -645,7 +652,12 @@ class HipEngine:
                                     a.ctypes.data_as(C.POINTER(C.c_float)), a.size)
         n_ext = len(batch.extensions)
         exts = (C.c_void_p * max(n_ext, 1))(*[x._h.value for x in batch.extensions])
+        # room for the samples this batch can produce: one per sample_interval
+        # steps (allocating the full default capacity, 3.2 MB, costs tens of
+        # milliseconds when the pages are fresh)
         cap = batch.energy_capacity
+        if batch.num_steps is not None:
+            cap = min(cap, batch.num_steps // batch.sample_interval + 1 if batch.sample_interval > 0 else 0)
         samples = (_ffi.EnergySample * max(cap, 1))()
         d = _ffi.BatchDesc()
         d.has_num_steps = int(batch.num_steps is not None)

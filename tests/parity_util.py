@@ -34,6 +34,7 @@ class Scenario:
     tiles: Optional[dict] = None
     fused: Optional[int] = None  # engine step scheme (HipEngine.fused_steps); None: the default
     table: Optional[int] = None  # material table on (1) or off (0); None: the default
+    graph: Optional[int] = None  # steps per captured graph (0: direct launches); None: the default
 
 
 def say(msg):
@@ -60,6 +61,8 @@ def run_pair(sc: Scenario, prec: int = 32, batches: Optional[List[int]] = None):
         eng.fused_steps(sc.fused)
     if sc.table is not None:
         eng.material_table(sc.table)
+    if sc.graph is not None:
+        eng.graph_steps(sc.graph)
     dt = op.dt
     assert dt == sim.dt
     exts = []
