@@ -199,11 +199,15 @@ __global__ __launch_bounds__(256) void k_nf2ff_pack(const void* __restrict__ src
     if (tid == 0) part[blockIdx.x] = red[0];
 }
 
+#ifndef REDUCE_M
+#define REDUCE_M 2  // 2: 0.894 ms device per frequency; 3: 0.978; 4: 0.914 (fewer waves)
+#endif
 struct ReduceArgs {
     const float* G;        // GEMM output of this chunk: rows 2 (u - u0) (C) and 2 (u - u0) + 1 (S)
     int ldg;
     const int* order;      // angles sorted by u
-    int b0, b1;            // this chunk's range of order[]
+    const int2* groups;    // (first index into order, count) of up to REDUCE_M angles of one row
+    int g0, g1;            // this chunk's range of groups[]
     int u0;
     const int* uidx;       // per angle
     const float* usgn;     // per angle: sign of kappa_p
@@ -218,96 +222,100 @@ struct ReduceArgs {
     double* NL;            // [n_ang][6] complex
 };
 
-// QL lanes share one angle and split the q sum between them (lane l takes
-// q = l, l + QL, ...), so that consecutive lanes read consecutive G entries;
-// the angles of a work-group are consecutive in the u-sorted order, so a
-// wave's QL-lane groups mostly read the same G row.  The QL partial sums are
-// added by a fixed xor butterfly (deterministic); the angle's first lane adds
-// the result, rotated by the face-position phase, into NL.
+// Second contraction, one group of angles at a time: the angles of a group
+// share one G row (the same |kappa_p|), so each G load serves all of them.
+// QL lanes split the q sum (lane l takes q = l, l + QL, ...; consecutive
+// lanes read consecutive G entries); the QL partial sums of every angle are
+// added by a fixed xor butterfly (deterministic) and the group's first lane
+// adds the results, rotated by the face-position phase, into NL.  G = GC -/+
+// j GS is formed in float32 (one rounding of the matrix-core output, 6e-8
+// relative); the phase products are accumulated in float64 with fused
+// multiply-adds.
 constexpr int QL = 16;
 constexpr int REDUCE_THREADS = 256;
-constexpr int REDUCE_ANGLES = REDUCE_THREADS / QL;
+constexpr int REDUCE_GROUPS = REDUCE_THREADS / QL;
 
 __global__ __launch_bounds__(REDUCE_THREADS) void k_nf2ff_reduce(ReduceArgs r) {
+    constexpr int M = REDUCE_M;
     const int ql = threadIdx.x & (QL - 1);
-    const int pos = r.b0 + blockIdx.x * REDUCE_ANGLES + (int)(threadIdx.x / QL);
-    const bool act = pos < r.b1;
-    const int a = r.order[act ? pos : r.b0];
-    const float sgf = r.usgn[a];
-    const float* gc = r.G + (size_t)(2 * (r.uidx[a] - r.u0)) * r.ldg;
-    const float* gs = gc + r.ldg;
-    const double kk = r.k * r.kapq[a];
-    double sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
-    double pr = 1, pi = 0, str = 1, sti = 0;
-    if (r.uniform && ql < r.nq) {
-        sincos(kk * r.xq[ql], &pi, &pr);
-        sincos(kk * (QL * r.dxq), &sti, &str);
-    }
-    // G = GC -/+ j GS in float32 (one rounding of the matrix-core output,
-    // 6e-8 relative), then the phase product accumulated in float64 with
-    // fused multiply-adds.  The loads of the lane's next q are issued before
-    // the current q is accumulated (two ahead: 138 VGPRs, 3 waves/SIMD,
-    // slower).
-    struct Row {
-        float4 cr, ci, sr, si;
-        double w;
-    };
-    auto fetch = [&](int iq) {
-        Row v;
-        v.cr = *reinterpret_cast<const float4*>(gc + 4 * iq);
-        v.ci = *reinterpret_cast<const float4*>(gc + 4 * r.nq + 4 * iq);
-        v.sr = *reinterpret_cast<const float4*>(gs + 4 * iq);
-        v.si = *reinterpret_cast<const float4*>(gs + 4 * r.nq + 4 * iq);
-        v.w = r.wq[iq];
-        return v;
-    };
-    auto accumulate = [&](const Row& v, double cs, double sn) {
-        const double wr = v.w * cs, wi = v.w * sn;
-        const float gcr[4] = {v.cr.x, v.cr.y, v.cr.z, v.cr.w}, gci[4] = {v.ci.x, v.ci.y, v.ci.z, v.ci.w};
-        const float gsr[4] = {v.sr.x, v.sr.y, v.sr.z, v.sr.w}, gsi[4] = {v.si.x, v.si.y, v.si.z, v.si.w};
+    const int gi = r.g0 + blockIdx.x * REDUCE_GROUPS + (int)(threadIdx.x / QL);
+    const bool act = gi < r.g1;
+    const int2 grp = r.groups[act ? gi : r.g0];
+    const int n = act ? grp.y : 0;
+    int am[M];
+    float sgf[M];
+    double pr[M], pi[M], str[M], sti[M];
+    double sr[M][4], si[M][4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const double gr = (double)(gcr[c] - sgf * gsi[c]);
-            const double gi = (double)(gci[c] + sgf * gsr[c]);
-            sr[c] = fma(wr, gr, fma(-wi, gi, sr[c]));
-            si[c] = fma(wr, gi, fma(wi, gr, si[c]));
+    for (int m = 0; m < M; ++m) {
+        am[m] = r.order[grp.x + min(m, grp.y - 1)];
+        sgf[m] = r.usgn[am[m]];
+        const double kk = r.k * r.kapq[am[m]];
+        pr[m] = 1.0; pi[m] = 0.0; str[m] = 1.0; sti[m] = 0.0;
+        if (r.uniform && ql < r.nq) {
+            sincos(kk * r.xq[ql], &pi[m], &pr[m]);
+            sincos(kk * (QL * r.dxq), &sti[m], &str[m]);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sr[m][c] = si[m][c] = 0.0;
+    }
+    const float* gc = r.G + (size_t)(2 * (r.uidx[am[0]] - r.u0)) * r.ldg;
+    const float* gs = gc + r.ldg;
+    auto body = [&](int iq, bool uniform) {
+        const float4 cr4 = *reinterpret_cast<const float4*>(gc + 4 * iq);
+        const float4 ci4 = *reinterpret_cast<const float4*>(gc + 4 * r.nq + 4 * iq);
+        const float4 sr4 = *reinterpret_cast<const float4*>(gs + 4 * iq);
+        const float4 si4 = *reinterpret_cast<const float4*>(gs + 4 * r.nq + 4 * iq);
+        const double w = r.wq[iq];
+        const float gcr[4] = {cr4.x, cr4.y, cr4.z, cr4.w}, gci[4] = {ci4.x, ci4.y, ci4.z, ci4.w};
+        const float gsr[4] = {sr4.x, sr4.y, sr4.z, sr4.w}, gsi[4] = {si4.x, si4.y, si4.z, si4.w};
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            double cs = pr[m], sn = pi[m];
+            if (uniform) {
+                const double t = pr[m] * str[m] - pi[m] * sti[m];
+                pi[m] = pr[m] * sti[m] + pi[m] * str[m];
+                pr[m] = t;
+            } else {
+                sincos(r.k * r.kapq[am[m]] * r.xq[iq], &sn, &cs);
+            }
+            const double wr = w * cs, wi = w * sn;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const double gr = (double)(gcr[c] - sgf[m] * gsi[c]);
+                const double gi = (double)(gci[c] + sgf[m] * gsr[c]);
+                sr[m][c] = fma(wr, gr, fma(-wi, gi, sr[m][c]));
+                si[m][c] = fma(wr, gi, fma(wi, gr, si[m][c]));
+            }
         }
     };
-    if (ql < r.nq) {
-        Row cur = fetch(ql);
-        for (int iq = ql; iq < r.nq; iq += QL) {
-            const int nx = iq + QL;
-            Row nxt = cur;
-            if (nx < r.nq) nxt = fetch(nx);
-            if (r.uniform) {
-                // phases by recurrence along the uniform grid
-                accumulate(cur, pr, pi);
-                const double t = pr * str - pi * sti;
-                pi = pr * sti + pi * str;
-                pr = t;
-            } else {
-                double sn, cs;
-                sincos(kk * r.xq[iq], &sn, &cs);
-                accumulate(cur, cs, sn);
-            }
-            cur = nxt;
-        }
+    if (r.uniform) {
+        for (int iq = ql; iq < r.nq; iq += QL) body(iq, true);
+    } else {
+        for (int iq = ql; iq < r.nq; iq += QL) body(iq, false);
     }
 #pragma unroll
     for (int o = QL / 2; o > 0; o >>= 1)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            sr[c] += __shfl_xor(sr[c], o, QL);
-            si[c] += __shfl_xor(si[c], o, QL);
-        }
-    if (ql != 0 || !act) return;
-    double sn, cs;
-    sincos(r.k * r.kap0[a] * r.x0, &sn, &cs);
+        for (int m = 0; m < M; ++m)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        double* o = r.NL + ((size_t)a * 6 + r.slot[c]) * 2;
-        o[0] += cs * sr[c] - sn * si[c];
-        o[1] += cs * si[c] + sn * sr[c];
+            for (int c = 0; c < 4; ++c) {
+                sr[m][c] += __shfl_xor(sr[m][c], o, QL);
+                si[m][c] += __shfl_xor(si[m][c], o, QL);
+            }
+    if (ql != 0) return;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        if (m >= n) break;
+        const int a = am[m];
+        double sn, cs;
+        sincos(r.k * r.kap0[a] * r.x0, &sn, &cs);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            double* o = r.NL + ((size_t)a * 6 + r.slot[c]) * 2;
+            o[0] += cs * sr[m][c] - sn * si[m][c];
+            o[1] += cs * si[m][c] + sn * sr[m][c];
+        }
     }
 }
 
@@ -415,6 +423,8 @@ struct Nf2ffCache {
     struct Axis {
         int U = 0;
         std::vector<int> start;  // U + 1 offsets into order
+        std::vector<int> gstart; // U + 1 offsets into groups
+        DevBuf groups;           // int2 per group: first index into order, angle count (<= REDUCE_M)
         DevBuf uval;             // U doubles (|kappa| of each distinct value)
         DevBuf uidx;             // n_ang ints
         DevBuf usgn;             // n_ang floats
@@ -470,6 +480,17 @@ static hipError_t plan_angles(Nf2ffCache& c, const double* theta, int nt, const 
         }
         A.start.push_back((int)n);
         A.U = (int)uval.size();
+        // groups of up to REDUCE_M angles of one row: the reduction applies
+        // every G load of the row to all of a group's angles
+        std::vector<int2> groups;
+        A.gstart.assign(1, 0);
+        for (int u = 0; u < A.U; ++u) {
+            for (int i = A.start[u]; i < A.start[u + 1]; i += REDUCE_M)
+                groups.push_back(make_int2(i, std::min(REDUCE_M, A.start[u + 1] - i)));
+            A.gstart.push_back((int)groups.size());
+        }
+        NF_OK(A.groups.alloc(groups.size() * sizeof(int2)));
+        NF_OK(hipMemcpyAsync(A.groups.p, groups.data(), groups.size() * sizeof(int2), hipMemcpyHostToDevice, s));
         NF_OK(A.uval.alloc(uval.size() * 8));
         NF_OK(A.uidx.alloc(n * 4));
         NF_OK(A.usgn.alloc(n * 4));
@@ -600,6 +621,7 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
         ra.G = G;
         ra.ldg = pl.N;
         ra.order = A.order.as<int>();
+        ra.groups = A.groups.as<int2>();
         ra.uidx = A.uidx.as<int>();
         ra.usgn = A.usgn.as<float>();
         ra.nq = pl.nq;
@@ -629,11 +651,11 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
             NF_OK(launch_gemm(Am, Bm, G, mp, pl.N, pl.K, s));
             NF_OK(hipEventRecord(gemm_ev.back()->b, s));
             flop += 2.0 * (2.0 * (u1 - u0)) * (8.0 * pl.nq) * pl.np;  // useful (unpadded) work
-            ra.b0 = A.start[u0];
-            ra.b1 = A.start[u1];
+            ra.g0 = A.gstart[u0];
+            ra.g1 = A.gstart[u1];
             ra.u0 = u0;
-            const int na = ra.b1 - ra.b0;
-            hipLaunchKernelGGL(k_nf2ff_reduce, dim3((na + REDUCE_ANGLES - 1) / REDUCE_ANGLES), dim3(REDUCE_THREADS), 0,
+            const int ng = ra.g1 - ra.g0;
+            hipLaunchKernelGGL(k_nf2ff_reduce, dim3((ng + REDUCE_GROUPS - 1) / REDUCE_GROUPS), dim3(REDUCE_THREADS), 0,
                                s, ra);
             NF_OK(hipGetLastError());
         }
