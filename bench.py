@@ -427,12 +427,15 @@ def bench_composite(args):
 
 
 def bench_learned(args):
-    """Learned-stop vs plain microstrip sweeps.  Training data: full runs of
-    --learned-train structures (seed 5) recorded to --learned-max-steps;
-    held-out: --learned-test structures (seed 6) run both plain (to -50 dB)
-    and learned (window 2048 steps + 256-step check, fallback to the plain
-    run when the check's squared error exceeds 1e-4 of the window energy).
-    Reports wall times, the accepted fraction and max |dS11| over 1-20 GHz
+    """Learned-stop vs plain microstrip sweeps.  Training data: runs of
+    --learned-train structures (seed 5) recorded for --learned-max-steps
+    steps; each gives pairs (2048-step window, next 4096 steps) at offsets
+    0, 512, ..., 5632.  Held-out: --learned-test structures (seed 6) run both
+    plain (to -50 dB) and learned (checkpoints at 2048 + 256 steps, then
+    every --learned-stride steps; a structure stops at a checkpoint when the
+    modes predicted from the window before the last 256 steps reproduce
+    those steps to 1e-4 of the window energy, and otherwise continues).
+    Reports wall times, the accepted count and max |dS11| over 1-20 GHz
     against the plain runs."""
     import numpy as np
 
@@ -441,7 +444,9 @@ def bench_learned(args):
     from openems_amd.extrapolator import Extrapolator, ExtrapolatorDims
 
     window, dec, check, T = 2048, 8, 256, args.learned_max_steps
-    tail = (T - window) // dec
+    tail_steps = 4096
+    offsets = list(range(0, T - window - tail_steps + 1, 512))
+    tail = tail_steps // dec
     rng_tr = np.random.default_rng(5)
     rng_te = np.random.default_rng(6)
     train_s = [W.microstrip(rng_tr) for _ in range(args.learned_train)]
@@ -449,8 +454,8 @@ def bench_learned(args):
     X, Y = [], []
     t_data = time.perf_counter()
     for b0 in range(0, len(train_s), 256):
-        run = LS.run_sweep(train_s[b0:b0 + 256], T, 50.0, record_steps=T)
-        x, y = LS.dataset(run, window, dec, tail)
+        run = LS.run_sweep(train_s[b0:b0 + 256], T, None, record_steps=T)
+        x, y = LS.dataset(run, window, dec, tail, offsets)
         X.append(x)
         Y.append(y)
         log(f"[bench] learned: training data {b0 + len(run.u)}/{len(train_s)}, sweep {run.wall:.1f} s")
@@ -463,9 +468,11 @@ def bench_learned(args):
     losses = LS.train(model, X, Y, args.learned_epochs, 1e-3, log=lambda m: log(f"[bench] learned: {m}"))
     t_train = time.perf_counter() - t_train
     plain = LS.run_sweep(test_s, T, 50.0, record_steps=0)
-    learned = LS.learned_sweep(model, test_s, T, window, dec, check)
+    learned = LS.learned_sweep(model, test_s, T, window, dec, check, stride=args.learned_stride)
     ds11 = np.abs(learned.s11 - plain.s11).max(axis=1)
     ok = ds11 <= 1e-3
+    log(f"[bench] learned: checkpoints {learned.checkpoints}; accepted {int(learned.accepted.sum())}; "
+        f"steps plain {int(plain.steps.sum())} learned {int(learned.steps_learned.sum())}")
     out = {
         "metric": "sweep time-to-|dS11|<=1e-3, learned-stop vs plain",
         "value": round(learned.wall_learned / plain.wall, 4),
@@ -479,15 +486,19 @@ def bench_learned(args):
         "vs_baseline": None,
         "dtype": "f32 fields, bf16 network (fp32 accumulate)",
         "data": "synthetic",
-        "config": {"workload": f"microstrip sweep, {len(train_s)} training runs (seed 5), {len(test_s)} held-out "
-                               f"(seed 6), max {T} steps, window {window}, check {check}, decimation {dec}",
+        "config": {"workload": f"microstrip sweep, {len(train_s)} training runs (seed 5) x {len(offsets)} window "
+                               f"offsets, {len(test_s)} held-out (seed 6), max {T} steps, window {window}, "
+                               f"check {check}, checkpoint stride {args.learned_stride}, decimation {dec}",
                    "params": int(model.n_params)},
         "plain_wall_s": round(plain.wall, 3),
         "learned_wall_s": round(learned.wall_learned, 3),
-        "fallback_wall_s": round(learned.wall_fallback, 3),
+        "inference_wall_s": round(learned.wall_inference, 3),
+        "steps_plain": int(plain.steps.sum()),
+        "steps_learned": int(learned.steps_learned.sum()),
         "accepted": int(learned.accepted.sum()),
         "max_dS11_accepted": float(ds11[learned.accepted].max()) if learned.accepted.any() else None,
         "within_1e-3": int(ok.sum()),
+        "continued_equal_plain": bool(np.array_equal(learned.s11[~learned.accepted], plain.s11[~learned.accepted])),
         "train_loss_first_last": [losses[0], losses[-1]],
         "data_s": round(t_data, 1),
         "train_s": round(t_train, 1),
@@ -525,6 +536,7 @@ def main():
     ap.add_argument("--learned-test", type=int, default=64)
     ap.add_argument("--learned-max-steps", type=int, default=12000)
     ap.add_argument("--learned-epochs", type=int, default=40)
+    ap.add_argument("--learned-stride", type=int, default=1024)
     ap.add_argument("--nf2ff", action="store_true",
                     help="time the NF2FF transform of the 384x384x256 workload's box instead (steps = repetitions)")
     ap.add_argument("--copy-calibration", action="store_true",

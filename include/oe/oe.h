@@ -476,12 +476,17 @@ oe_status oe_ext_nf2ff_calculate(oe_extension* ext, size_t freq_index, const dou
  * calls.  Every half-step of all active structures is one launch per kernel
  * variant; the steps between two energy checks are captured as one HIP graph
  * and replayed.  Each structure stops on its own: energy is sampled after
- * every step whose index is a multiple of check_interval (0 included), the
- * peak is the whole-run maximum of those samples, and the structure stops
- * when the energy falls energy_decay_db below it (the rule of
- * src/fdtd/basic_engine.rs:128-141 with a persistent peak) or at max_steps.
- * Structures share dt.  The engines and extensions must outlive the sweep's
- * use of them; the engines must not be used concurrently. */
+ * every step whose timestep (the engine's count, which continues across
+ * runs, as a single engine's does) is a multiple of check_interval (0
+ * included); the peak is the maximum of those samples over all of the
+ * engine's sweep runs and persistent-peak batches (it is kept in the engine),
+ * and the structure stops when the energy falls energy_decay_db below it
+ * (the rule of src/fdtd/basic_engine.rs:128-141 with a persistent peak) or
+ * after max_steps steps of this run.  So a run of K steps followed by a run
+ * to the end gives the same fields, spectra and stopping step as one run.
+ * Structures share dt and, at the start of a run, the timestep.  The engines
+ * and extensions must outlive the sweep's use of them; the engines must not
+ * be used concurrently. */
 typedef struct oe_sweep oe_sweep;
 typedef struct {
     uint64_t max_steps;
@@ -503,8 +508,8 @@ typedef struct {
 oe_status oe_sweep_create(int device, oe_sweep** out);
 void oe_sweep_destroy(oe_sweep* sw);
 /* Adds a structure: its engine, CPML (or null), port DFT (or null; at most
- * 256 signals, every step) and excitations (sample s is applied at sweep
- * step s).  *index receives its position. */
+ * 256 signals, every step) and excitations (sample s is applied at step s of
+ * each run, counted from the run's start).  *index receives its position. */
 oe_status oe_sweep_add(oe_sweep* sw, oe_engine* eng, oe_extension* cpml, oe_extension* port,
                        const oe_excitation* excitations, size_t n_excitations, size_t* index);
 /* Runs every structure to its end; results has one entry per structure. */

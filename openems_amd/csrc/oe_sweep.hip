@@ -8,8 +8,10 @@
 // work-groups of all structures share the grid through a box table), the
 // excitations, port DFTs and the step counter follow as one launch each, and
 // the step sequence between two termination checks is captured once as a
-// HIP graph and replayed.  Each structure stops on its own energy decay
-// (whole-run peak) or at max_steps; stopped structures leave the tables.
+// HIP graph and replayed.  Each structure stops on its own energy decay or at
+// max_steps; stopped structures leave the tables.  Checks fall on the
+// engines' absolute timesteps and the peak is kept in the engine, so a run
+// can continue where an earlier one stopped with the same result as one run.
 
 #include "oe_objects.h"
 
@@ -344,7 +346,13 @@ oe_status oe_sweep_run(oe_sweep* sw, const oe_sweep_config* cfg, oe_sweep_entry*
     const int n = (int)sw->items.size();
     const uint64_t check = cfg->check_interval ? cfg->check_interval : 100;
     const double dt = sw->items[0].eng->dt;
-    for (auto& it : sw->items) HIP_OK(hipStreamSynchronize(it.eng->stream));
+    const uint64_t T0 = sw->items[0].eng->timestep;  // timestep of the run's first step
+    for (auto& it : sw->items) {
+        if (it.eng->timestep != T0)
+            return fail(OE_ERR_CONFIG, "structures of a sweep run start at one timestep (%llu and %llu)",
+                        (unsigned long long)T0, (unsigned long long)it.eng->timestep);
+        HIP_OK(hipStreamSynchronize(it.eng->stream));
+    }
 
     // static device data: structures, excitation samples
     std::vector<SweepStruct> st(n);
@@ -393,6 +401,7 @@ oe_status oe_sweep_run(oe_sweep* sw, const oe_sweep_config* cfg, oe_sweep_entry*
     std::vector<int> active(n);
     for (int s = 0; s < n; ++s) active[s] = s;
     std::vector<double> peak(n, 0.0);
+    for (int s = 0; s < n; ++s) peak[s] = sw->items[s].eng->peak_persistent;
     for (int s = 0; s < n; ++s) {
         results[s] = oe_sweep_entry{};
         results[s].termination_reason = OE_TERM_STEPS_COMPLETED;
@@ -447,12 +456,13 @@ oe_status oe_sweep_run(oe_sweep* sw, const oe_sweep_config* cfg, oe_sweep_entry*
     bool dirty = false;  // tables changed since the last capture
     sw->drop_graph();
     while (!active.empty() && done < cfg->max_steps) {
-        // Energy is sampled after the steps whose index is a multiple of
+        // Energy is sampled after the steps whose timestep is a multiple of
         // check (0, check, 2 check, ...), as a single engine with
-        // sample_interval = check_interval does: step 0 alone, then chunks
-        // of check steps.
-        const uint64_t next = done == 0 ? 1 : done + check;
-        const uint64_t chunk = std::min<uint64_t>(next - done, cfg->max_steps - done);
+        // sample_interval = check_interval does: from timestep 0, step 0
+        // alone, then chunks of check steps.
+        const uint64_t ts = T0 + done;                            // timestep of the next step
+        const uint64_t next_check = (ts + check - 1) / check * check;  // first check at or after it
+        const uint64_t chunk = std::min<uint64_t>(next_check - ts + 1, cfg->max_steps - done);
         if (cfg->use_graph && chunk == check && done > 0) {
             if (!sw->exec || dirty || sw->graph_steps != (int)chunk) {
                 OE_TRY(capture(sw, tab, dt, (int)chunk));
@@ -463,7 +473,7 @@ oe_status oe_sweep_run(oe_sweep* sw, const oe_sweep_config* cfg, oe_sweep_entry*
             for (uint64_t q = 0; q < chunk; ++q) HIP_OK(sweep_step(sw, tab, dt));
         }
         done += chunk;
-        const uint64_t cur = done - 1;  // index of the last step run
+        const uint64_t cur = T0 + done - 1;  // timestep of the last step run
         if (cur % check != 0) continue;  // max_steps reached between checks
         OE_TRY(energies(sw, (int)active.size()));
         std::vector<int> keep;
@@ -474,7 +484,7 @@ oe_status oe_sweep_run(oe_sweep* sw, const oe_sweep_config* cfg, oe_sweep_entry*
             results[s].final_energy = E;
             results[s].peak_energy = peak[s];
             bool stop = false;
-            // src/fdtd/basic_engine.rs:128-141, with the whole-run peak
+            // src/fdtd/basic_engine.rs:128-141, with the engine's persistent peak
             if (cfg->has_energy_decay_db && cur > 0 && peak[s] > 0.0 && E > 0.0) {
                 const double ddb = 10.0 * std::log10(E / peak[s]);
                 if (ddb < -std::fabs(cfg->energy_decay_db)) {
@@ -501,6 +511,7 @@ oe_status oe_sweep_run(oe_sweep* sw, const oe_sweep_config* cfg, oe_sweep_entry*
     for (int s = 0; s < n; ++s) {
         Item& it = sw->items[s];
         it.eng->timestep += results[s].timesteps_executed;
+        it.eng->peak_persistent = peak[s];
         if (it.port) {
             it.port->dft_count += results[s].timesteps_executed;
             it.port->dft_seen += results[s].timesteps_executed;

@@ -3,14 +3,18 @@ signals with the extrapolator's modes instead of running to -50 dB.
 
   plain run    every structure runs until its energy is 50 dB below its
                whole-run peak (or max_steps); S11 from the port's running DFT
-  learned run  every structure runs `window + check` steps; the network maps
-               the decimated first `window` steps of V and I to damped modes;
-               the modes' prediction over the next `check` steps is compared
-               with the simulated samples there: if the squared error exceeds
-               `max_residual` times the energy of the window, the structure
-               falls back to the plain run.  Otherwise its S11 combines the
-               simulated DFT over [0, window + check) with the closed-form DFT
-               of the predicted modes from window + check to max_steps.
+  learned run  all structures run together to a checkpoint P (first
+               window + check, then every `stride` steps).  At P the network
+               maps the decimated V and I of [P - check - window, P - check)
+               to damped modes; their prediction over [P - check, P) is
+               compared with the simulated samples there: if the squared
+               error is at most `max_residual` times the energy of the window,
+               the structure stops and its S11 combines the simulated DFT
+               over [0, P) with the closed-form DFT of the predicted modes
+               from P to max_steps.  The others continue from their state at
+               P (same fields, DFT sums, energy peak and decay-check steps as
+               the plain run), so a structure that is never accepted ends
+               exactly as in the plain run, with the same S11.
 
 The DFT of a mode sum is a geometric series per mode (U sampled at (k + 1)
 dt, I at (k + 1/2) dt, as the engine samples E and H), so no tail samples
@@ -26,7 +30,7 @@ from typing import List, Optional
 import numpy as np
 
 from . import workloads as W
-from .engine import Sweep
+from .engine import ScheduledExcitation, Sweep
 from .extrapolator import Extrapolator, ExtrapolatorDims, training_pair
 
 A_SCALE = 0.05
@@ -45,28 +49,34 @@ class SweepRun:
     freqs: np.ndarray
 
 
+def shifted(exc, k: int):
+    """Excitations of a run that starts k steps into the schedule."""
+    return [ScheduledExcitation(e.position, e.direction, e.samples[k:], e.soft_source) for e in exc]
+
+
+def build(structs: List[W.Microstrip], max_steps: int, record_steps: int, freqs):
+    return [W.build_microstrip_hip(ms, max_steps, record_steps=record_steps, freqs=freqs) for ms in structs]
+
+
 def run_sweep(structs: List[W.Microstrip], max_steps: int, decay_db: Optional[float], record_steps: int,
               freqs=None, check: int = 100, autotune: bool = True) -> SweepRun:
     fr = np.linspace(1e9, 20e9, 64) if freqs is None else np.asarray(freqs)
+    built = build(structs, max_steps, record_steps, fr)
     sw = Sweep()
-    keep = []
-    dt = None
-    for ms in structs:
-        op, eng, cp, port, exc = W.build_microstrip_hip(ms, max_steps, record_steps=record_steps, freqs=fr)
+    for op, eng, cp, port, exc in built:
         sw.add(eng, cp, port, exc)
-        keep.append((eng, port))
-        dt = op.dt
     t0 = time.perf_counter()
     res = sw.run(max_steps, energy_decay_db=decay_db, check_interval=check, autotune=autotune)
     wall = time.perf_counter() - t0
     u, i, s11, dft = [], [], [], []
-    for eng, port in keep:
+    for op, eng, cp, port, exc in built:
         ser = port.series()
         u.append(ser[:, 0].copy())
         i.append(ser[:, 1].copy())
         s11.append(port.s11(50.0))
         dft.append(port.dft_values())
-    return SweepRun(np.array([r.timesteps_executed for r in res]), u, i, np.array(s11), np.array(dft), wall, dt, fr)
+    return SweepRun(np.array([r.timesteps_executed for r in res]), u, i, np.array(s11), np.array(dft), wall,
+                    built[0][0].dt, fr)
 
 
 def mode_params(o: np.ndarray, n_modes: int):
@@ -105,67 +115,118 @@ def calc_s11(U, I, z_ref=50.0):
 class LearnedResult:
     s11: np.ndarray
     accepted: np.ndarray            # bool per structure
-    residual: np.ndarray            # validation error / window energy
-    wall_learned: float             # early sweep + inference + fallback runs
-    wall_fallback: float
-    steps_learned: np.ndarray
+    residual: np.ndarray            # validation error / window energy at the last checkpoint seen
+    wall_learned: float             # all sweep segments + inference
+    wall_inference: float
+    steps_learned: np.ndarray       # steps each structure ran
+    checkpoints: List[int] = field(default_factory=list)
+
+
+def _window(u: np.ndarray, i: np.ndarray, a: int, window: int, decimate: int, n_chk: int):
+    """Network input [2, window/decimate] of V, I over [a, a + window), the
+    next n_chk decimated samples (the check), and the window peaks."""
+    L0 = window // decimate
+    sv = max(np.abs(u[a:a + window]).max(), 1e-30)
+    si = max(np.abs(i[a:a + window]).max(), 1e-30)
+    x = np.stack([u[a:a + window:decimate][:L0] / sv, i[a:a + window:decimate][:L0] / si])
+    b = a + window
+    chk = np.stack([u[b:b + n_chk * decimate:decimate] / sv, i[b:b + n_chk * decimate:decimate] / si])
+    return x.astype(np.float32), chk, (sv, si)
 
 
 def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, decimate: int, check: int,
-                  max_residual: float = 1e-4, freqs=None, decay_db: float = 50.0) -> LearnedResult:
+                  max_residual: float = 1e-4, freqs=None, decay_db: float = 50.0, stride: Optional[int] = None,
+                  decay_check: int = 100) -> LearnedResult:
     d = model.dims
     early = window + check
-    t0 = time.perf_counter()
-    run = run_sweep(structs, early, None, record_steps=early, freqs=freqs)
-    X = np.zeros((len(structs), 2, d.L0), np.float32)
-    scales = []
-    for q in range(len(structs)):
-        x, _, sc = training_pair(run.u[q], run.i[q], window, decimate, 1)
-        X[q] = x
-        scales.append(sc)
-    modes = np.zeros((len(structs), 6 * d.modes), np.float32)
-    for b0 in range(0, len(structs), d.batch):
-        m, _ = model.forward(X[b0:b0 + d.batch], tails=False)
-        modes[b0:b0 + d.batch] = m
-    s, aV, aI = mode_params(modes, d.modes)
+    stride = stride or window
+    fr = np.linspace(1e9, 20e9, 64) if freqs is None else np.asarray(freqs)
+    built = build(structs, max_steps, max_steps, fr)
+    dt = built[0][0].dt
+    n = len(structs)
+    s11 = np.zeros((n, len(fr)), np.complex128)
+    accepted = np.zeros(n, bool)
+    resid = np.full(n, np.nan)
+    steps = np.zeros(n, np.int64)
     n_chk = check // decimate
-    n = np.arange(n_chk)
-    s11 = run.s11.copy()
-    accepted = np.zeros(len(structs), bool)
-    resid = np.zeros(len(structs))
-    for q in range(len(structs)):
-        sv, si = scales[q]
-        z = np.exp(s[q][:, None] * n)
-        pv = np.real((aV[q][:, None] * z).sum(0))
-        pi = np.real((aI[q][:, None] * z).sum(0))
-        av = run.u[q][window:early:decimate][:n_chk] / sv
-        ai = run.i[q][window:early:decimate][:n_chk] / si
-        e_win = (run.u[q][:window:decimate] / sv) ** 2 + (run.i[q][:window:decimate] / si) ** 2
-        resid[q] = float(((pv - av) ** 2 + (pi - ai) ** 2).sum() / max(e_win.sum(), 1e-30))
-        if resid[q] <= max_residual:
-            accepted[q] = True
-            U = run.dft[q, :, 0] + sv * tail_dft(s[q], aV[q], window, early, max_steps, decimate, run.dt, 1.0, run.freqs)
-            I = run.dft[q, :, 1] + si * tail_dft(s[q], aI[q], window, early, max_steps, decimate, run.dt, 0.5, run.freqs)
-            s11[q] = calc_s11(U, I)
-    wall_early = time.perf_counter() - t0
-    t1 = time.perf_counter()
-    steps = np.full(len(structs), early)
-    rej = [q for q in range(len(structs)) if not accepted[q]]
-    if rej:
-        full = run_sweep([structs[q] for q in rej], max_steps, decay_db, record_steps=0, freqs=freqs)
-        for k, q in enumerate(rej):
-            s11[q] = full.s11[k]
-            steps[q] = full.steps[k]
-    wall_fb = time.perf_counter() - t1
-    return LearnedResult(s11, accepted, resid, wall_early + wall_fb, wall_fb, steps)
+    live = list(range(n))
+    pos = 0
+    ci = None
+    checkpoints = []
+    wall = 0.0
+    t_inf = 0.0
+    while live:
+        target = min(max_steps, early if pos < early else pos + stride)
+        sw = Sweep()
+        for q in live:
+            op, eng, cp, port, exc = built[q]
+            sw.add(eng, cp, port, shifted(exc, pos))
+        if ci is not None:
+            sw.set_ci(ci)
+        t0 = time.perf_counter()
+        res = sw.run(target - pos, energy_decay_db=decay_db, check_interval=decay_check, autotune=ci is None)
+        wall += time.perf_counter() - t0
+        if ci is None:
+            ci = sw.tuning()["ci"]
+        pos = target
+        still = []
+        for k, q in enumerate(live):
+            steps[q] += res[k].timesteps_executed
+            if res[k].termination_reason != 0 or pos >= max_steps:
+                s11[q] = built[q][3].s11(50.0)
+            else:
+                still.append(q)
+        if not still:
+            break
+        checkpoints.append(pos)
+        t1 = time.perf_counter()
+        X = np.zeros((len(still), 2, d.L0), np.float32)
+        chk, scales, dfts = [], [], []
+        for k, q in enumerate(still):
+            port = built[q][3]
+            ser = port.series()
+            X[k], c, sc = _window(ser[:, 0], ser[:, 1], pos - early, window, decimate, n_chk)
+            chk.append(c)
+            scales.append(sc)
+            dfts.append(port.dft_values())
+        modes = np.zeros((len(still), 6 * d.modes), np.float32)
+        for b0 in range(0, len(still), d.batch):
+            m, _ = model.forward(X[b0:b0 + d.batch], tails=False)
+            modes[b0:b0 + d.batch] = m
+        s, aV, aI = mode_params(modes, d.modes)
+        z_n = np.arange(n_chk)
+        rejected = []
+        for k, q in enumerate(still):
+            sv, si = scales[k]
+            z = np.exp(s[k][:, None] * z_n)
+            pv = np.real((aV[k][:, None] * z).sum(0))
+            pi = np.real((aI[k][:, None] * z).sum(0))
+            e_win = (X[k].astype(np.float64) ** 2).sum()
+            resid[q] = float(((pv - chk[k][0]) ** 2 + (pi - chk[k][1]) ** 2).sum() / max(e_win, 1e-30))
+            if resid[q] <= max_residual:
+                accepted[q] = True
+                a = pos - check
+                U = dfts[k][:, 0] + sv * tail_dft(s[k], aV[k], a, pos, max_steps, decimate, dt, 1.0, fr)
+                I = dfts[k][:, 1] + si * tail_dft(s[k], aI[k], a, pos, max_steps, decimate, dt, 0.5, fr)
+                s11[q] = calc_s11(U, I)
+            else:
+                rejected.append(q)
+        t_inf += time.perf_counter() - t1
+        live = rejected
+    return LearnedResult(s11, accepted, resid, wall + t_inf, t_inf, steps, checkpoints)
 
 
-def dataset(run: SweepRun, window: int, decimate: int, tail: int):
+def dataset(run: SweepRun, window: int, decimate: int, tail: int, offsets=(0,)):
+    """Training pairs from recorded runs: for every offset o, the decimated
+    window [o, o + window) and the next `tail` decimated samples (past the
+    end of a record they count as zero, so records should cover
+    max(offsets) + window + tail * decimate steps)."""
     X, T = [], []
     for q in range(len(run.u)):
-        x, t, _ = training_pair(run.u[q], run.i[q], window, decimate, tail)
-        X.append(x)
-        T.append(t)
+        for o in offsets:
+            x, t, _ = training_pair(run.u[q][o:], run.i[q][o:], window, decimate, tail)
+            X.append(x)
+            T.append(t)
     return np.array(X), np.array(T)
 
 

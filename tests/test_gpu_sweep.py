@@ -95,3 +95,61 @@ def test_sweep_structure_against_oracle():
     E, H = eng.read_fields()
     assert np.array_equal(np.concatenate([E, H]), sim.fields())
     assert np.array_equal(port.series(), sim.signal_series(0, 2))
+
+
+def _shifted(exc, k):
+    from openems_amd import ScheduledExcitation
+    return [ScheduledExcitation(e.position, e.direction, e.samples[k:], e.soft_source) for e in exc]
+
+
+@pytest.mark.parametrize("first", [173, 200])
+def test_sweep_continued_equals_one_run(first):
+    """A run of `first` steps, then a second sweep over the same engines
+    (excitations shifted by the steps already run) to the end: decay checks
+    fall on the same timesteps and the peak carries over, so steps, reason,
+    fields, port series and spectra equal those of one run."""
+    structs = _structures()
+    _, keep1, res1 = _sweep(structs, True, autotune=False)
+    sw = Sweep()
+    built = []
+    for ms in structs:
+        op, eng, cp, port, exc = W.build_microstrip_hip(ms, MAX_STEPS, record_steps=MAX_STEPS)
+        sw.add(eng, cp, port, exc)
+        built.append((eng, cp, port, exc))
+    ra = sw.run(first, energy_decay_db=DECAY, check_interval=CHECK, autotune=False)
+    cont = [q for q in range(len(structs)) if ra[q].termination_reason == 0]
+    sw2 = Sweep()
+    for q in cont:
+        eng, cp, port, exc = built[q]
+        sw2.add(eng, cp, port, _shifted(exc, first))
+    rb = sw2.run(MAX_STEPS - first, energy_decay_db=DECAY, check_interval=CHECK, autotune=False)
+    total = [r.timesteps_executed for r in ra]
+    reason = [r.termination_reason for r in ra]
+    for k, q in enumerate(cont):
+        total[q] += rb[k].timesteps_executed
+        reason[q] = rb[k].termination_reason
+    for q in range(len(structs)):
+        eng1, port1 = keep1[q]
+        eng2, _, port2, _ = built[q]
+        say(f"[sweep] continued at {first}: {structs[q].shape} {total[q]} steps (reason {reason[q]}), "
+            f"one run {res1[q].timesteps_executed} (reason {res1[q].termination_reason})")
+        assert total[q] == res1[q].timesteps_executed and reason[q] == res1[q].termination_reason
+        assert eng2.timestep == eng1.timestep
+        assert np.array_equal(np.concatenate(eng1.read_fields()), np.concatenate(eng2.read_fields()))
+        assert np.array_equal(port1.series(), port2.series())
+        assert np.array_equal(port1.dft_values(), port2.dft_values())
+    assert cont, "every structure stopped in the first run"
+
+
+def test_sweep_rejects_mixed_timesteps():
+    structs = _structures()[:2]
+    built = [W.build_microstrip_hip(ms, 50) for ms in structs]
+    sw = Sweep()
+    _, eng, cp, port, exc = built[0]
+    sw.add(eng, cp, port, exc)
+    sw.run(3, check_interval=CHECK, autotune=False)
+    sw2 = Sweep()
+    for _, eng, cp, port, exc in built:
+        sw2.add(eng, cp, port, exc)
+    with pytest.raises(Exception, match="one timestep"):
+        sw2.run(5, check_interval=CHECK, autotune=False)
