@@ -430,11 +430,15 @@ def bench_learned(args):
     """Learned-stop vs plain microstrip sweeps.  Training data: runs of
     --learned-train structures (seed 5) recorded for --learned-max-steps
     steps; each gives pairs (2048-step window, next 4096 steps) at offsets
-    0, 512, ..., 5632.  Held-out: --learned-test structures (seed 6) run both
+    0, 512, ..., 5632; training starts from spread mode frequencies and
+    grows the tail over the epochs.  The network's tail error relative to
+    predicting zero is reported on held-out windows of the plain runs.  Held-out: --learned-test structures (seed 6) run both
     plain (to -50 dB) and learned (checkpoints at 2048 + 256 steps, then
     every --learned-stride steps; a structure stops at a checkpoint when the
     modes predicted from the window before the last 256 steps reproduce
-    those steps to 1e-4 of the window energy, and otherwise continues).
+    those steps to 1e-4 of the window energy and predict at most 1e-4 of
+    the energy recorded so far after the checkpoint, and otherwise
+    continues).
     Reports wall times, the accepted count and max |dS11| over 1-20 GHz
     against the plain runs."""
     import numpy as np
@@ -464,10 +468,23 @@ def bench_learned(args):
     t_data = time.perf_counter() - t_data
     dims = ExtrapolatorDims(L0=window // dec, batch=min(256, len(X)), tail=tail)
     model = Extrapolator(dims, seed=0)
+    LS.spread_modes(model)
     t_train = time.perf_counter()
     losses = LS.train(model, X, Y, args.learned_epochs, 1e-3, log=lambda m: log(f"[bench] learned: {m}"))
     t_train = time.perf_counter() - t_train
-    plain = LS.run_sweep(test_s, T, 50.0, record_steps=0)
+    plain = LS.run_sweep(test_s, T, 50.0, record_steps=T)
+    # held-out windows: offsets whose tail lies inside the plain run
+    Xh, Yh = [], []
+    for q in range(len(test_s)):
+        for o in offsets:
+            if o + window + tail_steps <= plain.steps[q]:
+                x, y, _ = LS.training_pair(plain.u[q][o:], plain.i[q][o:], window, dec, tail)
+                Xh.append(x)
+                Yh.append(y)
+    gain_ho = LS.tail_error(model, np.array(Xh), np.array(Yh)) if Xh else None
+    gain_tr = LS.tail_error(model, X[:len(Xh) or 256], Y[:len(Xh) or 256])
+    log(f"[bench] learned: tail error / zero-prediction error, held-out {gain_ho} ({len(Xh)} windows), "
+        f"training {gain_tr}")
     learned = LS.learned_sweep(model, test_s, T, window, dec, check, stride=args.learned_stride)
     ds11 = np.abs(learned.s11 - plain.s11).max(axis=1)
     ok = ds11 <= 1e-3
@@ -496,10 +513,14 @@ def bench_learned(args):
         "steps_plain": int(plain.steps.sum()),
         "steps_learned": int(learned.steps_learned.sum()),
         "accepted": int(learned.accepted.sum()),
+        "acceptance": "check residual <= 1e-4 of window energy and predicted tail energy <= 1e-4 of energy so far",
         "max_dS11_accepted": float(ds11[learned.accepted].max()) if learned.accepted.any() else None,
         "within_1e-3": int(ok.sum()),
         "continued_equal_plain": bool(np.array_equal(learned.s11[~learned.accepted], plain.s11[~learned.accepted])),
         "train_loss_first_last": [losses[0], losses[-1]],
+        "tail_error_vs_zero_heldout": gain_ho,
+        "tail_error_vs_zero_training": gain_tr,
+        "heldout_windows": len(Xh),
         "data_s": round(t_data, 1),
         "train_s": round(t_train, 1),
         "roofline": None,
@@ -532,7 +553,7 @@ def main():
     ap.add_argument("--composite", action="store_true", help="run the 384x384x256 composite workload instead")
     ap.add_argument("--nf2ff-stride", type=int, default=4)
     ap.add_argument("--learned", action="store_true", help="learned-stop vs plain sweep (trains the extrapolator)")
-    ap.add_argument("--learned-train", type=int, default=512)
+    ap.add_argument("--learned-train", type=int, default=1024)
     ap.add_argument("--learned-test", type=int, default=64)
     ap.add_argument("--learned-max-steps", type=int, default=12000)
     ap.add_argument("--learned-epochs", type=int, default=40)

@@ -8,8 +8,10 @@ signals with the extrapolator's modes instead of running to -50 dB.
                maps the decimated V and I of [P - check - window, P - check)
                to damped modes; their prediction over [P - check, P) is
                compared with the simulated samples there: if the squared
-               error is at most `max_residual` times the energy of the window,
-               the structure stops and its S11 combines the simulated DFT
+               error is at most `max_residual` times the energy of the window
+               and the energy the modes predict after P is at most
+               `max_tail_energy` of the energy recorded so far, the
+               structure stops and its S11 combines the simulated DFT
                over [0, P) with the closed-form DFT of the predicted modes
                from P to max_steps.  The others continue from their state at
                P (same fields, DFT sums, energy peak and decay-check steps as
@@ -24,7 +26,7 @@ from __future__ import annotations
 
 import math
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 from typing import List, Optional
 
 import numpy as np
@@ -120,6 +122,7 @@ class LearnedResult:
     wall_inference: float
     steps_learned: np.ndarray       # steps each structure ran
     checkpoints: List[int] = field(default_factory=list)
+    tail_energy: Optional[np.ndarray] = None  # predicted tail energy / energy so far, last checkpoint seen
 
 
 def _window(u: np.ndarray, i: np.ndarray, a: int, window: int, decimate: int, n_chk: int):
@@ -136,7 +139,12 @@ def _window(u: np.ndarray, i: np.ndarray, a: int, window: int, decimate: int, n_
 
 def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, decimate: int, check: int,
                   max_residual: float = 1e-4, freqs=None, decay_db: float = 50.0, stride: Optional[int] = None,
-                  decay_check: int = 100) -> LearnedResult:
+                  decay_check: int = 100, max_tail_energy: float = 1e-4) -> LearnedResult:
+    """A structure stops at a checkpoint when (1) the modes reproduce the
+    check steps to max_residual of the window energy and (2) the energy the
+    modes predict from the checkpoint to max_steps is at most
+    max_tail_energy of the energy recorded so far (both V and I, decimated,
+    in units of the window peaks)."""
     d = model.dims
     early = window + check
     stride = stride or window
@@ -147,6 +155,7 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
     s11 = np.zeros((n, len(fr)), np.complex128)
     accepted = np.zeros(n, bool)
     resid = np.full(n, np.nan)
+    tail_frac = np.full(n, np.nan)
     steps = np.zeros(n, np.int64)
     n_chk = check // decimate
     live = list(range(n))
@@ -181,13 +190,14 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
         checkpoints.append(pos)
         t1 = time.perf_counter()
         X = np.zeros((len(still), 2, d.L0), np.float32)
-        chk, scales, dfts = [], [], []
+        chk, scales, dfts, sers = [], [], [], []
         for k, q in enumerate(still):
             port = built[q][3]
             ser = port.series()
             X[k], c, sc = _window(ser[:, 0], ser[:, 1], pos - early, window, decimate, n_chk)
             chk.append(c)
             scales.append(sc)
+            sers.append(ser)
             dfts.append(port.dft_values())
         modes = np.zeros((len(still), 6 * d.modes), np.float32)
         for b0 in range(0, len(still), d.batch):
@@ -195,6 +205,7 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
             modes[b0:b0 + d.batch] = m
         s, aV, aI = mode_params(modes, d.modes)
         z_n = np.arange(n_chk)
+        z_t = np.arange(n_chk, n_chk + (max_steps - pos) // decimate)
         rejected = []
         for k, q in enumerate(still):
             sv, si = scales[k]
@@ -203,7 +214,13 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
             pi = np.real((aI[k][:, None] * z).sum(0))
             e_win = (X[k].astype(np.float64) ** 2).sum()
             resid[q] = float(((pv - chk[k][0]) ** 2 + (pi - chk[k][1]) ** 2).sum() / max(e_win, 1e-30))
-            if resid[q] <= max_residual:
+            zt = np.exp(s[k][:, None] * z_t)
+            e_tail = float((np.real((aV[k][:, None] * zt).sum(0)) ** 2).sum()
+                           + (np.real((aI[k][:, None] * zt).sum(0)) ** 2).sum())
+            e_run = float((sers[k][:pos:decimate, 0] / sv) @ (sers[k][:pos:decimate, 0] / sv)
+                          + (sers[k][:pos:decimate, 1] / si) @ (sers[k][:pos:decimate, 1] / si))
+            tail_frac[q] = e_tail / max(e_run, 1e-300)
+            if resid[q] <= max_residual and tail_frac[q] <= max_tail_energy:
                 accepted[q] = True
                 a = pos - check
                 U = dfts[k][:, 0] + sv * tail_dft(s[k], aV[k], a, pos, max_steps, decimate, dt, 1.0, fr)
@@ -213,7 +230,7 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
                 rejected.append(q)
         t_inf += time.perf_counter() - t1
         live = rejected
-    return LearnedResult(s11, accepted, resid, wall + t_inf, t_inf, steps, checkpoints)
+    return LearnedResult(s11, accepted, resid, wall + t_inf, t_inf, steps, checkpoints, tail_frac)
 
 
 def dataset(run: SweepRun, window: int, decimate: int, tail: int, offsets=(0,)):
@@ -230,24 +247,97 @@ def dataset(run: SweepRun, window: int, decimate: int, tail: int, offsets=(0,)):
     return np.array(X), np.array(T)
 
 
+def spread_modes(model: Extrapolator, f_lo: float = 0.004, f_hi: float = 0.2, damping: float = -3.0,
+                 shrink: float = 0.1):
+    """Training start for the output layer: mode m's frequency output at
+    f_lo..f_hi cycles per decimated sample (evenly spaced; 0.2 is 26 GHz at
+    8 steps of 0.95 ps), its damping output at `damping` (softplus(-3) *
+    A_SCALE = 2.4e-3 per sample) and the layer's weights scaled by `shrink`.
+    From the default start every mode sits near F_MAX / 2 and gradient
+    descent shrinks the residues to zero rather than moving frequencies."""
+    d = model.dims
+    M = d.modes
+    p = model.params()
+    b5 = p[-6 * M:].reshape(M, 6)
+    f = np.linspace(f_lo, f_hi, M)
+    b5[:, 0] = damping
+    b5[:, 1] = np.log(f / (F_MAX - f))       # F_MAX * sigmoid(o) = f
+    p[-6 * M:] = b5.reshape(-1)
+    p[-6 * M - 6 * M * d.hidden:-6 * M] *= shrink
+    model.set_params(p)
+
+
+def tail_schedule(epochs: int, full: int, first: int = 16, share: float = 0.7) -> List[int]:
+    """Tail length per epoch: first, 2 first, 4 first, ... below `full` over
+    the first `share` of the epochs (equal parts), then the full tail.  A
+    short horizon keeps a frequency error's phase drift small, so the modes
+    find their frequencies before the long tail weights their damping."""
+    stages = []
+    n = first
+    while n < full:
+        stages.append(n)
+        n *= 2
+    if not stages:
+        return [full] * epochs
+    n_cur = int(round(share * epochs))
+    out = []
+    for ep in range(epochs):
+        out.append(stages[min(len(stages) - 1, ep * len(stages) // max(n_cur, 1))] if ep < n_cur else full)
+    return out
+
+
 def train(model: Extrapolator, X: np.ndarray, T: np.ndarray, epochs: int, lr: float, seed: int = 0,
-          log=None) -> List[float]:
+          log=None, curriculum: bool = True) -> List[float]:
+    """Adam with a cosine learning rate; with `curriculum`, the tail grows
+    over the epochs (tail_schedule) on models of the same parameters and
+    shorter tails, and the last epochs train `model` itself."""
     rng = np.random.default_rng(seed)
-    B = model.dims.batch
+    d = model.dims
+    B = d.batch
     losses = []
     steps = epochs * max(1, len(X) // B)
+    sched = tail_schedule(epochs, d.tail) if curriculum else [d.tail] * epochs
     k = 0
+    cur = model
     for ep in range(epochs):
+        n = sched[ep]
+        if n != cur.dims.tail:
+            p = cur.params()
+            cur = model if n == d.tail else Extrapolator(replace(d, tail=n), params=p, seed=seed)
+            if cur is model:
+                model.set_params(p)
+        Tn = np.ascontiguousarray(T[:, :, :n])
         perm = rng.permutation(len(X))
         tot = 0.0
         nb = 0
         for b0 in range(0, len(X) - B + 1 if len(X) >= B else 1, B):
             idx = perm[b0:b0 + B]
             lr_k = lr * 0.5 * (1 + math.cos(math.pi * k / max(steps, 1)))
-            tot += model.train_step(X[idx], T[idx], max(lr_k, lr * 1e-2))
+            tot += cur.train_step(X[idx], Tn[idx], max(lr_k, lr * 1e-2))
             nb += 1
             k += 1
         losses.append(tot / max(nb, 1))
         if log:
-            log(f"epoch {ep}: loss {losses[-1]:.4e}")
+            log(f"epoch {ep}: tail {n}, loss {losses[-1]:.4e}")
+    if cur is not model:
+        model.set_params(cur.params())
     return losses
+
+
+def tail_error(model: Extrapolator, X: np.ndarray, T: np.ndarray, horizons=(32, 128, 512)):
+    """Mean squared tail error of the network over the first n decimated
+    samples, divided by that of predicting zero (stopping without
+    extrapolating), for each n in horizons."""
+    B = model.dims.batch
+    err = np.zeros(len(horizons))
+    zero = np.zeros(len(horizons))
+    for b0 in range(0, len(X), B):
+        xb = X[b0:b0 + B]
+        nb = len(xb)
+        if nb < B:
+            xb = np.concatenate([xb, np.zeros((B - nb,) + xb.shape[1:], xb.dtype)])
+        _, tl = model.forward(xb)
+        for h, n in enumerate(horizons):
+            err[h] += float(((tl[:nb, :, :n] - T[b0:b0 + nb, :, :n]) ** 2).sum())
+            zero[h] += float((T[b0:b0 + nb, :, :n].astype(np.float64) ** 2).sum())
+    return {int(n): float(err[h] / max(zero[h], 1e-300)) for h, n in enumerate(horizons)}

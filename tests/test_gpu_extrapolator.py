@@ -78,3 +78,26 @@ def test_params_roundtrip_and_training_reduces_loss():
     say(f"[extrap] training: loss {losses[0]:.4e} -> {losses[-1]:.4e}")
     assert losses[-1] < 0.5 * losses[0]
     assert np.all(np.isfinite(ex.params()))
+
+
+def test_spread_modes_and_tail_curriculum():
+    """The output-layer start puts the modes' frequencies on the requested
+    grid, and curriculum training (growing tails on models that share the
+    parameters) ends on the caller's model with its own tail length."""
+    from openems_amd import learned_stop as LS
+    p, x, tgt = _case(4)
+    ex = Extrapolator(SMALL, params=pack(p, SMALL))
+    LS.spread_modes(ex, f_lo=0.01, f_hi=0.2, damping=-3.0, shrink=0.0)
+    modes, _ = ex.forward(x, tails=False)
+    s, _, _ = LS.mode_params(modes, SMALL.modes)
+    assert np.allclose(s.imag[0] / (2 * np.pi), np.linspace(0.01, 0.2, SMALL.modes), rtol=1e-5)
+    assert np.allclose(-s.real[0], np.logaddexp(0.0, -3.0) * LS.A_SCALE, rtol=1e-5)
+    X6 = np.concatenate([x] * 4)
+    T6 = np.concatenate([tgt] * 4)
+    before = ex.params()
+    losses = LS.train(ex, X6, T6, 12, 3e-3)
+    say(f"[extrap] curriculum {LS.tail_schedule(12, SMALL.tail)}: loss {losses[0]:.3e} -> {losses[-1]:.3e}")
+    assert ex.dims.tail == SMALL.tail and np.all(np.isfinite(losses))
+    assert not np.array_equal(ex.params(), before)
+    err = LS.tail_error(ex, X6, T6, horizons=(16, SMALL.tail))
+    assert all(np.isfinite(v) for v in err.values())

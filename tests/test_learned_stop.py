@@ -46,3 +46,14 @@ def test_calc_s11_limits():
     I = np.array([1 + 1j, 2.0])
     assert np.allclose(calc_s11(50 * I, I), 0)
     assert np.allclose(calc_s11(0 * I, I), -1)
+
+
+def test_tail_schedule():
+    from openems_amd.learned_stop import tail_schedule
+    s = tail_schedule(10, 512)
+    # 16, 32, ..., 256 over the first 7 epochs, then the full tail
+    assert s[:1] == [16] and s[7:] == [512, 512, 512]
+    assert all(a <= b for a, b in zip(s, s[1:]))
+    assert set(s[:7]) <= {16, 32, 64, 128, 256}
+    assert tail_schedule(4, 10) == [10] * 4          # shorter than the first stage
+    assert tail_schedule(3, 64, first=16, share=0.0) == [64] * 3
