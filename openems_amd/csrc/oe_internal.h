@@ -187,6 +187,7 @@ struct SweepBox {
     int i0, i1, j0, j1;
     int n_jb, n_kb;
     int bx, by, ci;
+    int ns;     // i-chunks of ci planes per work-group (boxes with fewer j rows than 256 / bx)
     int first;  // first work-group of the box
 };
 struct SweepArgs {

@@ -114,8 +114,14 @@ void structure_boxes(const oe_sweep* sw, int sid, const Item& it, std::vector<Gr
         }
         SweepBox b = proto;
         b.i0 = i0; b.i1 = i1; b.j0 = j0; b.j1 = j1;
+        // A box thinner in j than a work-group's rows (the y-face slabs and
+        // corner columns: `layers` rows) stacks several i-chunks in one
+        // work-group instead of leaving the surplus rows idle.
+        const int rows = 256 / b.bx;
+        b.by = std::min(rows, j1 - j0);
+        b.ns = rows / b.by;
         b.n_jb = ceil_div(j1 - j0, b.by);
-        const int n_ib = ceil_div(i1 - i0, b.ci);
+        const int n_ib = ceil_div(i1 - i0, b.ci * b.ns);
         b.first = G->n_wg;
         const int n = b.n_kb * b.n_jb * n_ib;
         for (int q = 0; q < n; ++q) G->wg_box.push_back((int)G->boxes.size());

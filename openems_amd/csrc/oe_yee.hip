@@ -583,8 +583,9 @@ __global__ __launch_bounds__(256) void k_update_e(YeeArgs a, CpmlArgs c) {
 }
 
 // Many grids in one launch (a parameter sweep): work-group -> box of some
-// structure through the launch's table; each structure has its own tile
-// shape (bx * by <= 256, surplus lanes exit).
+// structure through the launch's table; each box has its own tile shape
+// (bx * by * ns <= 256 lanes: ns i-chunks of by rows of bx lanes; surplus
+// lanes exit).
 template <int V>
 __device__ __forceinline__ bool decode_sweep(const SweepArgs& a, Tile& r, int& sid) {
     int bid = blockIdx.x;
@@ -599,12 +600,16 @@ __device__ __forceinline__ bool decode_sweep(const SweepArgs& a, Tile& r, int& s
     const int jb = t % B.n_jb;
     const int ib = t / B.n_jb;
     const int kx = threadIdx.x % B.bx;
-    const int jy = threadIdx.x / B.bx;
+    const int rest = threadIdx.x / B.bx;
+    const int jy = rest % B.by;
+    const int sub = rest / B.by;  // i-chunk within the work-group
     r.k0 = (kb * B.bx + kx) * V;
     r.j = B.j0 + jb * B.by + jy;
-    r.i0 = B.i0 + ib * B.ci;
+    r.i0 = B.i0 + (ib * B.ns + sub) * B.ci;
     r.i1 = min(r.i0 + B.ci, B.i1);
-    r.jend = jy < B.by ? B.j1 : r.j;  // surplus lanes get an empty row range
+    // surplus lanes (beyond ns chunks, or a chunk past the box) get an empty
+    // row range and return before any load
+    r.jend = (sub < B.ns && r.i0 < B.i1) ? B.j1 : r.j;
     sid = B.sid;
     return true;
 }
