@@ -291,6 +291,7 @@ oe_status engine_alloc(oe_engine* e, size_t nx, size_t ny, size_t nz, int device
     if (const char* v = getenv("OE_XCD_REMAP")) e->xcd_remap = atoi(v) != 0;
     if (const char* v = getenv("OE_FUSED")) e->fused_mode = std::max(0, std::min(2, atoi(v)));
     if (const char* v = getenv("OE_GRAPH_STEPS")) e->graph_steps = std::max(0, atoi(v));
+    if (const char* v = getenv("OE_OVERLAP_SLABS")) e->overlap_slabs = atoi(v) != 0;
     return OE_OK;
 }
 
@@ -331,6 +332,7 @@ bool material_table_default() {
 void engine_free(oe_engine* e) {
     if (!e) return;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->side) (void)hipStreamSynchronize(e->side);
     // Extensions may outlive their engine: detach them so that later calls on
     // them report an error instead of touching the freed engine.
     for (oe_extension* x : e->exts) x->eng = nullptr;
@@ -348,6 +350,9 @@ void engine_free(oe_engine* e) {
     if (e->partials) (void)hipFree(e->partials);
     if (e->energy_dev) (void)hipFree(e->energy_dev);
     if (e->energy_host) (void)hipHostFree(e->energy_host);
+    if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
+    if (e->ev_join) (void)hipEventDestroy(e->ev_join);
+    if (e->side) (void)hipStreamDestroy(e->side);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -1674,7 +1679,47 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
         dft0[q] = dfts[q]->dft_count;
     }
 
+    // The slab launches of a half-step write other cells than the interior
+    // launch and read only the other field (and their own psi), so they run
+    // on the side stream beside it: fork after the previous work, join
+    // before the next half-step.  Steps whose launches are timed stay on
+    // one stream.
+    bool overlap = e->overlap_slabs && !fused && !upml;
+    {
+        bool any_slab = false, any_inner = false;
+        for (const Launch& L : plan) ((L.axes & (AX_X | AX_Y)) ? any_slab : any_inner) = true;
+        overlap = overlap && any_slab && any_inner;
+    }
+    if (overlap && !e->side) {
+        HIP_OK(hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking));
+        HIP_OK(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
+    }
+    auto overlapped_half = [&](bool h, uint64_t st) -> oe_status {
+        auto one = [&](const Launch& L, hipStream_t s) -> hipError_t {
+            if (h) return launch_update_h(L.a, cp, L.axes, s);
+            if (!fold_sources) return launch_update_e(L.a, cp, L.axes, s);
+            YeeArgs a = L.a;
+            a.exc = e->exc_table.as<PointExc>();
+            a.n_exc = (int)px.size();
+            for (size_t q = 0; q < px.size(); ++q) a.exc_idx[q] = px[q].idx;
+            a.exc_step = st;
+            a.exc_step_dev = step_dev;
+            return launch_update_e(a, cp, L.axes, s);
+        };
+        HIP_OK(hipEventRecord(e->ev_fork, e->stream));
+        HIP_OK(hipStreamWaitEvent(e->side, e->ev_fork, 0));
+        for (const Launch& L : plan)
+            if (L.axes & (AX_X | AX_Y)) HIP_OK(one(L, e->side));
+        for (const Launch& L : plan)
+            if (!(L.axes & (AX_X | AX_Y))) HIP_OK(one(L, e->stream));
+        HIP_OK(hipEventRecord(e->ev_join, e->side));
+        HIP_OK(hipStreamWaitEvent(e->stream, e->ev_join, 0));
+        return OE_OK;
+    };
+
     auto enqueue_step = [&](uint64_t st, uint64_t cur) -> oe_status {
+        const bool ovl = overlap && !(e->profiling && e->prof_step);
         if (fused) {
             // slab H (old -> new buffer), the fused interior, slab E; then the
             // new buffer becomes the current one
@@ -1723,7 +1768,9 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             Timed tm(e, "upml");
             HIP_OK(launch_upml_pre(e->F, e->g, upml->upml, false, e->stream));
         }
-        if (!fused)
+        if (!fused && ovl) {
+            OE_TRY(overlapped_half(true, st));
+        } else if (!fused)
             for (const Launch& L : plan) {
                 Timed tm(e, (L.axes & (AX_X | AX_Y)) ? "update_h:slabs" : (cp ? "update_h:interior" : "update_h:grid"));
                 HIP_OK(launch_update_h(L.a, cp, L.axes, e->stream));
@@ -1745,7 +1792,9 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             Timed tm(e, "upml");
             HIP_OK(launch_upml_pre(e->F, e->g, upml->upml, true, e->stream));
         }
-        if (!fused)
+        if (!fused && ovl) {
+            OE_TRY(overlapped_half(false, st));
+        } else if (!fused)
             for (const Launch& L : plan) {
                 Timed tm(e, (L.axes & (AX_X | AX_Y)) ? "update_e:slabs" : (cp ? "update_e:interior" : "update_e:grid"));
                 if (fold_sources) {

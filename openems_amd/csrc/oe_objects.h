@@ -138,6 +138,14 @@ struct oe_engine {
     DevBuf exc_samples;
     DevBuf step_dev;       // batch-relative step read by the kernels of a step
     int graph_steps = 16;  // steps per captured graph (0: no graphs)
+    // CPML slab launches on a second stream beside the interior launch of
+    // the same half-step (forked and joined with events; created on first
+    // use): 1 on, 0 off.  Off by default: the slab work-groups only start as
+    // interior ones retire and the cross-queue join adds about 15 us per
+    // half-step (0.2815 against 0.2577 ms per step at 256^3).
+    int overlap_slabs = 0;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     DevBuf exc_table;
     DevBuf probe_table;
     // profiling: 0 off, n > 0 times every launch of every n-th step

@@ -71,3 +71,21 @@ def test_decay_termination_graph_equals_direct():
         say(f"[graph] {g}: stopped after {r.timesteps_executed} steps, reason {r.termination_reason}")
     assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
     assert np.array_equal(out[0][2], out[1][2])
+
+
+@pytest.mark.parametrize("graph", [0, 16])
+def test_slab_overlap_equals_serial(graph, monkeypatch):
+    """The CPML slab launches on the side stream (forked and joined around
+    each interior launch) give the same fields as all launches on one
+    stream, eager and replayed from graphs, and both match the oracle."""
+    out = []
+    for ovl in ("1", "0"):
+        monkeypatch.setenv("OE_OVERLAP_SLABS", ovl)
+        sc = Scenario(f"overlap_{ovl}_g{graph}", (40, 36, 48), 0.5e-3, 150, cpml=dict(layers=(6,) * 6, mode=MODE_PHYSICAL),
+                      materials=[(3.0, 1.0, 0.01, 0.0, (8, 30, 8, 28, 10, 38))],
+                      excitations=[((20, 18, 24), 2, True, gauss_ref(12e9)), ((3, 2, 20), 1, True, gauss_ref(9e9))],
+                      probes=[(25, 18, 24)], graph=graph)
+        res = run_pair(sc, batches=[31, 119])
+        assert_parity(res, sc.name)
+        out.append(np.concatenate(res["eng"].read_fields()))
+    assert np.array_equal(out[0], out[1])
