@@ -14,6 +14,12 @@ Prints one JSON line:
   roofline   the dominant kernel's algorithmic bytes per launch divided by its
              mean launch time, measured with HIP events on the engine's stream
              inside the timed region, against 8 TB/s
+  hbm_frac_step       the whole step's algorithmic bytes (this layout) / wall
+                      / 8 TB/s
+  hbm_frac_step_b120  the metric's convention: 120 B per cell-update plus
+                      32 B per CPML slab cell and direction / wall / 8 TB/s
+  stream_copy         a 1 GiB device-to-device copy measured after the timed
+                      region, and the step and kernel rates as fractions of it
   cpu_baseline  the oracle's OpenMP restatement of the reference's
              ParallelEngine (same arithmetic, AVX-512 when present) timed on
              the host cores on a bounded sample of the same workload
@@ -199,6 +205,28 @@ def run_steps(eng, exts, wl, dt, start, n, sample_interval=100):
     smp = Waveform.gaussian(t0, tau).sample(start, n, dt)
     return eng.run_batch(EngineBatch(num_steps=n, excitations=[ScheduledExcitation(wl.source, wl.source_dir, smp)],
                                      extensions=exts, sample_interval=sample_interval, track_peak=True))
+
+
+def stream_copy_gbs(nbytes=1 << 30, reps=20):
+    """Device-to-device copy of nbytes (read + write counted) on the current
+    stream, mean over reps after two warm-up copies: the streaming ceiling a
+    read/write kernel can reach on this device."""
+    import torch
+
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    b.synchronize()
+    ms = a.elapsed_time(b) / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
 def cpu_baseline(wl, budget_s=12.0, max_steps=4000):
@@ -667,7 +695,15 @@ def main():
         cpu = cpu_baseline(wl, budget_s=args.cpu_budget)
         log(f"[bench] CPU baseline {cpu['value']:.1f} MCells/s")
 
+    copy_gbs = stream_copy_gbs()
+    log(f"[bench] stream copy (1 GiB device-to-device): {copy_gbs:.1f} GB/s")
     mcells = wl.cells * args.steps / wall / 1e6
+    # the metric's own convention: 120 B per cell-update (the reference's 12
+    # coefficient arrays, two half-steps) plus 32 B per CPML slab cell and
+    # slab direction, against 8 TB/s
+    nx, ny, nz = wl.shape
+    L = wl.cpml_layers
+    conv_bytes = 120 * wl.cells + 32 * 2 * L * (ny * nz + nx * nz + nx * ny)
     step_bytes = 2 * kernel_bytes(wl.shape, wl.cpml_layers, coef)
     if "fused:interior" in kern:
         step_bytes = (launch_bytes("fused:interior", wl.shape, wl.cpml_layers, coef)
@@ -697,6 +733,10 @@ def main():
             "step_scheme": "fused" if any(k.startswith("fused:") for k in kern) else "two half-steps",
         },
         "hbm_frac_step": round(step_bytes * args.steps / wall / 1e9 / HBM_PEAK_GBS, 4),
+        "hbm_frac_step_b120": round(conv_bytes * args.steps / wall / 1e9 / HBM_PEAK_GBS, 4),
+        "stream_copy": {"gbs": round(copy_gbs, 1), "bytes": 2 << 30,
+                        "step_frac_of_copy": round(step_bytes * args.steps / wall / 1e9 / copy_gbs, 4),
+                        "kernel_frac_of_copy": round(achieved / copy_gbs, 4)},
         "roofline": {
             "bound": "hbm",
             "kernel": {"update_h:interior": "k_update_h<4,AX_Z,phys> (interior box, z-face CPML)",
