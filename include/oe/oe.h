@@ -416,8 +416,11 @@ oe_status oe_ext_port_s11(oe_extension* port, double z_ref, double* s11_re, doub
  * over the first in-plane axis runs on the FP32 matrix cores; the phase
  * factors, the second-axis reduction and P_rad are float64. */
 
-/* FarFieldResult (src/nf2ff/mod.rs:11-24) of one frequency.  The four arrays
- * are [n_theta][n_phi], caller-owned; any may be null. */
+/* FarFieldResult (src/nf2ff/mod.rs:11-24) of one frequency.  The field
+ * arrays are [n_theta][n_phi], caller-owned, and any may be null: split real
+ * and imaginary parts, or e_theta / e_phi as interleaved complex float64
+ * ([n_theta][n_phi][re, im], the layout of the reference's Vec<Complex64>;
+ * the cheaper form, copied without reordering). */
 typedef struct {
     double* e_theta_re;
     double* e_theta_im;
@@ -428,6 +431,8 @@ typedef struct {
     double gemm_ms;    /* device time of the matrix-core contractions */
     double total_ms;   /* device time of the whole transform */
     double gemm_flop;  /* real floating-point operations of the contractions */
+    double* e_theta;   /* interleaved complex, or null */
+    double* e_phi;
 } oe_far_field;
 
 typedef struct oe_nf2ff oe_nf2ff;

@@ -1199,6 +1199,8 @@ Nf2ffOut far_field_in(const oe_far_field* out) {
     o.et_im = out->e_theta_im;
     o.ep_re = out->e_phi_re;
     o.ep_im = out->e_phi_im;
+    o.et = out->e_theta;
+    o.ep = out->e_phi;
     return o;
 }
 oe_status angles_ok(const double* theta, size_t nt, const double* phi, size_t np, double r, oe_far_field* out) {

@@ -333,6 +333,8 @@ struct Nf2ffOut {
     double* et_im;
     double* ep_re;
     double* ep_im;
+    double* et;     // [n_theta][n_phi][re, im] each, or null
+    double* ep;
     double prad, dmax;
     double gemm_ms, total_ms, gemm_flop;
 };

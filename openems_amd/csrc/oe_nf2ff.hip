@@ -346,10 +346,11 @@ __global__ __launch_bounds__(256) void k_nf2ff_project(const double* __restrict_
         const double br = ltr - Z0 * npr, bi = lti - Z0 * npi;  // E_phi = fac * B
         const double etr = -(fr * ar - fi * ai), eti = -(fr * ai + fi * ar);
         const double epr = fr * br - fi * bi, epi = fr * bi + fi * br;
-        out[a] = etr;
-        out[n_ang + a] = eti;
-        out[2 * n_ang + a] = epr;
-        out[3 * n_ang + a] = epi;
+        // interleaved complex: [E_theta][a][re, im], then [E_phi][a][re, im]
+        out[2 * a] = etr;
+        out[2 * a + 1] = eti;
+        out[2 * n_ang + 2 * a] = epr;
+        out[2 * n_ang + 2 * a + 1] = epi;
         u = r * r / (2.0 * Z0) * (etr * etr + eti * eti + epr * epr + epi * epi);
     }
     red[threadIdx.x] = u;
@@ -431,6 +432,9 @@ struct Nf2ffCache {
         DevBuf order;            // n_ang ints, angles sorted by u
     } ax[3];
     DevBuf work;
+    // timing events, created on first use and kept: [0] brackets the whole
+    // transform, the others one GEMM launch each
+    std::vector<std::unique_ptr<EventPair>> events;
 };
 
 Nf2ffCache* nf2ff_cache_new() { return new Nf2ffCache(); }
@@ -587,11 +591,21 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
     NF_OK(hipMemcpyAsync(d_coord, coords.data(), coords.size() * 8, hipMemcpyHostToDevice, s));
     NF_OK(hipMemsetAsync(d_nl, 0, 12 * n_ang * sizeof(double), s));
 
-    std::vector<std::unique_ptr<EventPair>> gemm_ev;
-    EventPair all;
-    NF_OK(hipEventCreate(&all.a));
-    NF_OK(hipEventCreate(&all.b));
-    NF_OK(hipEventRecord(all.a, s));
+    size_t n_ev = 0;
+    auto next_events = [&](EventPair** out) -> hipError_t {
+        if (n_ev == C.events.size()) {
+            std::unique_ptr<EventPair> p(new EventPair());
+            NF_OK(hipEventCreate(&p->a));
+            NF_OK(hipEventCreate(&p->b));
+            C.events.push_back(std::move(p));
+        }
+        *out = C.events[n_ev++].get();
+        return hipSuccess;
+    };
+    EventPair* all = nullptr;
+    NF_OK(next_events(&all));
+    std::vector<EventPair*> gemm_ev;
+    NF_OK(hipEventRecord(all->a, s));
     double flop = 0.0;
     size_t co = 0, bo = 0;
     for (size_t fi = 0; fi < faces.size(); ++fi) {
@@ -644,10 +658,10 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
             hipLaunchKernelGGL(k_phase, dim3(pb), dim3(256), 0, s, Am, pl.K, total, u0, u1, pl.np, A.uval.as<double>(),
                                xp, k);
             NF_OK(hipGetLastError());
-            gemm_ev.emplace_back(new EventPair());
-            NF_OK(hipEventCreate(&gemm_ev.back()->a));
-            NF_OK(hipEventCreate(&gemm_ev.back()->b));
-            NF_OK(hipEventRecord(gemm_ev.back()->a, s));
+            EventPair* ev = nullptr;
+            NF_OK(next_events(&ev));
+            gemm_ev.push_back(ev);
+            NF_OK(hipEventRecord(ev->a, s));
             NF_OK(launch_gemm(Am, Bm, G, mp, pl.N, pl.K, s));
             NF_OK(hipEventRecord(gemm_ev.back()->b, s));
             flop += 2.0 * (2.0 * (u1 - u0)) * (8.0 * pl.nq) * pl.np;  // useful (unpadded) work
@@ -665,7 +679,7 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
     hipLaunchKernelGGL(k_nf2ff_project, dim3((unsigned)pblocks), dim3(256), 0, s, d_nl, d_kap + 3 * n_ang,
                        d_kap + 3 * n_ang + n_theta, n_phi, n_ang, fac.real(), fac.imag(), r, d_ff, d_umax);
     NF_OK(hipGetLastError());
-    NF_OK(hipEventRecord(all.b, s));
+    NF_OK(hipEventRecord(all->b, s));
     const size_t hb = (4 * n_ang + pblocks + blocks) * sizeof(double);
     if (C.host.bytes < hb) NF_OK(C.host.alloc(hb));
     double* h = reinterpret_cast<double*>(C.host.p);
@@ -673,7 +687,7 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
     NF_OK(hipMemcpyAsync(h + 4 * n_ang + pblocks, d_part, blocks * sizeof(double), hipMemcpyDeviceToHost, s));
     NF_OK(hipStreamSynchronize(s));
     float ms = 0.f;
-    NF_OK(hipEventElapsedTime(&ms, all.a, all.b));
+    NF_OK(hipEventElapsedTime(&ms, all->a, all->b));
     out->total_ms = ms;
     out->gemm_ms = 0.0;
     for (auto& e : gemm_ev) {
@@ -686,9 +700,14 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
     out->prad = prad;
     double umax = 0.0;
     for (size_t i = 0; i < pblocks; ++i) umax = std::max(umax, h[4 * n_ang + i]);
+    if (out->et) std::memcpy(out->et, h, 2 * n_ang * sizeof(double));
+    if (out->ep) std::memcpy(out->ep, h + 2 * n_ang, 2 * n_ang * sizeof(double));
     double* dst[4] = {out->et_re, out->et_im, out->ep_re, out->ep_im};
     for (int q = 0; q < 4; ++q)
-        if (dst[q]) std::memcpy(dst[q], h + q * n_ang, n_ang * sizeof(double));
+        if (dst[q]) {
+            const double* src = h + (q / 2) * 2 * n_ang + (q % 2);
+            for (size_t a = 0; a < n_ang; ++a) dst[q][a] = src[2 * a];
+        }
     out->dmax = prad > 0.0 ? 4.0 * M_PI * umax / prad : 0.0;
     return hipSuccess;
 }

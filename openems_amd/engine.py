@@ -285,17 +285,29 @@ class FarFieldResult:
         return self.dmax
 
 
-def _far_field(call, frequency, theta, phi, radius, center) -> FarFieldResult:
+def _far_field(call, frequency, theta, phi, radius, center, split: bool = False) -> FarFieldResult:
+    """The C call writes E_theta and E_phi as interleaved complex straight
+    into the result arrays; split=True asks for the separate real and
+    imaginary arrays instead (the other output form of the C-ABI)."""
     th = np.ascontiguousarray(theta, dtype=np.float64).ravel()
     ph = np.ascontiguousarray(phi, dtype=np.float64).ravel()
     shape = (th.size, ph.size)
-    bufs = [np.empty(shape, np.float64) for _ in range(4)]
     dp = C.POINTER(C.c_double)
-    out = _ffi.FarField(*[b.ctypes.data_as(dp) for b in bufs], 0.0, 0.0, 0.0, 0.0, 0.0)
+    out = _ffi.FarField()
+    if split:
+        bufs = [np.empty(shape, np.float64) for _ in range(4)]
+        out.e_theta_re, out.e_theta_im, out.e_phi_re, out.e_phi_im = (b.ctypes.data_as(dp) for b in bufs)
+    else:
+        et = np.empty(shape, np.complex128)
+        ep = np.empty(shape, np.complex128)
+        out.e_theta = et.view(np.float64).ctypes.data_as(dp)
+        out.e_phi = ep.view(np.float64).ctypes.data_as(dp)
     c = (C.c_double * 3)(*map(float, center))
     check(call(th.ctypes.data_as(dp), th.size, ph.ctypes.data_as(dp), ph.size, float(radius), c, C.byref(out)))
-    return FarFieldResult(th, ph, bufs[0] + 1j * bufs[1], bufs[2] + 1j * bufs[3], float(frequency), out.prad, out.dmax,
-                          out.gemm_ms, out.total_ms, out.gemm_flop)
+    if split:
+        et, ep = bufs[0] + 1j * bufs[1], bufs[2] + 1j * bufs[3]
+    return FarFieldResult(th, ph, et, ep, float(frequency), out.prad, out.dmax, out.gemm_ms, out.total_ms,
+                          out.gemm_flop)
 
 
 class Nf2ff:
@@ -326,9 +338,9 @@ class Nf2ff:
                                          c1.size, c2.ctypes.data_as(dp), c2.size,
                                          e.view(np.float64).ctypes.data_as(dp), hh.view(np.float64).ctypes.data_as(dp)))
 
-    def calculate(self, theta, phi, radius: float = 1.0, center=(0.0, 0.0, 0.0)) -> FarFieldResult:
+    def calculate(self, theta, phi, radius: float = 1.0, center=(0.0, 0.0, 0.0), split: bool = False) -> FarFieldResult:
         return _far_field(lambda th, nt, ph, np_, r, c, out: lib().oe_nf2ff_calculate(self._h, th, nt, ph, np_, r, c, out),
-                          self.frequency, theta, phi, radius, center)
+                          self.frequency, theta, phi, radius, center, split)
 
 
 @dataclass

@@ -36,7 +36,12 @@ def _gpu_far_field(faces, f, th, ph, r=1.0, center=(0, 0, 0)):
     for fc in faces:
         a1, a2 = (fc.axis + 1) % 3, (fc.axis + 2) % 3
         nf.add_surface(fc.axis, fc.sign, fc.c0, fc.c1, fc.c2, fc.E[[a1, a2]], fc.H[[a1, a2]])
-    return nf.calculate(th, ph, r, center)
+    g = nf.calculate(th, ph, r, center)
+    # the split output form of the C-ABI holds the same values
+    s = nf.calculate(th, ph, r, center, split=True)
+    assert np.array_equal(g.e_theta, s.e_theta) and np.array_equal(g.e_phi, s.e_phi)
+    assert g.prad == s.prad and g.dmax == s.dmax
+    return g
 
 
 def _check(name, faces, f, th, ph, r=1.0, center=(0, 0, 0)):
