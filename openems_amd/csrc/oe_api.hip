@@ -296,6 +296,7 @@ oe_status engine_alloc(oe_engine* e, size_t nx, size_t ny, size_t nz, int device
     if (const char* v = getenv("OE_FUSED")) e->fused_mode = std::max(0, std::min(2, atoi(v)));
     if (const char* v = getenv("OE_GRAPH_STEPS")) e->graph_steps = std::max(0, atoi(v));
     if (const char* v = getenv("OE_OVERLAP_SLABS")) e->overlap_slabs = atoi(v) != 0;
+    if (const char* v = getenv("OE_FOLD_MUR")) e->fold_mur = atoi(v) != 0;
     return OE_OK;
 }
 
@@ -960,6 +961,8 @@ oe_status oe_ext_ade_create(oe_engine* e, int kind, const double params[3], cons
         if (i >= (uint32_t)e->g.nx || j >= (uint32_t)e->g.ny || k >= (uint32_t)e->g.nz)
             return fail(OE_ERR_CONFIG, "ADE position %zu out of the grid", c);
         idx[c] = (uint32_t)(((size_t)i * e->g.ny + j) * e->g.nz + k);
+        if (c == 0 || (int)k < x->ade_kmin) x->ade_kmin = (int)k;
+        if ((int)k > x->ade_kmax) x->ade_kmax = (int)k;
     }
     AdeDev& a = x->ade;
     a.kind = kind; a.mode = mode; a.n = (int)n;
@@ -1685,6 +1688,33 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
         dft0[q] = dfts[q]->dft_count;
     }
 
+    // First-order Mur on the z faces goes into the E kernels when that is
+    // exact: one Mur extension with z faces only, float4 lanes (a face's
+    // line and shift cells then belong to one thread), in-place steps, no
+    // UPML, and no ADE cell on a face's line or shift plane (ADE post runs
+    // between the E update and the Mur post step).
+    MurZ mz{};
+    bool fold_mur = e->fold_mur && murs.size() == 1 && !upml && !fused && tl.vec == 4 && e->g.nz >= 4;
+    if (fold_mur)
+        for (const MurDev& m : murs[0]->mur_faces) {
+            if (m.ny != 2) {
+                fold_mur = false;
+                break;
+            }
+            const int side = m.line == 0 ? 0 : 1;
+            mz.on[side] = 1;
+            mz.coeff[side] = m.coeff;
+        }
+    if (fold_mur)
+        for (auto* x : ades) {
+            if (x->ade_kmax < x->ade_kmin) continue;  // no cells
+            if (mz.on[0] && x->ade_kmin <= 1) fold_mur = false;
+            if (mz.on[1] && x->ade_kmax >= e->g.nz - 2) fold_mur = false;
+        }
+    if (!fold_mur) mz = MurZ{};
+    std::vector<Launch> plan_e = plan;  // the E launches carry the folded Mur faces
+    for (Launch& L : plan_e) L.a.mur_z = mz;
+
     // The slab launches of a half-step write other cells than the interior
     // launch and read only the other field (and their own psi), so they run
     // on the side stream beside it: fork after the previous work, join
@@ -1713,11 +1743,12 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             a.exc_step_dev = step_dev;
             return launch_update_e(a, cp, L.axes, s);
         };
+        const std::vector<Launch>& pl = h ? plan : plan_e;
         HIP_OK(hipEventRecord(e->ev_fork, e->stream));
         HIP_OK(hipStreamWaitEvent(e->side, e->ev_fork, 0));
-        for (const Launch& L : plan)
+        for (const Launch& L : pl)
             if (L.axes & (AX_X | AX_Y)) HIP_OK(one(L, e->side));
-        for (const Launch& L : plan)
+        for (const Launch& L : pl)
             if (!(L.axes & (AX_X | AX_Y))) HIP_OK(one(L, e->stream));
         HIP_OK(hipEventRecord(e->ev_join, e->side));
         HIP_OK(hipStreamWaitEvent(e->stream, e->ev_join, 0));
@@ -1786,6 +1817,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             HIP_OK(launch_upml_post(e->F, e->g, upml->upml, false, e->stream));
         }
         for (auto* x : murs) {
+            if (fold_mur) break;
             Timed tm(e, "mur_pre");
             HIP_OK(launch_mur_pre(e->F, e->g, x->mur_dev.as<MurDev>(), (int)x->mur_faces.size(), e->stream));
         }
@@ -1801,7 +1833,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
         if (!fused && ovl) {
             OE_TRY(overlapped_half(false, st));
         } else if (!fused)
-            for (const Launch& L : plan) {
+            for (const Launch& L : plan_e) {
                 Timed tm(e, (L.axes & (AX_X | AX_Y)) ? "update_e:slabs" : (cp ? "update_e:interior" : "update_e:grid"));
                 if (fold_sources) {
                     YeeArgs a = L.a;
@@ -1824,6 +1856,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             HIP_OK(launch_ade_post(e->F, e->Cf, e->g.N, x->ade, e->stream));
         }
         for (auto* x : murs) {
+            if (fold_mur) break;
             Timed tm(e, "mur_post");
             HIP_OK(launch_mur_post(e->F, e->g, x->mur_dev.as<MurDev>(), (int)x->mur_faces.size(), e->stream));
         }

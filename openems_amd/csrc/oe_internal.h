@@ -87,6 +87,14 @@ struct Box {
 struct PointExc;
 constexpr int MAX_FOLD_SOURCES = 8;
 
+// Mur faces folded into the E update: a z face's line and shift cells are
+// lanes of one float4 lane group, so the thread that updates them also takes
+// the pre value, the post correction and the boundary write.
+struct MurZ {
+    int on[2];       // [k = 0 face, k = nz - 1 face]
+    float coeff[2];
+};
+
 struct YeeArgs {
     Grid g;
     // Fields read (Fin) and written (Fout); equal for an in-place update.
@@ -120,6 +128,9 @@ struct YeeArgs {
     uint32_t exc_idx[MAX_FOLD_SOURCES];
     uint64_t exc_step;
     const uint64_t* exc_step_dev;  // non-null: the step comes from the device counter
+    // First-order Mur on the z faces applied by the E half-step itself (see
+    // e_body): on[0] the k = 0 face, on[1] the k = nz - 1 face.
+    MurZ mur_z;
 };
 
 // CPML code compiled into a launch: bit 0 x faces, bit 1 y faces, bit 2 z faces.

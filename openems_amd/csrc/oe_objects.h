@@ -83,6 +83,7 @@ struct oe_extension {
     // ADE
     AdeDev ade{};
     DevBuf ade_mem;
+    int ade_kmin = 0, ade_kmax = -1;  // k range of the cells (empty: kmax < kmin)
     // probe
     uint32_t probe_idx = 0;
     DevBuf probe_mem;   // data
@@ -144,6 +145,9 @@ struct oe_engine {
     // interior ones retire and the cross-queue join adds about 15 us per
     // half-step (0.2815 against 0.2577 ms per step at 256^3).
     int overlap_slabs = 0;
+    // z-face Mur applied by the E kernels where exact (oe_engine_run_batch
+    // checks the conditions): 1 on, 0 off
+    int fold_mur = 1;
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     DevBuf exc_table;

@@ -418,10 +418,19 @@ __device__ __forceinline__ void apply_sources(const Sources& src, size_t p, Vec<
     }
 }
 
-template <int V, int AX, bool PHYS, bool TAB>
+// MUR: first-order Mur on the z faces in mz (MurAbc, src/extensions/
+// mur_abc.rs:213-324) applied by the thread holding the face's line and
+// shift cells (V = 4 lanes): pre v = E_s - c E_l from the old values, post
+// v += c E_s from the updated ones, then E_l = v, for Ex and Ey; rows the
+// update skips (i = 0 or j = 0) get the same writes from their old values.
+// Valid when no other hook acts on those cells between the E update and the
+// Mur post step (the host checks it).
+template <int V, int AX, bool PHYS, bool TAB, bool MUR = false>
 __device__ __forceinline__ void e_body(const Grid& g, const float* Fin, float* Fout, const float* __restrict__ Cf,
                                        const uint16_t* __restrict__ mat, const float2* tab,
-                                       const CpmlArgs& c, const Tile& t, const Sources* src = nullptr) {
+                                       const CpmlArgs& c, const Tile& t, const Sources* src = nullptr,
+                                       const MurZ& mz = MurZ{}) {
+    static_assert(!MUR || V == 4, "the folded Mur needs float4 lanes");
     const int ny = g.ny, nz = g.nz;
     if (t.k0 >= nz || t.j >= t.jend) return;
     const size_t N = g.N;
@@ -457,6 +466,9 @@ __device__ __forceinline__ void e_body(const Grid& g, const float* Fin, float* F
         hy_p = ldv<V>(Hy + p - plane);
         hz_p = ldv<V>(Hz + p - plane);
     }
+    // lanes of the folded Mur faces held by this thread
+    const bool mur_lo = MUR && mz.on[0] && k0 == 0;
+    const bool mur_hi = MUR && mz.on[1] && k0 + V == nz;
 
     for (int i = t.i0; i < t.i1; ++i, p += plane) {
         const bool gi = i > 0;
@@ -480,6 +492,24 @@ __device__ __forceinline__ void e_body(const Grid& g, const float* Fin, float* F
             } else {
                 s0 = ldn<V, OE_NT_COEF>(ca0 + p); s1 = ldn<V, OE_NT_COEF>(ca1 + p); s2 = ldn<V, OE_NT_COEF>(ca2 + p);
                 c0 = ldn<V, OE_NT_COEF>(cb0 + p); c1 = ldn<V, OE_NT_COEF>(cb1 + p); c2 = ldn<V, OE_NT_COEF>(cb2 + p);
+            }
+        }
+        // Mur pre step on the old values: v = E(shift) - c E(line)
+        float mvx[2] = {0.f, 0.f}, mvy[2] = {0.f, 0.f};
+        if constexpr (MUR) {
+            if (mur_lo || mur_hi) {
+                if (!row) {
+                    ex = ldn<V, OE_NT_OWN>(Ex + p);
+                    ey = ldn<V, OE_NT_OWN>(Ey + p);
+                }
+                if (mur_lo) {
+                    mvx[0] = ex.v[1] - mz.coeff[0] * ex.v[0];
+                    mvy[0] = ey.v[1] - mz.coeff[0] * ey.v[0];
+                }
+                if (mur_hi) {
+                    mvx[1] = ex.v[V - 2] - mz.coeff[1] * ex.v[V - 1];
+                    mvy[1] = ey.v[V - 2] - mz.coeff[1] * ey.v[V - 1];
+                }
             }
         }
 
@@ -537,6 +567,22 @@ __device__ __forceinline__ void e_body(const Grid& g, const float* Fin, float* F
         if constexpr ((AX & AX_Y) != 0) psi_store<V>(py);
         if constexpr ((AX & AX_Z) != 0) psi_store<V>(pz);
 
+        // Mur post step and boundary write: v += c E(shift), E(line) = v
+        if constexpr (MUR) {
+            if (mur_lo) {
+                ex.v[0] = mvx[0] + mz.coeff[0] * ex.v[1];
+                ey.v[0] = mvy[0] + mz.coeff[0] * ey.v[1];
+            }
+            if (mur_hi) {
+                ex.v[V - 1] = mvx[1] + mz.coeff[1] * ex.v[V - 2];
+                ey.v[V - 1] = mvy[1] + mz.coeff[1] * ey.v[V - 2];
+            }
+            if (!row && (mur_lo || mur_hi)) {
+                stn<V, OE_NT_STORE>(Exw + p, ex);
+                stn<V, OE_NT_STORE>(Eyw + p, ey);
+            }
+        }
+
         if (row) {
             if (src) apply_sources<V>(*src, p, ex, ey, ez);
             stn<V, OE_NT_STORE>(Exw + p, ex);
@@ -566,13 +612,18 @@ __global__ __launch_bounds__(256) void k_update_h(YeeArgs a, CpmlArgs c) {
     h_body<V, AX, PHYS, TAB>(a.g, a.Fin, a.Fout, a.Cf, a.mat_h, lds_rows, c, decode<V>(a));
 }
 
-template <int V, int AX, bool PHYS, bool TAB>
+template <int V, int AX, bool PHYS, bool TAB, bool MUR = false>
 __global__ __launch_bounds__(256) void k_update_e(YeeArgs a, CpmlArgs c) {
     extern __shared__ float2 lds_rows[];
     if constexpr (TAB) {
         const float2* src = reinterpret_cast<const float2*>(a.tab_e);
         for (int q = threadIdx.x; q < 3 * a.n_mat_e; q += blockDim.x) lds_rows[q] = src[q];
         __syncthreads();
+    }
+    if constexpr (MUR) {
+        // sources follow the Mur step, so they are never folded with it
+        e_body<V, AX, PHYS, TAB, true>(a.g, a.Fin, a.Fout, a.Cf, a.mat_e, lds_rows, c, decode<V>(a), nullptr, a.mur_z);
+        return;
     }
     if (a.n_exc) {
         const Sources src{a.exc, a.exc_idx, a.n_exc, a.exc_step, a.exc_step_dev};
@@ -659,6 +710,19 @@ hipError_t launch_h_t(const YeeArgs& a, const CpmlArgs& c, hipStream_t s) {
 
 template <int V, int AX, bool PHYS>
 hipError_t launch_e_t(const YeeArgs& a, const CpmlArgs& c, hipStream_t s) {
+    if constexpr (V == 4) {
+        if (a.mur_z.on[0] || a.mur_z.on[1]) {
+            if (a.n_mat_e > 0)
+                hipLaunchKernelGGL((k_update_e<V, AX, PHYS, true, true>), dim3(a.first[a.nbox]), dim3(256),
+                                   3 * a.n_mat_e * sizeof(float2), s, a, c);
+            else
+                hipLaunchKernelGGL((k_update_e<V, AX, PHYS, false, true>), dim3(a.first[a.nbox]), dim3(256), 0, s, a,
+                                   c);
+            return hipGetLastError();
+        }
+    } else {
+        if (a.mur_z.on[0] || a.mur_z.on[1]) return hipErrorInvalidValue;  // needs float4 lanes
+    }
     if (a.n_mat_e > 0)
         hipLaunchKernelGGL((k_update_e<V, AX, PHYS, true>), dim3(a.first[a.nbox]), dim3(256),
                            3 * a.n_mat_e * sizeof(float2), s, a, c);
