@@ -1678,8 +1678,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
     // check or timed step) is replayed from one captured graph.  The DFTs
     // then form their sample times from the batch's first step.
     if (e->step_dev.bytes < sizeof(uint64_t)) HIP_OK(e->step_dev.alloc(sizeof(uint64_t)));
-    uint64_t* step_dev = e->step_dev.as<uint64_t>();
-    HIP_OK(hipMemsetAsync(step_dev, 0, sizeof(uint64_t), e->stream));
+    uint64_t* step_dev = e->step_dev.as<uint64_t>();  // reset below, once the step scheme is known
     const uint64_t t_batch0 = e->timestep;
     bool dev_dft = true;
     std::vector<uint64_t> dft0(dfts.size());
@@ -1714,6 +1713,13 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
     if (!fold_mur) mz = MurZ{};
     std::vector<Launch> plan_e = plan;  // the E launches carry the folded Mur faces
     for (Launch& L : plan_e) L.a.mur_z = mz;
+    // Two-half-step batches advance the device step counter in their first H
+    // launch (from -1, so that step st reads st); fused batches launch a
+    // counter kernel at the end of each step (from 0).
+    std::vector<Launch> plan_h = plan;
+    const bool step_in_h = !fused && !plan_h.empty();
+    if (step_in_h) plan_h[0].a.step_inc = step_dev;
+    HIP_OK(hipMemsetAsync(step_dev, step_in_h ? 0xff : 0, sizeof(uint64_t), e->stream));
 
     // The slab launches of a half-step write other cells than the interior
     // launch and read only the other field (and their own psi), so they run
@@ -1743,7 +1749,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             a.exc_step_dev = step_dev;
             return launch_update_e(a, cp, L.axes, s);
         };
-        const std::vector<Launch>& pl = h ? plan : plan_e;
+        const std::vector<Launch>& pl = h ? plan_h : plan_e;
         HIP_OK(hipEventRecord(e->ev_fork, e->stream));
         HIP_OK(hipStreamWaitEvent(e->side, e->ev_fork, 0));
         for (const Launch& L : pl)
@@ -1808,7 +1814,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
         if (!fused && ovl) {
             OE_TRY(overlapped_half(true, st));
         } else if (!fused)
-            for (const Launch& L : plan) {
+            for (const Launch& L : plan_h) {
                 Timed tm(e, (L.axes & (AX_X | AX_Y)) ? "update_h:slabs" : (cp ? "update_h:interior" : "update_h:grid"));
                 HIP_OK(launch_update_h(L.a, cp, L.axes, e->stream));
             }
@@ -1878,7 +1884,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
                 x->dft_count += 1;
             }
         }
-        HIP_OK(launch_sweep_step(step_dev, e->stream));
+        if (!step_in_h) HIP_OK(launch_sweep_step(step_dev, e->stream));
         return OE_OK;
     };
 

@@ -131,6 +131,10 @@ struct YeeArgs {
     // First-order Mur on the z faces applied by the E half-step itself (see
     // e_body): on[0] the k = 0 face, on[1] the k = nz - 1 face.
     MurZ mur_z;
+    // Non-null on one H launch per step: its first lane advances the device
+    // step counter before anything of the step reads it (the counter starts
+    // a batch at -1).
+    uint64_t* step_inc;
 };
 
 // CPML code compiled into a launch: bit 0 x faces, bit 1 y faces, bit 2 z faces.

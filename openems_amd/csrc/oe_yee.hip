@@ -604,6 +604,8 @@ __device__ __forceinline__ void e_body(const Grid& g, const float* Fin, float* F
 template <int V, int AX, bool PHYS, bool TAB>
 __global__ __launch_bounds__(256) void k_update_h(YeeArgs a, CpmlArgs c) {
     extern __shared__ float2 lds_rows[];
+    // the step's first launch advances the step counter (no H lane reads it)
+    if (a.step_inc && blockIdx.x == 0 && threadIdx.x == 0) *a.step_inc += 1;
     if constexpr (TAB) {
         const float2* src = reinterpret_cast<const float2*>(a.tab_h);
         for (int q = threadIdx.x; q < 3 * a.n_mat_h; q += blockDim.x) lds_rows[q] = src[q];
