@@ -204,19 +204,19 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
             m, _ = model.forward(X[b0:b0 + d.batch], tails=False)
             modes[b0:b0 + d.batch] = m
         s, aV, aI = mode_params(modes, d.modes)
-        z_n = np.arange(n_chk)
-        z_t = np.arange(n_chk, n_chk + (max_steps - pos) // decimate)
+        # the modes over the check steps and from the checkpoint to max_steps,
+        # all structures at once: x[k, n] = Re sum_m a[k, m] exp(s[k, m] n)
+        n_all = np.arange(n_chk + (max_steps - pos) // decimate)
+        zn = np.exp(s[:, :, None] * n_all)                       # [K, M, n]
+        pv = np.real(np.einsum("km,kmn->kn", aV, zn))
+        pi = np.real(np.einsum("km,kmn->kn", aI, zn))
         rejected = []
         for k, q in enumerate(still):
             sv, si = scales[k]
-            z = np.exp(s[k][:, None] * z_n)
-            pv = np.real((aV[k][:, None] * z).sum(0))
-            pi = np.real((aI[k][:, None] * z).sum(0))
             e_win = (X[k].astype(np.float64) ** 2).sum()
-            resid[q] = float(((pv - chk[k][0]) ** 2 + (pi - chk[k][1]) ** 2).sum() / max(e_win, 1e-30))
-            zt = np.exp(s[k][:, None] * z_t)
-            e_tail = float((np.real((aV[k][:, None] * zt).sum(0)) ** 2).sum()
-                           + (np.real((aI[k][:, None] * zt).sum(0)) ** 2).sum())
+            nc = len(chk[k][0])
+            resid[q] = float(((pv[k, :nc] - chk[k][0]) ** 2 + (pi[k, :nc] - chk[k][1]) ** 2).sum() / max(e_win, 1e-30))
+            e_tail = float((pv[k, n_chk:] ** 2).sum() + (pi[k, n_chk:] ** 2).sum())
             e_run = float((sers[k][:pos:decimate, 0] / sv) @ (sers[k][:pos:decimate, 0] / sv)
                           + (sers[k][:pos:decimate, 1] / si) @ (sers[k][:pos:decimate, 1] / si))
             tail_frac[q] = e_tail / max(e_run, 1e-300)

@@ -52,3 +52,11 @@ def test_ade_on_mur_plane_keeps_separate_launches():
     assert_parity(res, sc.name)
     for g, o in zip(res["gpu_ade"], res["oracle_ade"]):
         assert np.array_equal(g, o.astype(np.float32))
+
+
+def test_scalar_lanes_keep_separate_launches():
+    """nz = 30 gives scalar lanes: the z-face Mur stays in its own launches
+    and still matches the oracle."""
+    sc = Scenario("mur_scalar_lanes", (22, 20, 30), 0.5e-3, 180, mur=(False, False, False, False, True, True),
+                  excitations=[((11, 10, 27), 2, True, gauss_ref(10e9)), ((11, 10, 2), 0, True, gauss_ref(12e9))])
+    assert_parity(run_pair(sc, batches=[41, 139]), sc.name)
