@@ -435,6 +435,8 @@ struct Nf2ffCache {
     // timing events, created on first use and kept: [0] brackets the whole
     // transform, the others one GEMM launch each
     std::vector<std::unique_ptr<EventPair>> events;
+    // byte offset and length of the B region last zeroed in `work`
+    size_t bm_off = SIZE_MAX, bm_len = 0;
 };
 
 Nf2ffCache* nf2ff_cache_new() { return new Nf2ffCache(); }
@@ -568,13 +570,29 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
     const size_t pblocks = (n_ang + 255) / 256;
     const size_t n_dbl = 12 * n_ang + 4 * n_ang + pblocks + coord_len + blocks;
     const size_t bytes = round_up(n_dbl * sizeof(double), 256) + (Kmax * Nmax + rows * Kmax + rows * Nmax) * sizeof(float);
-    if (C.work.bytes < bytes) NF_OK(C.work.alloc(bytes));
+    if (C.work.bytes < bytes) {
+        NF_OK(C.work.alloc(bytes));
+        C.bm_off = SIZE_MAX;
+    }
     double* d_nl = C.work.as<double>();
     double* d_ff = d_nl + 12 * n_ang;
     double* d_umax = d_ff + 4 * n_ang;
     double* d_coord = d_umax + pblocks;
     double* d_part = d_coord + coord_len;
     float* Bm = reinterpret_cast<float*>(reinterpret_cast<char*>(C.work.p) + round_up(n_dbl * sizeof(double), 256));
+    // B is zeroed only when its region moves or grows.  Its padding rows
+    // (p >= np) are never cleared per face: they keep finite values (zeros
+    // or an earlier face's entries), and A's padding columns are exact zeros
+    // (k_phase), so they add signed zeros to every product; B's padding
+    // columns only reach G columns the reduction never reads.
+    {
+        const size_t off = round_up(n_dbl * sizeof(double), 256), len = Kmax * Nmax * sizeof(float);
+        if (off != C.bm_off || len > C.bm_len) {
+            NF_OK(hipMemsetAsync(Bm, 0, len, s));
+            C.bm_off = off;
+            C.bm_len = len;
+        }
+    }
     float* Am = Bm + Kmax * Nmax;
     float* G = Am + rows * Kmax;
 
@@ -620,7 +638,6 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
         const double* wq = xq + pl.nq;
         co += 2 * ((size_t)pl.np + pl.nq);
         const int nb = (n1 * n2 + 255) / 256;
-        NF_OK(hipMemsetAsync(Bm, 0, (size_t)pl.K * pl.N * sizeof(float), s));
         if (f.src_kind == 0)
             hipLaunchKernelGGL(k_nf2ff_pack<0>, dim3(nb), dim3(256), 0, s, f.src, f.scale, n1, n2, pl.swap, f.sign,
                                wp, wq, Bm, pl.N, d_part + bo);
