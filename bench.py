@@ -5,7 +5,10 @@ Workload (default): 256^3 grid, 0.5 mm cells, 8-cell CPML on every face
 (physical mode), 200 seeded dielectric boxes, soft Ez Gaussian (fc = 15 GHz)
 at the centre; energy sampled every 100 steps as the reference's
 Simulation::run does.  A "step" is one full timestep (H update with fused
-CPML, E update with fused CPML, excitation, energy monitoring).
+CPML, E update with fused CPML, excitation, energy monitoring).  --cavity
+runs the 100^3 plain vacuum cavity instead (1 mm cells, the reference's PEC
+faces), the size the reference publishes its own per-step times for; its
+vs_baseline is against the reference's best published 100^3 figure.
 
 Prints one JSON line:
   value      MCells/s = cells * steps / wall over the timed steps (inputs and
@@ -39,6 +42,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MCells/s & %HBM peak; sweep time-to-|ΔS11|≤1e-3, learned-stop vs plain"
 HBM_PEAK_GBS = 8000.0
+REF_MCELLS_100 = 718.0  # reference WGPU engine, 100^3 vacuum Yee, Apple M2 Pro
 
 
 def log(msg):
@@ -254,7 +258,8 @@ def cpu_baseline(wl, budget_s=12.0, max_steps=4000):
         "unit": "MCells/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{steps} timesteps (H+E with physical CPML) of the {wl.shape[0]}x{wl.shape[1]}x{wl.shape[2]} "
+        "sample": f"{steps} timesteps (H+E{' with physical CPML' if wl.cpml_layers else ''}) of the "
+                  f"{wl.shape[0]}x{wl.shape[1]}x{wl.shape[2]} "
                   f"workload on {threads} OpenMP threads ({isa}), {wall:.1f} s",
     }
 
@@ -579,6 +584,8 @@ def main():
     ap.add_argument("--sweep-count", type=int, default=256)
     ap.add_argument("--sweep-max-steps", type=int, default=20000)
     ap.add_argument("--composite", action="store_true", help="run the 384x384x256 composite workload instead")
+    ap.add_argument("--cavity", action="store_true",
+                    help="the 100^3 plain vacuum cavity (the reference's published size) instead of the CPML workload")
     ap.add_argument("--nf2ff-stride", type=int, default=4)
     ap.add_argument("--learned", action="store_true", help="learned-stop vs plain sweep (trains the extrapolator)")
     ap.add_argument("--learned-train", type=int, default=1024)
@@ -630,7 +637,7 @@ def main():
     from openems_amd import MODE_COMPAT, MODE_PHYSICAL
     from openems_amd import workloads as W
 
-    wl = W.cpml_boxes(n=args.n, steps=args.steps)
+    wl = W.plain_cavity(steps=args.steps) if args.cavity else W.cpml_boxes(n=args.n, steps=args.steps)
     mode = MODE_PHYSICAL if args.mode == "physical" else MODE_COMPAT
     log(f"[bench] building {wl.shape} with {len(wl.boxes)} boxes, CPML {wl.cpml_layers} ({args.mode})")
     op, eng, exts = build_engine(wl, mode)
@@ -720,13 +727,19 @@ def main():
         "ms_per_step": round(wall / args.steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": None,
+        # the reference's published 100^3 plain-Yee figure (its WGPU engine,
+        # 1.39 ms per step = 718 Mcells/s, BENCHMARK_RESULTS.md:14-17): the
+        # cavity workload only
+        "vs_baseline": round(mcells / REF_MCELLS_100, 2) if args.cavity else None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"cpml_boxes: {wl.shape[0]}x{wl.shape[1]}x{wl.shape[2]} FP32 Yee, 0.5 mm cells, "
-                        f"{wl.cpml_layers}-cell CPML ({args.mode}) on all faces, {len(wl.boxes)} seeded dielectric "
-                        f"boxes, soft Ez Gaussian fc=15 GHz, energy sampled every 100 steps",
+            "workload": (f"plain_cavity: {wl.shape[0]}^3 FP32 Yee, 1 mm cells, vacuum, the reference's PEC faces "
+                         f"(no boundary extension), soft Ez Gaussian fc=15 GHz at the centre, energy sampled every "
+                         f"100 steps" if args.cavity else
+                         f"cpml_boxes: {wl.shape[0]}x{wl.shape[1]}x{wl.shape[2]} FP32 Yee, 0.5 mm cells, "
+                         f"{wl.cpml_layers}-cell CPML ({args.mode}) on all faces, {len(wl.boxes)} seeded dielectric "
+                         f"boxes, soft Ez Gaussian fc=15 GHz, energy sampled every 100 steps"),
             "grid": list(wl.shape),
             "tiles": eng.tiles(),
             "coefficients": coef,
