@@ -416,6 +416,9 @@ def bench_composite(args):
 
     wl = W.port_spheres(steps=args.steps)
     op, eng, exts = W.build_port_spheres_hip(wl, precision=0, nf2ff_stride=args.nf2ff_stride)
+    if args.tiles:
+        bx, by, ci, vec = (int(x) for x in args.tiles.split(","))
+        eng.set_tiles(bx, by, ci, vec)
     smp = wl.excitation_samples(op.dt, args.warmup + args.steps)
     exc = port_excitations(2, wl.port_region, smp)
     log(f"[bench] composite {wl.shape}: {len(wl.spheres)} spheres, warmup {args.warmup}")

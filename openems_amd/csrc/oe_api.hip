@@ -60,11 +60,12 @@ int pow2ceil(int v) {
 // Automatic tile choice: 256-lane work-groups, as many lanes along k as the
 // row holds (up to one wave of 64 float4 lanes), and i-chunks sized for
 // about 16384 work-groups (64 per CU on 256 CUs), at least 4 planes deep;
-// 2 planes on planes of 256 work-groups or more (512^2 and up) and on grids
+// 2 planes on planes of more than 64 work-groups (384^2 and up) and on grids
 // that 4 planes would cut into fewer than 1024 work-groups.  Measured with
 // the material table, ms per step: 100^3 ci 1 / 2 / 4 = 0.0198 / 0.0197 /
-// 0.0233; 256^3 ci 2 / 3 / 4 = 0.263 / 0.264 / 0.258; 512^3 ci 1 / 2 / 3 / 4
-// / 8 = 2.040 / 1.950 / 2.002 / 1.961 / 1.985.
+// 0.0233; 256^3 ci 2 / 3 / 4 = 0.263 / 0.264 / 0.258; 384x384x256 with
+// every extension ci 2 / 4 = 0.650 / 0.663; 512^3 ci 1 / 2 / 3 / 4 / 8 =
+// 2.040 / 1.950 / 2.002 / 1.961 / 1.985.
 Tiles auto_tiles(const Grid& g, int vec_req) {
     Tiles t{};
     t.vec = (vec_req == 1) ? 1 : ((g.nz % 4 == 0) ? 4 : 1);
@@ -80,7 +81,7 @@ Tiles auto_tiles(const Grid& g, int vec_req) {
     int ci = (int)((g.nx + n_ib - 1) / n_ib);
     ci = std::max(ci, std::min(4, g.nx));
     const long wg_at_4 = plane_blocks * ((g.nx + 3) / 4);
-    if (plane_blocks >= 256 || wg_at_4 < 1024) ci = std::min(2, g.nx);
+    if (plane_blocks > 64 || wg_at_4 < 1024) ci = std::min(2, g.nx);
     t.ci = ci;
     return t;
 }
