@@ -433,6 +433,12 @@ typedef struct {
     double gemm_flop;  /* real floating-point operations of the contractions */
     double* e_theta;   /* interleaved complex, or null */
     double* e_phi;
+    /* The large contractions run on the FP16 matrix cores with each FP32
+     * operand split into two fp16 halves (hi + lo, three products per
+     * element pair; OE_NF2FF_SPLIT16=0 keeps FP32): their device time and
+     * their share of gemm_flop. */
+    double gemm_split_ms;
+    double gemm_split_flop;
 } oe_far_field;
 
 typedef struct oe_nf2ff oe_nf2ff;

@@ -71,7 +71,8 @@ class FarField(C.Structure):
     _fields_ = [("e_theta_re", C.POINTER(C.c_double)), ("e_theta_im", C.POINTER(C.c_double)),
                 ("e_phi_re", C.POINTER(C.c_double)), ("e_phi_im", C.POINTER(C.c_double)),
                 ("prad", C.c_double), ("dmax", C.c_double), ("gemm_ms", C.c_double), ("total_ms", C.c_double),
-                ("gemm_flop", C.c_double), ("e_theta", C.POINTER(C.c_double)), ("e_phi", C.POINTER(C.c_double))]
+                ("gemm_flop", C.c_double), ("e_theta", C.POINTER(C.c_double)), ("e_phi", C.POINTER(C.c_double)),
+                ("gemm_split_ms", C.c_double), ("gemm_split_flop", C.c_double)]
 
 
 class SweepConfig(C.Structure):

@@ -1203,6 +1203,8 @@ void far_field_out(const Nf2ffOut& o, oe_far_field* out) {
     out->gemm_ms = o.gemm_ms;
     out->total_ms = o.total_ms;
     out->gemm_flop = o.gemm_flop;
+    out->gemm_split_ms = o.gemm_split_ms;
+    out->gemm_split_flop = o.gemm_split_flop;
 }
 Nf2ffOut far_field_in(const oe_far_field* out) {
     Nf2ffOut o{};

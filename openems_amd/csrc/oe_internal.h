@@ -352,6 +352,7 @@ struct Nf2ffOut {
     double* ep;
     double prad, dmax;
     double gemm_ms, total_ms, gemm_flop;
+    double gemm_split_ms, gemm_split_flop;  // the part on the FP16 matrix cores (split operands)
 };
 // Angle plan (distinct direction cosines per axis) and work buffers of the
 // transform, reused by later calls with the same angles.

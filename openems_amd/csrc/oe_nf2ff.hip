@@ -33,7 +33,8 @@ namespace {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-constexpr int GBM = 128, GBN = 128, GBK = 16;  // M, N padding and K step of every GEMM
+constexpr int GBM = 128, GBN = 128, GBK = 16;  // M, N padding and K step of the FP32 GEMM
+constexpr int KPAD = 32;                        // K padding (a multiple of both GEMMs' K steps)
 constexpr int LDS_PAD = 32;  // row stride = tile + 32 words: the two half-waves read different bank halves
 
 // C[M x N] = A[M x K] * B[K x N], row-major float32; M, N multiples of TILE,
@@ -117,9 +118,183 @@ __global__ __launch_bounds__(256) void k_sgemm_mfma(const float* __restrict__ A,
             }
 }
 
+// C = A B on the FP16 matrix cores with FP32-like accuracy: each operand
+// arrives split into two fp16 halves, x = hi + lo (hi = fp16(x), lo =
+// fp16(x - hi): 22 significant bits together), and C accumulates
+// hi_a hi_b + hi_a lo_b + lo_a hi_b in FP32; the dropped lo_a lo_b term is
+// below 2^-22 of |a||b|.  A is [M][K] (k fastest, halves Ah then Al), B is
+// given transposed, [N][K] (Bh then Bl), scaled by a power of two whose
+// inverse bsc[1] multiplies C at the end (exact).  Tiles of 128 x 128, K
+// steps of 32 (K a multiple of 32), 4 waves of 2 x 2 blocks of
+// v_mfma_f32_32x32x8f16: lane l takes
+// A[l % 32][4 (l / 32) + j] and B[4 (l / 32) + j][l % 32], j = 0..3, and
+// holds the 32x32 accumulator as the FP32 MFMA does.  In LDS the tiles keep
+// k fastest (rows of 40 halves), so each lane reads its 4 halves with one
+// 8-byte load, and every global load is a 16-byte copy.
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+constexpr int HK = 32;    // K step of the split FP16 GEMM
+constexpr int HROW = 40;  // LDS row stride in halves: 32 of K plus 8 of padding
+
+__global__ __launch_bounds__(256) void k_hgemm3(const _Float16* __restrict__ Ahi, const _Float16* __restrict__ Alo,
+                                               const _Float16* __restrict__ Bhi, const _Float16* __restrict__ Blo,
+                                               float* __restrict__ C, int M, int N, int K,
+                                               const float* __restrict__ bsc) {
+    constexpr int TILE = 128, MB = 2;
+    __shared__ __align__(16) _Float16 Ah[TILE][HROW];
+    __shared__ __align__(16) _Float16 Al[TILE][HROW];
+    __shared__ __align__(16) _Float16 Bh[TILE][HROW];
+    __shared__ __align__(16) _Float16 Bl[TILE][HROW];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int bm = blockIdx.y * TILE, bn = blockIdx.x * TILE;
+    const int wm = (wave >> 1) * (TILE / 2), wn = (wave & 1) * (TILE / 2);
+    // every thread copies 16 halves (2 x 16 B) of one row of each of the four tiles
+    const int r = tid >> 1, kq = (tid & 1) * 16;
+    floatx16 acc[MB][MB];
+#pragma unroll
+    for (int a = 0; a < MB; ++a)
+#pragma unroll
+        for (int b = 0; b < MB; ++b)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
+    half8 ga[4], gb[4];  // [hi 0..7, hi 8..15, lo 0..7, lo 8..15] of this thread's row segment
+    auto load = [&](int k0) {
+        const _Float16* pa = Ahi + (size_t)(bm + r) * K + k0 + kq;
+        const _Float16* pl = Alo + (size_t)(bm + r) * K + k0 + kq;
+        const _Float16* pb = Bhi + (size_t)(bn + r) * K + k0 + kq;
+        const _Float16* ql = Blo + (size_t)(bn + r) * K + k0 + kq;
+        ga[0] = *reinterpret_cast<const half8*>(pa);
+        ga[1] = *reinterpret_cast<const half8*>(pa + 8);
+        ga[2] = *reinterpret_cast<const half8*>(pl);
+        ga[3] = *reinterpret_cast<const half8*>(pl + 8);
+        gb[0] = *reinterpret_cast<const half8*>(pb);
+        gb[1] = *reinterpret_cast<const half8*>(pb + 8);
+        gb[2] = *reinterpret_cast<const half8*>(ql);
+        gb[3] = *reinterpret_cast<const half8*>(ql + 8);
+    };
+    load(0);
+    for (int k0 = 0; k0 < K; k0 += HK) {
+        *reinterpret_cast<half8*>(&Ah[r][kq]) = ga[0];
+        *reinterpret_cast<half8*>(&Ah[r][kq + 8]) = ga[1];
+        *reinterpret_cast<half8*>(&Al[r][kq]) = ga[2];
+        *reinterpret_cast<half8*>(&Al[r][kq + 8]) = ga[3];
+        *reinterpret_cast<half8*>(&Bh[r][kq]) = gb[0];
+        *reinterpret_cast<half8*>(&Bh[r][kq + 8]) = gb[1];
+        *reinterpret_cast<half8*>(&Bl[r][kq]) = gb[2];
+        *reinterpret_cast<half8*>(&Bl[r][kq + 8]) = gb[3];
+        __syncthreads();
+        if (k0 + HK < K) load(k0 + HK);
+#pragma unroll
+        for (int kk = 0; kk < HK; kk += 8) {
+            const int kc = kk + 4 * (lane >> 5);
+            half4 ah[MB], al[MB], bh[MB], bl[MB];
+#pragma unroll
+            for (int q = 0; q < MB; ++q) {
+                ah[q] = *reinterpret_cast<const half4*>(&Ah[wm + 32 * q + (lane & 31)][kc]);
+                al[q] = *reinterpret_cast<const half4*>(&Al[wm + 32 * q + (lane & 31)][kc]);
+                bh[q] = *reinterpret_cast<const half4*>(&Bh[wn + 32 * q + (lane & 31)][kc]);
+                bl[q] = *reinterpret_cast<const half4*>(&Bl[wn + 32 * q + (lane & 31)][kc]);
+            }
+#pragma unroll
+            for (int mi = 0; mi < MB; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < MB; ++ni) {
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x8f16(al[mi], bh[ni], acc[mi][ni], 0, 0, 0);
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x8f16(ah[mi], bl[ni], acc[mi][ni], 0, 0, 0);
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x8f16(ah[mi], bh[ni], acc[mi][ni], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+    const float cs = bsc[1];
+#pragma unroll
+    for (int mi = 0; mi < MB; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < MB; ++ni)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int row = bm + wm + mi * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                const int col = bn + wn + ni * 32 + (lane & 31);
+                C[(size_t)row * N + col] = acc[mi][ni][v] * cs;
+            }
+}
+
+// B [K][N] float -> Bhi, Blo [N][K] halves of B s, s = 2^(14 - e) for the
+// largest entry m = f 2^e (0.5 <= f < 1) of the pack's block maxima
+// pmax[0..npmax) (s = 1 when m is 0 or not finite), so every scaled entry is
+// below 2^14.  Entries outside the packed p < np, n < nv (padding, which may
+// hold an earlier face's values) are written as zeros.  64 x 64 tiles through
+// LDS: reads along n, writes along k.  Work-group 0 stores (s, 1 / s) to bsc.
+__global__ __launch_bounds__(256) void k_bsplit(const float* __restrict__ B, int K, int N, int np, int nv,
+                                               const float* __restrict__ pmax, int npmax,
+                                               _Float16* __restrict__ Bhi, _Float16* __restrict__ Blo,
+                                               float* __restrict__ bsc) {
+    __shared__ float red[256];
+    __shared__ float t[64][65];
+    const int tid = threadIdx.x;
+    float m = 0.f;
+    for (int q = tid; q < npmax; q += 256) m = fmaxf(m, pmax[q]);
+    red[tid] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) red[tid] = fmaxf(red[tid], red[tid + o]);
+        __syncthreads();
+    }
+    float sc = 1.f, inv = 1.f;
+    if (red[0] > 0.f && red[0] < 3.0e38f) {
+        int e = 0;
+        (void)frexpf(red[0], &e);
+        sc = ldexpf(1.f, 14 - e);
+        inv = ldexpf(1.f, e - 14);
+    }
+    if (blockIdx.x == 0 && blockIdx.y == 0 && tid == 0) {
+        bsc[0] = sc;
+        bsc[1] = inv;
+    }
+    const int k0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const int tx = tid & 63, ty = tid >> 6;
+    for (int y = ty; y < 64; y += 4) {
+        const int k = k0 + y, n = n0 + tx;
+        t[y][tx] = (k < np && n < nv) ? B[(size_t)k * N + n] * sc : 0.f;
+    }
+    __syncthreads();
+    for (int y = ty; y < 64; y += 4) {
+        const int n = n0 + y, k = k0 + tx;
+        if (n < N && k < K) {
+            const float x = t[tx][y];
+            const _Float16 h = (_Float16)x;
+            Bhi[(size_t)n * K + k] = h;
+            Blo[(size_t)n * K + k] = (_Float16)(x - (float)h);
+        }
+    }
+}
+
+// Phase rows of k_phase as fp16 halves (hi, lo of the float64 value).
+__global__ __launch_bounds__(256) void k_phase16(_Float16* __restrict__ Ahi, _Float16* __restrict__ Alo, int ld,
+                                                size_t total, int u0, int u1, int np, const double* __restrict__ uval,
+                                                const double* __restrict__ xp, double k) {
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(t / ld), i = (int)(t - (size_t)r * ld);
+        const int u = u0 + (r >> 1);
+        double v = 0.0;
+        if (u < u1 && i < np) {
+            double sn, cs;
+            sincos(k * uval[u] * xp[i], &sn, &cs);
+            v = (r & 1) ? sn : cs;
+        }
+        const _Float16 h = (_Float16)v;
+        Ahi[t] = h;
+        Alo[t] = (_Float16)(v - (double)h);
+    }
+}
+
 // 128 x 128 tiles when they give the launch at least 512 work-groups, else
 // 64 x 64 tiles (4x the work-groups for the short GEMMs of few distinct
 // direction cosines).
+// The large contractions on the FP16 matrix cores when split16 (operands
+// split into fp16 halves), the others in FP32.
+bool use_split16(int M, int N, bool split16) { return split16 && (M / 128) * (N / 128) >= 512; }
+
 hipError_t launch_gemm(const float* A, const float* B, float* C, int M, int N, int K, hipStream_t s) {
     if ((M / 128) * (N / 128) >= 512)
         hipLaunchKernelGGL(k_sgemm_mfma<128>, dim3(N / 128, M / 128), dim3(256), 0, s, A, B, C, M, N, K);
@@ -158,11 +333,13 @@ template <int KIND>
 __global__ __launch_bounds__(256) void k_nf2ff_pack(const void* __restrict__ src, double scale, int n1, int n2,
                                                     int swap, int sign, const double* __restrict__ wp,
                                                     const double* __restrict__ wq, float* __restrict__ B, int ldb,
-                                                    double* __restrict__ part) {
+                                                    double* __restrict__ part, float* __restrict__ pmax) {
     __shared__ double red[256];
+    __shared__ float redm[256];
     const int tid = threadIdx.x;
     const int q = blockIdx.x * 256 + tid;
     double p = 0.0;
+    float mx = 0.f;
     if (q < n1 * n2) {
         const int i1 = q / n2, i2 = q - i1 * n2;
         const int ip = swap ? i2 : i1, iq = swap ? i1 : i2, nq = swap ? n1 : n2;
@@ -185,18 +362,27 @@ __global__ __launch_bounds__(256) void k_nf2ff_pack(const void* __restrict__ src
         float* row = B + (size_t)ip * ldb;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            row[4 * iq + c] = (float)(w * cr[c]);
-            row[4 * nq + 4 * iq + c] = (float)(w * ci[c]);
+            const float br = (float)(w * cr[c]), bi = (float)(w * ci[c]);
+            row[4 * iq + c] = br;
+            row[4 * nq + 4 * iq + c] = bi;
+            mx = fmaxf(mx, fmaxf(fabsf(br), fabsf(bi)));
         }
         p = 0.5 * w * wq[iq] * sg * (vr[0] * vr[3] + vi[0] * vi[3] - vr[1] * vr[2] - vi[1] * vi[2]);
     }
     red[tid] = p;
+    redm[tid] = mx;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if (tid < o) red[tid] += red[tid + o];
+        if (tid < o) {
+            red[tid] += red[tid + o];
+            redm[tid] = fmaxf(redm[tid], redm[tid + o]);
+        }
         __syncthreads();
     }
-    if (tid == 0) part[blockIdx.x] = red[0];
+    if (tid == 0) {
+        part[blockIdx.x] = red[0];
+        pmax[blockIdx.x] = redm[0];
+    }
 }
 
 #ifndef REDUCE_M
@@ -540,7 +726,7 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
         std::vector<double> xp, xq;
     };
     std::vector<Plan> plans;
-    size_t Kmax = GBK, Nmax = GBN, coord_len = 0, blocks = 0;
+    size_t Kmax = KPAD, Nmax = GBN, coord_len = 0, blocks = 0;
     for (const auto& f : faces) {
         const int a1 = (f.axis + 1) % 3, a2 = (f.axis + 2) % 3;
         Plan pl{};
@@ -551,7 +737,7 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
         const auto& cq = pl.swap ? f.c1 : f.c2;
         pl.np = (int)cp.size();
         pl.nq = (int)cq.size();
-        pl.K = (int)round_up(pl.np, GBK);
+        pl.K = (int)round_up(pl.np, KPAD);
         pl.N = (int)round_up(8 * (size_t)pl.nq, GBN);
         for (double v : cp) pl.xp.push_back(v - center[pl.p]);
         for (double v : cq) pl.xq.push_back(v - center[pl.q]);
@@ -569,7 +755,9 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
 
     const size_t pblocks = (n_ang + 255) / 256;
     const size_t n_dbl = 12 * n_ang + 4 * n_ang + pblocks + coord_len + blocks;
-    const size_t bytes = round_up(n_dbl * sizeof(double), 256) + (Kmax * Nmax + rows * Kmax + rows * Nmax) * sizeof(float);
+    const size_t f_off = round_up(n_dbl * sizeof(double), 256);
+    const size_t bt_off = round_up(f_off + (Kmax * Nmax + rows * Kmax + rows * Nmax + blocks) * sizeof(float), 256);
+    const size_t bytes = bt_off + Kmax * Nmax * sizeof(float) + 2 * faces.size() * sizeof(float);
     if (C.work.bytes < bytes) {
         NF_OK(C.work.alloc(bytes));
         C.bm_off = SIZE_MAX;
@@ -595,6 +783,15 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
     }
     float* Am = Bm + Kmax * Nmax;
     float* G = Am + rows * Kmax;
+    float* d_pmax = G + rows * Nmax;  // the pack's block maxima of |B|, per face as d_part
+    // split fp16 halves of B (transposed, [N][K] each) and per face the scale
+    // of B and its inverse
+    _Float16* d_bt = reinterpret_cast<_Float16*>(reinterpret_cast<char*>(C.work.p) + bt_off);
+    float* d_bsc = reinterpret_cast<float*>(reinterpret_cast<char*>(C.work.p) + bt_off + Kmax * Nmax * sizeof(float));
+    // the large contractions on the FP16 matrix cores (operands split in two
+    // halves, k_hgemm3) unless OE_NF2FF_SPLIT16=0
+    bool split16 = true;
+    if (const char* v = getenv("OE_NF2FF_SPLIT16")) split16 = atoi(v) != 0;
 
     std::vector<double> coords;
     coords.reserve(coord_len);
@@ -623,6 +820,8 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
     EventPair* all = nullptr;
     NF_OK(next_events(&all));
     std::vector<EventPair*> gemm_ev;
+    std::vector<bool> gemm_split;  // launch ran the split FP16 kernel
+    double flop_split = 0.0;
     NF_OK(hipEventRecord(all->a, s));
     double flop = 0.0;
     size_t co = 0, bo = 0;
@@ -640,12 +839,27 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
         const int nb = (n1 * n2 + 255) / 256;
         if (f.src_kind == 0)
             hipLaunchKernelGGL(k_nf2ff_pack<0>, dim3(nb), dim3(256), 0, s, f.src, f.scale, n1, n2, pl.swap, f.sign,
-                               wp, wq, Bm, pl.N, d_part + bo);
+                               wp, wq, Bm, pl.N, d_part + bo, d_pmax + bo);
         else
             hipLaunchKernelGGL(k_nf2ff_pack<1>, dim3(nb), dim3(256), 0, s, f.src, f.scale, n1, n2, pl.swap, f.sign,
-                               wp, wq, Bm, pl.N, d_part + bo);
+                               wp, wq, Bm, pl.N, d_part + bo, d_pmax + bo);
         NF_OK(hipGetLastError());
+        const float* face_pmax = d_pmax + bo;
         bo += nb;
+        const int uchunk = (int)(rows / 2);
+        bool face16 = false;  // some contraction of this face runs on the FP16 matrix cores
+        for (int u0 = 0; u0 < A.U; u0 += uchunk) {
+            const int mp = (int)round_up(2 * (size_t)(std::min(A.U, u0 + uchunk) - u0), GBM);
+            face16 = face16 || use_split16(mp, pl.N, split16);
+        }
+        _Float16* Bhi = d_bt;
+        _Float16* Blo = d_bt + (size_t)pl.N * pl.K;
+        float* bsc = d_bsc + 2 * fi;
+        if (face16) {
+            hipLaunchKernelGGL(k_bsplit, dim3(pl.N / 64 + (pl.N % 64 != 0), (pl.K + 63) / 64), dim3(256), 0, s, Bm, pl.K,
+                               pl.N, pl.np, 8 * pl.nq, face_pmax, nb, Bhi, Blo, bsc);
+            NF_OK(hipGetLastError());
+        }
         double dxq = 0.0;
         const bool uni = uniform_spacing(pl.xq, &dxq);
         ReduceArgs ra{};
@@ -666,22 +880,37 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
         ra.k = k;
         ra.slot[0] = a1; ra.slot[1] = a2; ra.slot[2] = 3 + a1; ra.slot[3] = 3 + a2;
         ra.NL = d_nl;
-        const int uchunk = (int)(rows / 2);
         for (int u0 = 0; u0 < A.U; u0 += uchunk) {
             const int u1 = std::min(A.U, u0 + uchunk);
             const int mp = (int)round_up(2 * (size_t)(u1 - u0), GBM);
             const size_t total = (size_t)mp * pl.K;
             const int pb = (int)std::min<size_t>((total + 255) / 256, 16384);
-            hipLaunchKernelGGL(k_phase, dim3(pb), dim3(256), 0, s, Am, pl.K, total, u0, u1, pl.np, A.uval.as<double>(),
-                               xp, k);
+            const bool fp16 = use_split16(mp, pl.N, split16);
+            _Float16* Ahi = reinterpret_cast<_Float16*>(Am);
+            _Float16* Alo = Ahi + total;
+            if (fp16)
+                hipLaunchKernelGGL(k_phase16, dim3(pb), dim3(256), 0, s, Ahi, Alo, pl.K, total, u0, u1, pl.np,
+                                   A.uval.as<double>(), xp, k);
+            else
+                hipLaunchKernelGGL(k_phase, dim3(pb), dim3(256), 0, s, Am, pl.K, total, u0, u1, pl.np,
+                                   A.uval.as<double>(), xp, k);
             NF_OK(hipGetLastError());
             EventPair* ev = nullptr;
             NF_OK(next_events(&ev));
             gemm_ev.push_back(ev);
             NF_OK(hipEventRecord(ev->a, s));
-            NF_OK(launch_gemm(Am, Bm, G, mp, pl.N, pl.K, s));
+            if (fp16) {
+                hipLaunchKernelGGL(k_hgemm3, dim3(pl.N / 128, mp / 128), dim3(256), 0, s, Ahi, Alo, Bhi, Blo, G, mp, pl.N,
+                                   pl.K, bsc);
+                NF_OK(hipGetLastError());
+            } else {
+                NF_OK(launch_gemm(Am, Bm, G, mp, pl.N, pl.K, s));
+            }
             NF_OK(hipEventRecord(gemm_ev.back()->b, s));
-            flop += 2.0 * (2.0 * (u1 - u0)) * (8.0 * pl.nq) * pl.np;  // useful (unpadded) work
+            gemm_split.push_back(fp16);
+            const double fl = 2.0 * (2.0 * (u1 - u0)) * (8.0 * pl.nq) * pl.np;  // useful (unpadded) work
+            flop += fl;
+            if (fp16) flop_split += fl;
             ra.g0 = A.gstart[u0];
             ra.g1 = A.gstart[u1];
             ra.u0 = u0;
@@ -707,11 +936,14 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
     NF_OK(hipEventElapsedTime(&ms, all->a, all->b));
     out->total_ms = ms;
     out->gemm_ms = 0.0;
-    for (auto& e : gemm_ev) {
-        NF_OK(hipEventElapsedTime(&ms, e->a, e->b));
+    out->gemm_split_ms = 0.0;
+    for (size_t q = 0; q < gemm_ev.size(); ++q) {
+        NF_OK(hipEventElapsedTime(&ms, gemm_ev[q]->a, gemm_ev[q]->b));
         out->gemm_ms += ms;
+        if (gemm_split[q]) out->gemm_split_ms += ms;
     }
     out->gemm_flop = flop;
+    out->gemm_split_flop = flop_split;
     double prad = 0.0;
     for (size_t i = 0; i < blocks; ++i) prad += h[4 * n_ang + pblocks + i];
     out->prad = prad;

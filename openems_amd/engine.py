@@ -274,6 +274,8 @@ class FarFieldResult:
     gemm_ms: float
     total_ms: float
     gemm_flop: float
+    gemm_split_ms: float = 0.0     # the contractions run on the FP16 matrix cores (split operands)
+    gemm_split_flop: float = 0.0
 
     def directivity(self) -> np.ndarray:
         """|E|^2 normalised to its maximum (FarFieldResult::directivity, src/nf2ff/mod.rs:28-55)."""
@@ -307,7 +309,7 @@ def _far_field(call, frequency, theta, phi, radius, center, split: bool = False)
     if split:
         et, ep = bufs[0] + 1j * bufs[1], bufs[2] + 1j * bufs[3]
     return FarFieldResult(th, ph, et, ep, float(frequency), out.prad, out.dmax, out.gemm_ms, out.total_ms,
-                          out.gemm_flop)
+                          out.gemm_flop, out.gemm_split_ms, out.gemm_split_flop)
 
 
 class Nf2ff:
