@@ -328,7 +328,7 @@ def bench_nf2ff(args):
                                "0.5 mm cells, 181x360 angles, 5/10/15 GHz",
                    "device_ms_per_frequency": round(total_ms / n_calc, 3)},
         "roofline": ({"bound": "mfma",
-                      "kernel": "k_hgemm3 (3 x v_mfma_f32_32x32x8f16 per FP32 product, operands split hi + lo)",
+                      "kernel": "k_hgemm3 (3 x v_mfma_f32_32x32x16_f16 per FP32 product, operands split hi + lo)",
                       # matrix-core work: three fp16 products per useful FP32 multiply-add
                       "achieved": round(3 * split_flop / (split_ms * 1e-3) / 1e12, 2), "peak": 2500.0,
                       "unit": "TFLOP/s", "frac": round(3 * split_flop / (split_ms * 1e-3) / 1e12 / 2500.0, 4),

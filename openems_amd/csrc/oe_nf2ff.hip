@@ -126,12 +126,11 @@ __global__ __launch_bounds__(256) void k_sgemm_mfma(const float* __restrict__ A,
 // given transposed, [N][K] (Bh then Bl), scaled by a power of two whose
 // inverse bsc[1] multiplies C at the end (exact).  Tiles of 128 x 128, K
 // steps of 32 (K a multiple of 32), 4 waves of 2 x 2 blocks of
-// v_mfma_f32_32x32x8f16: lane l takes
-// A[l % 32][4 (l / 32) + j] and B[4 (l / 32) + j][l % 32], j = 0..3, and
-// holds the 32x32 accumulator as the FP32 MFMA does.  In LDS the tiles keep
-// k fastest (rows of 40 halves), so each lane reads its 4 halves with one
-// 8-byte load, and every global load is a 16-byte copy.
-typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+// v_mfma_f32_32x32x16_f16 (gfx950): lane l takes A[l % 32][8 (l / 32) + j]
+// and B[8 (l / 32) + j][l % 32], j = 0..7, and holds the 32x32 accumulator
+// as the FP32 MFMA does.  In LDS the tiles keep k fastest (rows of 40
+// halves), so each lane reads its 8 halves with one 16-byte load, and every
+// global load is a 16-byte copy.
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 constexpr int HK = 32;    // K step of the split FP16 GEMM
 constexpr int HROW = 40;  // LDS row stride in halves: 32 of K plus 8 of padding
@@ -185,23 +184,23 @@ __global__ __launch_bounds__(256) void k_hgemm3(const _Float16* __restrict__ Ahi
         __syncthreads();
         if (k0 + HK < K) load(k0 + HK);
 #pragma unroll
-        for (int kk = 0; kk < HK; kk += 8) {
-            const int kc = kk + 4 * (lane >> 5);
-            half4 ah[MB], al[MB], bh[MB], bl[MB];
+        for (int kk = 0; kk < HK; kk += 16) {
+            const int kc = kk + 8 * (lane >> 5);
+            half8 ah[MB], al[MB], bh[MB], bl[MB];
 #pragma unroll
             for (int q = 0; q < MB; ++q) {
-                ah[q] = *reinterpret_cast<const half4*>(&Ah[wm + 32 * q + (lane & 31)][kc]);
-                al[q] = *reinterpret_cast<const half4*>(&Al[wm + 32 * q + (lane & 31)][kc]);
-                bh[q] = *reinterpret_cast<const half4*>(&Bh[wn + 32 * q + (lane & 31)][kc]);
-                bl[q] = *reinterpret_cast<const half4*>(&Bl[wn + 32 * q + (lane & 31)][kc]);
+                ah[q] = *reinterpret_cast<const half8*>(&Ah[wm + 32 * q + (lane & 31)][kc]);
+                al[q] = *reinterpret_cast<const half8*>(&Al[wm + 32 * q + (lane & 31)][kc]);
+                bh[q] = *reinterpret_cast<const half8*>(&Bh[wn + 32 * q + (lane & 31)][kc]);
+                bl[q] = *reinterpret_cast<const half8*>(&Bl[wn + 32 * q + (lane & 31)][kc]);
             }
 #pragma unroll
             for (int mi = 0; mi < MB; ++mi)
 #pragma unroll
                 for (int ni = 0; ni < MB; ++ni) {
-                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x8f16(al[mi], bh[ni], acc[mi][ni], 0, 0, 0);
-                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x8f16(ah[mi], bl[ni], acc[mi][ni], 0, 0, 0);
-                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x8f16(ah[mi], bh[ni], acc[mi][ni], 0, 0, 0);
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[mi], bh[ni], acc[mi][ni], 0, 0, 0);
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[mi], bl[ni], acc[mi][ni], 0, 0, 0);
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[mi], bh[ni], acc[mi][ni], 0, 0, 0);
                 }
         }
         __syncthreads();
