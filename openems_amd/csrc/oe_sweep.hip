@@ -59,7 +59,7 @@ struct oe_sweep {
     // its coefficient arrays, so every structure does)
     int rows_h = 0, rows_e = 0;
     double tune_ms[4] = {0, 0, 0, 0};
-    int tune_ci[4] = {2, 4, 8, 16};
+    int tune_ci[4] = {1, 2, 4, 8};
     // device state of a run
     DevBuf structs, tables, excs, exc_first, exc_samples, dfts, ids, partials, energy, step;
     double* energy_host = nullptr;
