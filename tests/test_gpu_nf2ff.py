@@ -202,3 +202,38 @@ def test_fdtd_dipole_fp32_kahan_far_field():
             f"P_rad rel {abs(gb.prad / ga.prad - 1):.2e}")
         assert e <= PATTERN_TOL
         assert abs(gb.prad / ga.prad - 1) < 1e-5
+
+
+def test_split_fp16_contraction_against_oracle_and_fp32(monkeypatch):
+    """z faces of 100 x 300 nodes and 181 x 180 irregular angles: the large
+    contractions run on the FP16 matrix cores with split operands over
+    several K steps (K = 128 or 320).  The far field must match the float64
+    oracle to the pattern tolerance and the FP32 contraction closely."""
+    f = 10e9
+    lam = N.C0 / f
+    faces = N.box_faces([-1.2 * lam, -2 * lam, -0.1 * lam], [1.2 * lam, 2 * lam, 0.1 * lam], (100, 300, 6),
+                        lambda X_, Y, Z: N.hertzian_dipole_fields(1e-3, f, X_ + 0.002, Y - 0.001, Z))
+    rng = np.random.default_rng(13)
+    th = np.sort(rng.uniform(0, math.pi, 181))
+    ph = np.sort(rng.uniform(0, 2 * math.pi, 180))
+    out = {}
+    for split in ("1", "0"):
+        monkeypatch.setenv("OE_NF2FF_SPLIT16", split)
+        nf = Nf2ff(f)
+        for fc in faces:
+            a1, a2 = (fc.axis + 1) % 3, (fc.axis + 2) % 3
+            nf.add_surface(fc.axis, fc.sign, fc.c0, fc.c1, fc.c2, fc.E[[a1, a2]], fc.H[[a1, a2]])
+        out[split] = nf.calculate(th, ph)
+    g, g32 = out["1"], out["0"]
+    assert g.gemm_split_ms > 0 and g32.gemm_split_ms == 0
+    sub = slice(0, 181, 30)
+    Et, Ep, prad, _ = N.far_field(faces, f, th[sub], ph)
+    err = math.sqrt((np.linalg.norm(g.e_theta[sub] - Et) ** 2 + np.linalg.norm(g.e_phi[sub] - Ep) ** 2)
+                    / (np.linalg.norm(Et) ** 2 + np.linalg.norm(Ep) ** 2))
+    d32 = math.sqrt((np.linalg.norm(g.e_theta - g32.e_theta) ** 2 + np.linalg.norm(g.e_phi - g32.e_phi) ** 2)
+                    / (np.linalg.norm(g32.e_theta) ** 2 + np.linalg.norm(g32.e_phi) ** 2))
+    say(f"[nf2ff] split fp16, 100x300x6 box: rel L2 {err:.2e} vs oracle, {d32:.2e} vs fp32; split gemm "
+        f"{g.gemm_split_ms:.3f} ms of {g.gemm_ms:.3f} ms ({g.gemm_split_flop / 1e9:.1f} GFLOP)")
+    assert err <= PATTERN_TOL
+    assert d32 <= 1e-6
+    assert abs(g.prad - prad) <= 1e-12 * abs(prad)
