@@ -268,28 +268,49 @@ __global__ __launch_bounds__(256) void k_bsplit(const float* __restrict__ B, int
     }
 }
 
-// Phase rows of k_phase as fp16 halves (hi, lo of the float64 value).
+// Phase rows of k_phase as fp16 halves (hi, lo of the float64 value).  One
+// thread per 8 consecutive columns of a row (ld is a multiple of 8); on a
+// uniform grid (uniform, xp[i] = xp[0] + i dx) the thread takes one sincos
+// and steps the other 7 columns by angle addition in float64.
 __global__ __launch_bounds__(256) void k_phase16(_Float16* __restrict__ Ahi, _Float16* __restrict__ Alo, int ld,
                                                 size_t total, int u0, int u1, int np, const double* __restrict__ uval,
-                                                const double* __restrict__ xp, double k) {
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const int r = (int)(t / ld), i = (int)(t - (size_t)r * ld);
+                                                const double* __restrict__ xp, double k, int uniform, double dx) {
+    const size_t groups = total / 8;
+    for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups; g += (size_t)gridDim.x * blockDim.x) {
+        const size_t t0 = 8 * g;
+        const int r = (int)(t0 / ld), i0 = (int)(t0 - (size_t)r * ld);
         const int u = u0 + (r >> 1);
-        double v = 0.0;
-        if (u < u1 && i < np) {
-            double sn, cs;
-            sincos(k * uval[u] * xp[i], &sn, &cs);
-            v = (r & 1) ? sn : cs;
+        const bool odd = (r & 1) != 0;
+        half8 h, l;
+        if (u < u1 && i0 < np) {
+            const double ku = k * uval[u];
+            double sn, cs, sd = 0.0, cd = 1.0;
+            sincos(ku * xp[i0], &sn, &cs);
+            if (uniform) sincos(ku * dx, &sd, &cd);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                double v = 0.0;
+                if (i0 + q < np) {
+                    if (!uniform && q > 0) sincos(ku * xp[i0 + q], &sn, &cs);
+                    v = odd ? sn : cs;
+                }
+                h[q] = (_Float16)v;
+                l[q] = (_Float16)(v - (double)h[q]);
+                if (uniform) {
+                    const double c2 = cs * cd - sn * sd;
+                    sn = sn * cd + cs * sd;
+                    cs = c2;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) h[q] = l[q] = (_Float16)0.f;
         }
-        const _Float16 h = (_Float16)v;
-        Ahi[t] = h;
-        Alo[t] = (_Float16)(v - (double)h);
+        *reinterpret_cast<half8*>(Ahi + t0) = h;
+        *reinterpret_cast<half8*>(Alo + t0) = l;
     }
 }
 
-// 128 x 128 tiles when they give the launch at least 512 work-groups, else
-// 64 x 64 tiles (4x the work-groups for the short GEMMs of few distinct
-// direction cosines).
 // The large contractions on the FP16 matrix cores when split16 (operands
 // split into fp16 halves), the others in FP32.
 bool use_split16(int M, int N, bool split16) { return split16 && (M / 128) * (N / 128) >= 512; }
@@ -859,8 +880,9 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
                                pl.N, pl.np, 8 * pl.nq, face_pmax, nb, Bhi, Blo, bsc);
             NF_OK(hipGetLastError());
         }
-        double dxq = 0.0;
+        double dxq = 0.0, dxp = 0.0;
         const bool uni = uniform_spacing(pl.xq, &dxq);
+        const bool uni_p = uniform_spacing(pl.xp, &dxp);
         ReduceArgs ra{};
         ra.G = G;
         ra.ldg = pl.N;
@@ -888,8 +910,8 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
             _Float16* Ahi = reinterpret_cast<_Float16*>(Am);
             _Float16* Alo = Ahi + total;
             if (fp16)
-                hipLaunchKernelGGL(k_phase16, dim3(pb), dim3(256), 0, s, Ahi, Alo, pl.K, total, u0, u1, pl.np,
-                                   A.uval.as<double>(), xp, k);
+                hipLaunchKernelGGL(k_phase16, dim3(std::max(1, pb / 8)), dim3(256), 0, s, Ahi, Alo, pl.K, total, u0, u1,
+                                   pl.np, A.uval.as<double>(), xp, k, uni_p ? 1 : 0, dxp);
             else
                 hipLaunchKernelGGL(k_phase, dim3(pb), dim3(256), 0, s, Am, pl.K, total, u0, u1, pl.np,
                                    A.uval.as<double>(), xp, k);
