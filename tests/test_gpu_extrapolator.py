@@ -101,3 +101,41 @@ def test_spread_modes_and_tail_curriculum():
     assert not np.array_equal(ex.params(), before)
     err = LS.tail_error(ex, X6, T6, horizons=(16, SMALL.tail))
     assert all(np.isfinite(v) for v in err.values())
+
+
+def _damped_modes(n, rng, L0=64, tail=64):
+    """V and I windows of one damped mode each (decay 0.005-0.03, frequency
+    0.03-0.15 cycles per sample, random complex amplitude), divided by the
+    window peak, and the next `tail` samples."""
+    t = np.arange(L0 + tail)
+    X = np.zeros((n, 2, L0), np.float32)
+    Y = np.zeros((n, 2, tail), np.float32)
+    for q in range(n):
+        a, f = rng.uniform(0.005, 0.03), rng.uniform(0.03, 0.15)
+        for ch in range(2):
+            x = np.real((rng.normal() + 1j * rng.normal()) * np.exp((-a + 2j * np.pi * f) * t))
+            sc = np.abs(x[:L0]).max()
+            X[q, ch] = x[:L0] / sc
+            Y[q, ch] = x[L0:] / sc
+    return X, Y
+
+
+def test_training_gains_over_stopping_on_damped_modes():
+    """What the trained network gains over stopping without extrapolating:
+    on held-out ringdowns of the kind it was trained on (one damped mode per
+    signal), its predicted continuation must leave well under a fifth of the
+    squared error of predicting zero, where the untrained network does not."""
+    from openems_amd import learned_stop as LS
+    rng = np.random.default_rng(0)
+    X, Y = _damped_modes(2048, rng)
+    Xh, Yh = _damped_modes(256, rng)
+    dims = ExtrapolatorDims(L0=64, batch=256, tail=64)
+    ex = Extrapolator(dims, seed=0)
+    LS.spread_modes(ex)
+    before = LS.tail_error(ex, Xh, Yh, horizons=(16, 64))
+    losses = LS.train(ex, X, Y, 60, 1e-3)
+    after = LS.tail_error(ex, Xh, Yh, horizons=(16, 64))
+    say(f"[extrap] held-out tail error / zero-prediction error: untrained {before}, trained {after}; "
+        f"loss {losses[0]:.3e} -> {losses[-1]:.3e}")
+    assert after[64] < 0.2 and after[16] < 0.2
+    assert after[64] < 0.5 * before[64]
