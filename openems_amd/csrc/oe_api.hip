@@ -301,6 +301,7 @@ oe_status engine_alloc(oe_engine* e, size_t nx, size_t ny, size_t nz, int device
     if (const char* v = getenv("OE_GRAPH_STEPS")) e->graph_steps = std::max(0, atoi(v));
     if (const char* v = getenv("OE_OVERLAP_SLABS")) e->overlap_slabs = atoi(v) != 0;
     if (const char* v = getenv("OE_FOLD_MUR")) e->fold_mur = atoi(v) != 0;
+    if (const char* v = getenv("OE_MERGE_ADE")) e->merge_ade = atoi(v) != 0;
     return OE_OK;
 }
 
@@ -968,6 +969,8 @@ oe_status oe_ext_ade_create(oe_engine* e, int kind, const double params[3], cons
         if (c == 0 || (int)k < x->ade_kmin) x->ade_kmin = (int)k;
         if ((int)k > x->ade_kmax) x->ade_kmax = (int)k;
     }
+    x->ade_cells = idx;
+    std::sort(x->ade_cells.begin(), x->ade_cells.end());
     AdeDev& a = x->ade;
     a.kind = kind; a.mode = mode; a.n = (int)n;
     const double dt = e->dt;
@@ -1717,6 +1720,24 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             if (mz.on[1] && x->ade_kmax >= e->g.nz - 2) fold_mur = false;
         }
     if (!fold_mur) mz = MurZ{};
+    // Dispersive extensions whose cells are disjoint share one pre and one
+    // post launch per step (no cell is updated by two of them, so their list
+    // order does not matter).
+    bool merge_ade = e->merge_ade && ades.size() >= 2 && ades.size() <= (size_t)MAX_MERGE_ADE;
+    if (merge_ade) {
+        std::vector<std::pair<uint32_t, int>> all;
+        for (size_t q = 0; q < ades.size(); ++q)
+            for (uint32_t c : ades[q]->ade_cells) all.push_back({c, (int)q});
+        std::sort(all.begin(), all.end());
+        for (size_t q = 1; q < all.size() && merge_ade; ++q)
+            if (all[q].first == all[q - 1].first && all[q].second != all[q - 1].second) merge_ade = false;
+    }
+    AdeMulti ade_pre_m{}, ade_post_m{};
+    if (merge_ade)
+        for (auto* x : ades) {
+            if (x->ade.mode == OE_MODE_PHYSICAL) ade_pre_m.a[ade_pre_m.n++] = x->ade;
+            ade_post_m.a[ade_post_m.n++] = x->ade;
+        }
     std::vector<Launch> plan_e = plan;  // the E launches carry the folded Mur faces
     for (Launch& L : plan_e) L.a.mur_z = mz;
     // Two-half-step batches advance the device step counter in their first H
@@ -1833,11 +1854,18 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             Timed tm(e, "mur_pre");
             HIP_OK(launch_mur_pre(e->F, e->g, x->mur_dev.as<MurDev>(), (int)x->mur_faces.size(), e->stream));
         }
-        for (auto* x : ades)
-            if (x->ade.mode == OE_MODE_PHYSICAL) {
+        if (merge_ade) {
+            if (ade_pre_m.n) {
                 Timed tm(e, "ade_pre");
-                HIP_OK(launch_ade_pre(e->F, e->g.N, x->ade, e->stream));
+                HIP_OK(launch_ade_multi(e->F, e->Cf, e->g.N, ade_pre_m, true, e->stream));
             }
+        } else {
+            for (auto* x : ades)
+                if (x->ade.mode == OE_MODE_PHYSICAL) {
+                    Timed tm(e, "ade_pre");
+                    HIP_OK(launch_ade_pre(e->F, e->g.N, x->ade, e->stream));
+                }
+        }
         if (upml) {
             Timed tm(e, "upml");
             HIP_OK(launch_upml_pre(e->F, e->g, upml->upml, true, e->stream));
@@ -1863,9 +1891,14 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             Timed tm(e, "upml");
             HIP_OK(launch_upml_post(e->F, e->g, upml->upml, true, e->stream));
         }
-        for (auto* x : ades) {
+        if (merge_ade) {
             Timed tm(e, "ade_post");
-            HIP_OK(launch_ade_post(e->F, e->Cf, e->g.N, x->ade, e->stream));
+            HIP_OK(launch_ade_multi(e->F, e->Cf, e->g.N, ade_post_m, false, e->stream));
+        } else {
+            for (auto* x : ades) {
+                Timed tm(e, "ade_post");
+                HIP_OK(launch_ade_post(e->F, e->Cf, e->g.N, x->ade, e->stream));
+            }
         }
         for (auto* x : murs) {
             if (fold_mur) break;

@@ -170,21 +170,23 @@ __global__ void k_mur_apply(float* __restrict__ F, Grid g, const MurDev* __restr
     }
 }
 
-__global__ void k_ade_pre(const float* __restrict__ F, size_t N, AdeDev a) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= a.n) return;
+__device__ __forceinline__ void ade_pre_cell(const float* __restrict__ F, size_t N, const AdeDev& a, int c) {
     const size_t p = a.idx[c];
 #pragma unroll
     for (int d = 0; d < 3; ++d) a.Eprev[(size_t)d * a.n + c] = F[(size_t)d * N + p];
+}
+
+__global__ void k_ade_pre(const float* __restrict__ F, size_t N, AdeDev a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < a.n) ade_pre_cell(F, N, a, c);
 }
 
 // Dispersive post_update_e.  Compat: the reference bodies
 // (src/extensions/dispersive.rs:171-196, 510-535, 604-630).  Physical: the
 // trapezoidal Drude/Debye and second-order Lorentz schemes, with
 // K = Cb * delta = dt / eps_cell / (1 + loss) taken from the cell.
-__global__ void k_ade_post(float* __restrict__ F, const float* __restrict__ Cf, size_t N, AdeDev a) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= a.n) return;
+__device__ __forceinline__ void ade_post_cell(float* __restrict__ F, const float* __restrict__ Cf, size_t N,
+                                              const AdeDev& a, int c) {
     const size_t p = a.idx[c];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -230,6 +232,26 @@ __global__ void k_ade_post(float* __restrict__ F, const float* __restrict__ Cf, 
             }
         }
     }
+}
+
+__global__ void k_ade_post(float* __restrict__ F, const float* __restrict__ Cf, size_t N, AdeDev a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < a.n) ade_post_cell(F, Cf, N, a, c);
+}
+
+// Several dispersive extensions whose cells are disjoint in one launch each
+// for the pre and the post step: blockIdx.y selects the extension (their
+// order then does not matter).
+__global__ void k_ade_pre_multi(const float* __restrict__ F, size_t N, AdeMulti m) {
+    const AdeDev& a = m.a[blockIdx.y];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < a.n) ade_pre_cell(F, N, a, c);
+}
+
+__global__ void k_ade_post_multi(float* __restrict__ F, const float* __restrict__ Cf, size_t N, AdeMulti m) {
+    const AdeDev& a = m.a[blockIdx.y];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < a.n) ade_post_cell(F, Cf, N, a, c);
 }
 
 // One thread per signal: gather the float64 signal, record it, and update
@@ -573,6 +595,17 @@ hipError_t launch_dft_dev(const float* F, size_t N, const DftDev& d, const uint6
                           double t0, double dt, hipStream_t s) {
     hipLaunchKernelGGL(k_dft, dim3((d.n_sig + 255) / 256), dim3(256), 0, s, F, N, d, sample0, 0.0, 0.0, step_dev, t0,
                        dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_ade_multi(float* F, const float* Cf, size_t N, const AdeMulti& m, bool pre, hipStream_t s) {
+    if (m.n <= 0 || m.n > MAX_MERGE_ADE) return hipErrorInvalidValue;
+    int cells = 0;
+    for (int q = 0; q < m.n; ++q) cells = std::max(cells, m.a[q].n);
+    if (cells == 0) return hipSuccess;
+    const dim3 grid((cells + 255) / 256, m.n);
+    if (pre) hipLaunchKernelGGL(k_ade_pre_multi, grid, dim3(256), 0, s, F, N, m);
+    else hipLaunchKernelGGL(k_ade_post_multi, grid, dim3(256), 0, s, F, Cf, N, m);
     return hipGetLastError();
 }
 

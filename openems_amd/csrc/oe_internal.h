@@ -272,6 +272,14 @@ struct AdeDev {
 };
 hipError_t launch_ade_pre(const float* F, size_t N, const AdeDev& a, hipStream_t s);
 hipError_t launch_ade_post(float* F, const float* Cf, size_t N, const AdeDev& a, hipStream_t s);
+// Up to MAX_MERGE_ADE extensions with disjoint cells in one launch (pre: the
+// E^n capture, else the post-E step), passed by value.
+constexpr int MAX_MERGE_ADE = 8;
+struct AdeMulti {
+    int n;
+    AdeDev a[MAX_MERGE_ADE];
+};
+hipError_t launch_ade_multi(float* F, const float* Cf, size_t N, const AdeMulti& m, bool pre, hipStream_t s);
 
 // UPML (src/extensions/pml.rs): up to 6 regions whose cells are numbered
 // consecutively (region r holds cells first[r] .. first[r+1]); arrays are

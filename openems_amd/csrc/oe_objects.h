@@ -84,6 +84,7 @@ struct oe_extension {
     AdeDev ade{};
     DevBuf ade_mem;
     int ade_kmin = 0, ade_kmax = -1;  // k range of the cells (empty: kmax < kmin)
+    std::vector<uint32_t> ade_cells;   // the cells' linear indices, sorted
     // probe
     uint32_t probe_idx = 0;
     DevBuf probe_mem;   // data
@@ -148,6 +149,9 @@ struct oe_engine {
     // z-face Mur applied by the E kernels where exact (oe_engine_run_batch
     // checks the conditions): 1 on, 0 off
     int fold_mur = 1;
+    // dispersive extensions with disjoint cells in one pre and one post
+    // launch per step: 1 on, 0 off
+    int merge_ade = 1;
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     DevBuf exc_table;

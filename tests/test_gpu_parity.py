@@ -264,3 +264,25 @@ def test_extension_outliving_engine_is_detached():
     with pytest.raises(OEError):
         cp.reset()
     del cp, pr  # oe_extension_destroy on detached handles
+
+
+@pytest.mark.parametrize("overlap", [False, True], ids=["disjoint", "overlapping"])
+def test_ade_merged_launches(overlap, monkeypatch):
+    """Two dispersive extensions share one pre and one post launch when their
+    cells are disjoint; with overlapping cells the engine keeps one launch per
+    extension in list order.  Both match the oracle, merged or not."""
+    shape = (30, 30, 30)
+    c2 = (17, 15, 15) if overlap else (21, 21, 15)
+    ades = [(ADE_DRUDE, (1e11, 1e9, 0.0), _sphere((12, 12, 15), 4, shape), MODE_PHYSICAL),
+            (ADE_LORENTZ, (2 * math.pi * 10e9, 1e9, 3.0), _sphere(c2, 5, shape), MODE_PHYSICAL)]
+    out = []
+    for merge in ("1", "0"):
+        monkeypatch.setenv("OE_MERGE_ADE", merge)
+        sc = Scenario(f"ade_merge{merge}_{'ov' if overlap else 'dj'}", shape, 0.5e-3, 160, ades=ades,
+                      excitations=[((15, 15, 15), 2, True, gauss_ref(10e9))], probes=[(12, 12, 15)])
+        res = run_pair(sc)
+        assert_parity(res, sc.name)
+        for g, o in zip(res["gpu_ade"], res["oracle_ade"]):
+            assert np.array_equal(g, o.astype(np.float32))
+        out.append(np.concatenate(res["eng"].read_fields()))
+    assert np.array_equal(out[0], out[1])
