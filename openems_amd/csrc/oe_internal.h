@@ -210,6 +210,7 @@ struct SweepArgs {
     const SweepBox* box;
     const int* wg_box;
     int xcd_remap;
+    uint64_t* step_inc;  // non-null on the step's first H launch: lane 0 advances the step counter
 };
 // h: H (else E) half-step over n_wg work-groups; axes 0, AX_Z, AX_X|AX_Z,
 // AX_Y|AX_Z or all faces.  rows > 0: every structure reads its material

@@ -180,13 +180,17 @@ oe_status build_tables(oe_sweep* sw, const std::vector<int>& active) {
     return OE_OK;
 }
 
-hipError_t yee_half(oe_sweep* sw, bool h) {
+// step_inc (H half only): the first launch advances this step counter.
+hipError_t yee_half(oe_sweep* sw, bool h, uint64_t* step_inc = nullptr) {
+    bool first = true;
     for (const auto& G : sw->groups) {
         SweepArgs a{};
         a.st = sw->structs.as<SweepStruct>();
         a.box = G.d_box;
         a.wg_box = G.d_wg;
         a.xcd_remap = sw->xcd_remap;
+        a.step_inc = (h && first) ? step_inc : nullptr;
+        first = false;
         hipError_t e = launch_sweep(h, a, G.n_wg, G.axes, G.vec, h ? sw->rows_h : sw->rows_e, sw->stream);
         if (e != hipSuccess) return e;
     }
@@ -238,14 +242,13 @@ oe_status build_step_tables(oe_sweep* sw, const std::vector<int>& active, StepTa
 // One timestep of all active structures (no host synchronisation; capturable).
 hipError_t sweep_step(oe_sweep* sw, const StepTables& t, double dt) {
     hipError_t e;
-    if ((e = yee_half(sw, true)) != hipSuccess) return e;
+    // the first H launch advances the step counter (from -1: step st reads st)
+    if ((e = yee_half(sw, true, sw->step.as<uint64_t>())) != hipSuccess) return e;
     if ((e = yee_half(sw, false)) != hipSuccess) return e;
     if ((e = launch_sweep_exc(sw->excs.as<SweepExc>(), sw->exc_first.as<int>(), t.n_struct, sw->step.as<uint64_t>(),
                               sw->stream)) != hipSuccess)
         return e;
-    if ((e = launch_sweep_dft(sw->dfts.as<SweepDft>(), t.n_dft, sw->step.as<uint64_t>(), dt, sw->stream)) != hipSuccess)
-        return e;
-    return launch_sweep_step(sw->step.as<uint64_t>(), sw->stream);
+    return launch_sweep_dft(sw->dfts.as<SweepDft>(), t.n_dft, sw->step.as<uint64_t>(), dt, sw->stream);
 }
 
 oe_status capture(oe_sweep* sw, const StepTables& t, double dt, int steps) {
@@ -402,7 +405,7 @@ oe_status oe_sweep_run(oe_sweep* sw, const oe_sweep_config* cfg, oe_sweep_entry*
                                       it.samples[q].size() * sizeof(float), hipMemcpyHostToDevice, sw->stream));
             off += it.samples[q].size();
         }
-    HIP_OK(hipMemsetAsync(sw->step.p, 0, sizeof(uint64_t), sw->stream));
+    HIP_OK(hipMemsetAsync(sw->step.p, 0xff, sizeof(uint64_t), sw->stream));  // -1: the first step's H launch makes it 0
 
     std::vector<int> active(n);
     for (int s = 0; s < n; ++s) active[s] = s;

@@ -673,6 +673,7 @@ __device__ __forceinline__ bool decode_sweep(const SweepArgs& a, Tile& r, int& s
 template <int V, int AX, bool PHYS, bool TAB>
 __global__ __launch_bounds__(256) void k_update_h_sweep(SweepArgs a) {
     extern __shared__ float2 lds_rows[];
+    if (a.step_inc && blockIdx.x == 0 && threadIdx.x == 0) *a.step_inc += 1;
     Tile t;
     int sid;
     decode_sweep<V>(a, t, sid);
