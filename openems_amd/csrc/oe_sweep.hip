@@ -438,7 +438,8 @@ oe_status oe_sweep_run(oe_sweep* sw, const oe_sweep_config* cfg, oe_sweep_entry*
                     HIP_OK(yee_half(sw, false));
                 }
                 HIP_OK(hipEventRecord(a, sw->stream));
-                for (int w = 0; w < 6; ++w) {
+                constexpr int TUNE_STEPS = 12;  // 6 left candidates within 1% to chance
+                for (int w = 0; w < TUNE_STEPS; ++w) {
                     HIP_OK(yee_half(sw, true));
                     HIP_OK(yee_half(sw, false));
                 }
@@ -446,7 +447,7 @@ oe_status oe_sweep_run(oe_sweep* sw, const oe_sweep_config* cfg, oe_sweep_entry*
                 HIP_OK(hipEventSynchronize(b));
                 float ms = 0.f;
                 HIP_OK(hipEventElapsedTime(&ms, a, b));
-                sw->tune_ms[q] = ms / 6.0;
+                sw->tune_ms[q] = ms / TUNE_STEPS;
                 if (ms < best) {
                     best = ms;
                     best_ci = sw->tune_ci[q];
