@@ -32,6 +32,7 @@ namespace oe {
 namespace {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int GBM = 128, GBN = 128, GBK = 16;  // M, N padding and K step of the FP32 GEMM
 constexpr int KPAD = 32;                        // K padding (a multiple of both GEMMs' K steps)
@@ -67,28 +68,36 @@ __global__ __launch_bounds__(256) void k_sgemm_mfma(const float* __restrict__ A,
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
 
-    float4 ra[LV], rb[LV];
-    auto load = [&](int k0) {
-        const float4* sa = reinterpret_cast<const float4*>(A + (size_t)(bm + ar) * K + k0 + ac);
-        const float4* sb = reinterpret_cast<const float4*>(B + (size_t)(k0 + br) * N + bn + bc);
+    floatx4 ra[LV], rb[LV];
+    const floatx4* sa = reinterpret_cast<const floatx4*>(A + (size_t)(bm + ar) * K + ac);
+    const floatx4* sb = reinterpret_cast<const floatx4*>(B + (size_t)br * N + bn + bc);
+#pragma unroll
+    for (int q = 0; q < LV; ++q) {
+        ra[q] = sa[q];
+        rb[q] = sb[q];
+    }
+    for (int k0 = 0; k0 < K; k0 += GBK) {
+#pragma unroll
+        for (int q = 0; q < LV; ++q) {
+            As[ac + 4 * q + 0][ar] = ra[q][0];
+            As[ac + 4 * q + 1][ar] = ra[q][1];
+            As[ac + 4 * q + 2][ar] = ra[q][2];
+            As[ac + 4 * q + 3][ar] = ra[q][3];
+            *reinterpret_cast<floatx4*>(&Bs[br][bc + 4 * q]) = rb[q];
+        }
+        __syncthreads();
+        // prefetch the next K slice; the last step re-reads its own slice
+        // (in bounds) so that the loads stay unconditional and ra/rb stay in
+        // registers
+        if (k0 + GBK < K) {
+            sa += GBK / 4;
+            sb += (size_t)GBK * N / 4;
+        }
 #pragma unroll
         for (int q = 0; q < LV; ++q) {
             ra[q] = sa[q];
             rb[q] = sb[q];
         }
-    };
-    load(0);
-    for (int k0 = 0; k0 < K; k0 += GBK) {
-#pragma unroll
-        for (int q = 0; q < LV; ++q) {
-            As[ac + 4 * q + 0][ar] = ra[q].x;
-            As[ac + 4 * q + 1][ar] = ra[q].y;
-            As[ac + 4 * q + 2][ar] = ra[q].z;
-            As[ac + 4 * q + 3][ar] = ra[q].w;
-            *reinterpret_cast<float4*>(&Bs[br][bc + 4 * q]) = rb[q];
-        }
-        __syncthreads();
-        if (k0 + GBK < K) load(k0 + GBK);
 #pragma unroll
         for (int kk = 0; kk < GBK; kk += 2) {
             const int kr = kk + (lane >> 5);
