@@ -78,7 +78,14 @@ CELL_BYTES = {
     # field's 3 components read, and the coefficients
     "arrays": 60,   # 6 coefficient floats (the reference's 12-array layout)
     "table": 38,    # one 2-byte material-table row index
+    "table1": 36,   # a table of one row: the kernels read no index
 }
+
+
+def half_coef(coef, rows, h):
+    """Byte class of one half ('update_h' when h) from the engine's material
+    table rows (E, H): a tabulated half of one row reads no row index."""
+    return "table1" if coef == "table" and rows[1 if h else 0] == 1 else coef
 
 
 def kernel_bytes(shape, layers, coef="arrays"):
@@ -97,18 +104,22 @@ FUSED_CELL_BYTES = {
 }
 
 
-def launch_bytes(name, shape, layers, coef="arrays"):
+def launch_bytes(name, shape, layers, coef="arrays", rows=(0, 0)):
     """Algorithmic bytes of one launch of the plan: 'grid' (no CPML), or
     'interior' (x/y slabs excluded; z-face psi at the row ends) and 'slabs'
     (the x/y face slabs with all faces) when a CPML is attached.  A fused
     launch ('fused:interior', 'fused:grid') covers both half-steps of its
-    box: FUSED_CELL_BYTES per cell, 32 B per z-slab cell (H and E psi)."""
+    box: FUSED_CELL_BYTES per cell, 32 B per z-slab cell (H and E psi).
+    rows: the material table's rows of E and H (a half of one row reads no
+    index)."""
     nx, ny, nz = shape
     kind, part = name.split(":")
     if kind == "fused":
         ix, jy = (nx - 2 * layers, ny - 2 * layers) if part == "interior" else (nx, ny)
         zl = 2 * layers if layers and 2 * layers < nz else 0
-        return FUSED_CELL_BYTES[coef] * ix * jy * nz + 32 * ix * jy * zl
+        fb = FUSED_CELL_BYTES[coef] - (2 * sum(r == 1 for r in rows) if coef == "table" else 0)
+        return fb * ix * jy * nz + 32 * ix * jy * zl
+    coef = half_coef(coef, rows, kind == "update_h")
     if part == "grid" or not layers:
         return kernel_bytes(shape, 0, coef)
     ix, jy = nx - 2 * layers, ny - 2 * layers
@@ -698,7 +709,7 @@ def main():
     yee = [k for k in kern if k.startswith("update_") or k.startswith("fused:")]
     dom = max(yee, key=lambda k: kern[k][0] * kern[k][1])
     avg_ms = kern[dom][1]
-    kb = launch_bytes(dom, wl.shape, wl.cpml_layers, coef)
+    kb = launch_bytes(dom, wl.shape, wl.cpml_layers, coef, rows)
     # the reference's layout and scheme: 12 coefficient arrays, two in-place
     # half-steps (a fused launch counts as both halves of its box)
     kb_ref = launch_bytes(dom.replace("fused:", "update_h:"), wl.shape, wl.cpml_layers, "arrays")
@@ -729,12 +740,14 @@ def main():
     nx, ny, nz = wl.shape
     L = wl.cpml_layers
     conv_bytes = 120 * wl.cells + 32 * 2 * L * (ny * nz + nx * nz + nx * ny)
-    step_bytes = 2 * kernel_bytes(wl.shape, wl.cpml_layers, coef)
+    step_bytes = (kernel_bytes(wl.shape, wl.cpml_layers, half_coef(coef, rows, True))
+                  + kernel_bytes(wl.shape, wl.cpml_layers, half_coef(coef, rows, False)))
     if "fused:interior" in kern:
-        step_bytes = (launch_bytes("fused:interior", wl.shape, wl.cpml_layers, coef)
-                      + 2 * launch_bytes("update_h:slabs", wl.shape, wl.cpml_layers, coef))
+        step_bytes = (launch_bytes("fused:interior", wl.shape, wl.cpml_layers, coef, rows)
+                      + launch_bytes("update_h:slabs", wl.shape, wl.cpml_layers, coef, rows)
+                      + launch_bytes("update_e:slabs", wl.shape, wl.cpml_layers, coef, rows))
     elif "fused:grid" in kern:
-        step_bytes = launch_bytes("fused:grid", wl.shape, wl.cpml_layers, coef)
+        step_bytes = launch_bytes("fused:grid", wl.shape, wl.cpml_layers, coef, rows)
     out = {
         "metric": METRIC,
         "value": round(mcells, 1),

@@ -115,19 +115,24 @@ __device__ __forceinline__ Vec<V> zerov() {
     return r;
 }
 
-// Material table (oe_coef.hip): V row indices of a lane (nontemporal, one
-// 8-byte load for V = 4) and the rows' (a, b) pairs of x, y, z from LDS.
+// Material table (oe_coef.hip): V row indices of a lane at cell p
+// (nontemporal, one 8-byte load for V = 4) and the rows' (a, b) pairs of x,
+// y, z from LDS.  mat == nullptr stands for a table of one row: every index
+// is 0 and none is read (2 bytes per cell less).
 typedef unsigned int u2v __attribute__((ext_vector_type(2)));
 
 template <int V>
-__device__ __forceinline__ void coef_rows(const uint16_t* __restrict__ mat, const float2* tab, Vec<V>& a0,
-                                          Vec<V>& a1, Vec<V>& a2, Vec<V>& b0, Vec<V>& b1, Vec<V>& b2) {
+__device__ __forceinline__ void coef_rows(const uint16_t* __restrict__ mat, size_t p, const float2* tab,
+                                          Vec<V>& a0, Vec<V>& a1, Vec<V>& a2, Vec<V>& b0, Vec<V>& b1, Vec<V>& b2) {
     unsigned m[V];
-    if constexpr (V == 4) {
-        const u2v t = __builtin_nontemporal_load(reinterpret_cast<const u2v*>(mat));
+    if (mat == nullptr) {
+#pragma unroll
+        for (int e = 0; e < V; ++e) m[e] = 0;
+    } else if constexpr (V == 4) {
+        const u2v t = __builtin_nontemporal_load(reinterpret_cast<const u2v*>(mat + p));
         m[0] = t.x & 0xffffu; m[1] = t.x >> 16; m[2] = t.y & 0xffffu; m[3] = t.y >> 16;
     } else {
-        m[0] = *mat;
+        m[0] = mat[p];
     }
 #pragma unroll
     for (int e = 0; e < V; ++e) {
@@ -322,7 +327,7 @@ __device__ __forceinline__ void h_body(const Grid& g, const float* Fin, float* F
         Vec<V> hx = ldn<V, OE_NT_OWN>(Hx + p), hy = ldn<V, OE_NT_OWN>(Hy + p), hz = ldn<V, OE_NT_OWN>(Hz + p);
         Vec<V> a0, a1, a2, b0, b1, b2;
         if constexpr (TAB) {
-            coef_rows<V>(mat + p, tab, a0, a1, a2, b0, b1, b2);
+            coef_rows<V>(mat, p, tab, a0, a1, a2, b0, b1, b2);
         } else {
             a0 = ldn<V, OE_NT_COEF>(da0 + p); a1 = ldn<V, OE_NT_COEF>(da1 + p); a2 = ldn<V, OE_NT_COEF>(da2 + p);
             b0 = ldn<V, OE_NT_COEF>(db0 + p); b1 = ldn<V, OE_NT_COEF>(db1 + p); b2 = ldn<V, OE_NT_COEF>(db2 + p);
@@ -488,7 +493,7 @@ __device__ __forceinline__ void e_body(const Grid& g, const float* Fin, float* F
         if (row) {
             ex = ldn<V, OE_NT_OWN>(Ex + p); ey = ldn<V, OE_NT_OWN>(Ey + p); ez = ldn<V, OE_NT_OWN>(Ez + p);
             if constexpr (TAB) {
-                coef_rows<V>(mat + p, tab, s0, s1, s2, c0, c1, c2);
+                coef_rows<V>(mat, p, tab, s0, s1, s2, c0, c1, c2);
             } else {
                 s0 = ldn<V, OE_NT_COEF>(ca0 + p); s1 = ldn<V, OE_NT_COEF>(ca1 + p); s2 = ldn<V, OE_NT_COEF>(ca2 + p);
                 c0 = ldn<V, OE_NT_COEF>(cb0 + p); c1 = ldn<V, OE_NT_COEF>(cb1 + p); c2 = ldn<V, OE_NT_COEF>(cb2 + p);
@@ -611,7 +616,7 @@ __global__ __launch_bounds__(256) void k_update_h(YeeArgs a, CpmlArgs c) {
         for (int q = threadIdx.x; q < 3 * a.n_mat_h; q += blockDim.x) lds_rows[q] = src[q];
         __syncthreads();
     }
-    h_body<V, AX, PHYS, TAB>(a.g, a.Fin, a.Fout, a.Cf, a.mat_h, lds_rows, c, decode<V>(a));
+    h_body<V, AX, PHYS, TAB>(a.g, a.Fin, a.Fout, a.Cf, a.n_mat_h == 1 ? nullptr : a.mat_h, lds_rows, c, decode<V>(a));
 }
 
 template <int V, int AX, bool PHYS, bool TAB, bool MUR = false>
@@ -622,16 +627,17 @@ __global__ __launch_bounds__(256) void k_update_e(YeeArgs a, CpmlArgs c) {
         for (int q = threadIdx.x; q < 3 * a.n_mat_e; q += blockDim.x) lds_rows[q] = src[q];
         __syncthreads();
     }
+    const uint16_t* mat_e = a.n_mat_e == 1 ? nullptr : a.mat_e;
     if constexpr (MUR) {
         // sources follow the Mur step, so they are never folded with it
-        e_body<V, AX, PHYS, TAB, true>(a.g, a.Fin, a.Fout, a.Cf, a.mat_e, lds_rows, c, decode<V>(a), nullptr, a.mur_z);
+        e_body<V, AX, PHYS, TAB, true>(a.g, a.Fin, a.Fout, a.Cf, mat_e, lds_rows, c, decode<V>(a), nullptr, a.mur_z);
         return;
     }
     if (a.n_exc) {
         const Sources src{a.exc, a.exc_idx, a.n_exc, a.exc_step, a.exc_step_dev};
-        e_body<V, AX, PHYS, TAB>(a.g, a.Fin, a.Fout, a.Cf, a.mat_e, lds_rows, c, decode<V>(a), &src);
+        e_body<V, AX, PHYS, TAB>(a.g, a.Fin, a.Fout, a.Cf, mat_e, lds_rows, c, decode<V>(a), &src);
     } else {
-        e_body<V, AX, PHYS, TAB>(a.g, a.Fin, a.Fout, a.Cf, a.mat_e, lds_rows, c, decode<V>(a));
+        e_body<V, AX, PHYS, TAB>(a.g, a.Fin, a.Fout, a.Cf, mat_e, lds_rows, c, decode<V>(a));
     }
 }
 
@@ -683,7 +689,7 @@ __global__ __launch_bounds__(256) void k_update_h_sweep(SweepArgs a) {
         for (int q = threadIdx.x; q < 3 * S.n_mat_h; q += blockDim.x) lds_rows[q] = src[q];
         __syncthreads();
     }
-    h_body<V, AX, PHYS, TAB>(S.g, S.F, S.F, S.Cf, S.mat_h, lds_rows, S.c, t);
+    h_body<V, AX, PHYS, TAB>(S.g, S.F, S.F, S.Cf, S.n_mat_h == 1 ? nullptr : S.mat_h, lds_rows, S.c, t);
 }
 
 template <int V, int AX, bool PHYS, bool TAB>
@@ -698,7 +704,7 @@ __global__ __launch_bounds__(256) void k_update_e_sweep(SweepArgs a) {
         for (int q = threadIdx.x; q < 3 * S.n_mat_e; q += blockDim.x) lds_rows[q] = src[q];
         __syncthreads();
     }
-    e_body<V, AX, PHYS, TAB>(S.g, S.F, S.F, S.Cf, S.mat_e, lds_rows, S.c, t);
+    e_body<V, AX, PHYS, TAB>(S.g, S.F, S.F, S.Cf, S.n_mat_e == 1 ? nullptr : S.mat_e, lds_rows, S.c, t);
 }
 
 template <int V, int AX, bool PHYS>
@@ -845,7 +851,7 @@ __global__ __launch_bounds__((FUSED_ROWS + 1) * 64) void k_fused(FusedArgs a, Cp
         hz = ldn<V, OE_NT_OWN>(Hzo + p);
         Vec<V> a0, a1, a2, b0, b1, b2;
         if constexpr (TAB) {
-            coef_rows<V>(a.mat_h + p, tabh, a0, a1, a2, b0, b1, b2);
+            coef_rows<V>(a.n_mat_h == 1 ? nullptr : a.mat_h, p, tabh, a0, a1, a2, b0, b1, b2);
         } else {
             a0 = ldn<V, OE_NT_COEF>(a.Cf + 6 * N + p); a1 = ldn<V, OE_NT_COEF>(a.Cf + 7 * N + p);
             a2 = ldn<V, OE_NT_COEF>(a.Cf + 8 * N + p); b0 = ldn<V, OE_NT_COEF>(a.Cf + 9 * N + p);
@@ -912,7 +918,7 @@ __global__ __launch_bounds__((FUSED_ROWS + 1) * 64) void k_fused(FusedArgs a, Cp
         Psi<V> pe;
         auto e_loads = [&]() {
             if constexpr (TAB) {
-                coef_rows<V>(a.mat_e + p, tabe, s0, s1, s2, c0, c1, c2);
+                coef_rows<V>(a.n_mat_e == 1 ? nullptr : a.mat_e, p, tabe, s0, s1, s2, c0, c1, c2);
             } else {
                 s0 = ldn<V, OE_NT_COEF>(a.Cf + p); s1 = ldn<V, OE_NT_COEF>(a.Cf + N + p);
                 s2 = ldn<V, OE_NT_COEF>(a.Cf + 2 * N + p); c0 = ldn<V, OE_NT_COEF>(a.Cf + 3 * N + p);
