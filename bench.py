@@ -78,14 +78,23 @@ CELL_BYTES = {
     # field's 3 components read, and the coefficients
     "arrays": 60,   # 6 coefficient floats (the reference's 12-array layout)
     "table": 38,    # one 2-byte material-table row index
+    "table8": 37,   # a table of 2 to 256 rows: a 1-byte row index
     "table1": 36,   # a table of one row: the kernels read no index
 }
 
 
 def half_coef(coef, rows, h):
     """Byte class of one half ('update_h' when h) from the engine's material
-    table rows (E, H): a tabulated half of one row reads no row index."""
-    return "table1" if coef == "table" and rows[1 if h else 0] == 1 else coef
+    table rows (E, H): a tabulated half of one row reads no row index, one
+    of at most 256 rows reads it as a byte (unless OE_BYTE_INDEX=0)."""
+    r = rows[1 if h else 0]
+    if coef != "table":
+        return coef
+    if r == 1:
+        return "table1"
+    if r <= 256 and os.environ.get("OE_BYTE_INDEX", "1") != "0":
+        return "table8"
+    return "table"
 
 
 def kernel_bytes(shape, layers, coef="arrays"):

@@ -112,6 +112,10 @@ YeeArgs launch_args(const oe_engine* e, const Tiles& t, const std::vector<Box>& 
         a.tab_h = e->mtab + (size_t)MAX_MAT_ROWS * 6;
         a.n_mat_e = e->n_mat[0];
         a.n_mat_h = e->n_mat[1];
+        if (e->mat8) {
+            if (e->n_mat[0] >= 2 && e->n_mat[0] <= 256) a.mat8_e = e->mat8;
+            if (e->n_mat[1] >= 2 && e->n_mat[1] <= 256) a.mat8_h = e->mat8 + e->g.N;
+        }
     }
     a.t = t;
     const int nzv = (e->g.nz + t.vec - 1) / t.vec;
@@ -302,14 +306,17 @@ oe_status engine_alloc(oe_engine* e, size_t nx, size_t ny, size_t nz, int device
     if (const char* v = getenv("OE_OVERLAP_SLABS")) e->overlap_slabs = atoi(v) != 0;
     if (const char* v = getenv("OE_FOLD_MUR")) e->fold_mur = atoi(v) != 0;
     if (const char* v = getenv("OE_MERGE_ADE")) e->merge_ade = atoi(v) != 0;
+    if (const char* v = getenv("OE_BYTE_INDEX")) e->byte_index = atoi(v) != 0;
     return OE_OK;
 }
 
 void drop_material_table(oe_engine* e) {
     if (e->mat) (void)hipFree(e->mat);
     if (e->mtab) (void)hipFree(e->mtab);
+    if (e->mat8) (void)hipFree(e->mat8);
     e->mat = nullptr;
     e->mtab = nullptr;
+    e->mat8 = nullptr;
     e->n_mat[0] = e->n_mat[1] = 0;
 }
 
@@ -331,6 +338,14 @@ oe_status build_material_tables(oe_engine* e) {
     }
     e->n_mat[0] = n[0];
     e->n_mat[1] = n[1];
+    // halves of 2 to 256 rows also get their indices as bytes (1 B per cell
+    // less per launch)
+    const bool narrow[2] = {e->byte_index && n[0] >= 2 && n[0] <= 256, e->byte_index && n[1] >= 2 && n[1] <= 256};
+    if (narrow[0] || narrow[1]) {
+        HIP_OK(hipMalloc(&e->mat8, 2 * N));
+        for (int h = 0; h < 2; ++h)
+            if (narrow[h]) HIP_OK(narrow_material_index(e->mat + (size_t)h * N, e->mat8 + (size_t)h * N, N, e->stream));
+    }
     return OE_OK;
 }
 
@@ -357,6 +372,7 @@ void engine_free(oe_engine* e) {
     if (e->Cf) (void)hipFree(e->Cf);
     if (e->mat) (void)hipFree(e->mat);
     if (e->mtab) (void)hipFree(e->mtab);
+    if (e->mat8) (void)hipFree(e->mat8);
     if (e->partials) (void)hipFree(e->partials);
     if (e->energy_dev) (void)hipFree(e->energy_dev);
     if (e->energy_host) (void)hipHostFree(e->energy_host);

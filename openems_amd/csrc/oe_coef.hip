@@ -193,4 +193,18 @@ done:
     return err;
 }
 
+namespace {
+__global__ void k_narrow(const uint16_t* __restrict__ in, uint8_t* __restrict__ out, size_t n) {
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        out[q] = (uint8_t)in[q];
+}
+}  // namespace
+
+hipError_t narrow_material_index(const uint16_t* in, uint8_t* out, size_t n, hipStream_t s) {
+    const int grid = (int)std::min<size_t>((n + 255) / 256, 16384);
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_narrow, dim3(grid), dim3(256), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
 }  // namespace oe

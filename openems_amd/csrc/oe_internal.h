@@ -109,6 +109,9 @@ struct YeeArgs {
     // half-steps; a half whose row count is 0 reads the coefficient arrays.
     const uint16_t* mat_h;
     const uint16_t* mat_e;
+    // the same indices one byte each, for a half of 2 to 256 rows (else null)
+    const uint8_t* mat8_h;
+    const uint8_t* mat8_e;
     const float* tab_h;
     const float* tab_e;
     int n_mat_h, n_mat_e;
@@ -147,6 +150,8 @@ constexpr int MAX_MAT_ROWS = 1024;
 // distinct coefficient sets.
 hipError_t build_material_table(const float* Cf, size_t N, bool h, int cap, uint16_t* mat, float* tab,
                                 int* n_rows, hipStream_t s);
+// Copies n 16-bit row indices of a table of at most 256 rows to bytes.
+hipError_t narrow_material_index(const uint16_t* in, uint8_t* out, size_t n, hipStream_t s);
 
 // Fused step (oe_yee.hip): one pass over a box computes H^{n+1/2} and
 // E^{n+1} of every cell, reading the old fields from Fo and writing the new

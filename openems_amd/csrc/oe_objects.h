@@ -125,6 +125,7 @@ struct oe_engine {
     uint16_t* mat = nullptr;
     float* mtab = nullptr;
     int n_mat[2] = {0, 0};  // rows of the E and H tables; 0: that half reads Cf
+    uint8_t* mat8 = nullptr;  // [E: N][H: N] the indices as bytes, when a half has 2 to 256 rows
     uint64_t timestep = 0;
     Tiles tiles{};
     int xcd_remap = 1;
@@ -152,6 +153,8 @@ struct oe_engine {
     // dispersive extensions with disjoint cells in one pre and one post
     // launch per step: 1 on, 0 off
     int merge_ade = 1;
+    // byte-wide row indices for tables of at most 256 rows: 1 on, 0 off
+    int byte_index = 1;
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     DevBuf exc_table;

@@ -115,24 +115,37 @@ __device__ __forceinline__ Vec<V> zerov() {
     return r;
 }
 
-// Material table (oe_coef.hip): V row indices of a lane at cell p
-// (nontemporal, one 8-byte load for V = 4) and the rows' (a, b) pairs of x,
-// y, z from LDS.  mat == nullptr stands for a table of one row: every index
-// is 0 and none is read (2 bytes per cell less).
+// Material table (oe_coef.hip): V row indices of a lane at cell p and the
+// rows' (a, b) pairs of x, y, z from LDS.  The index is read from m8 (one
+// byte per cell, tables of at most 256 rows) when present, else from m16
+// (nontemporal, one 8-byte load for V = 4); neither stands for a table of
+// one row, whose indices are all 0 and are not read.
 typedef unsigned int u2v __attribute__((ext_vector_type(2)));
 
+struct MatIndex {
+    const uint16_t* m16;
+    const uint8_t* m8;
+};
+
 template <int V>
-__device__ __forceinline__ void coef_rows(const uint16_t* __restrict__ mat, size_t p, const float2* tab,
-                                          Vec<V>& a0, Vec<V>& a1, Vec<V>& a2, Vec<V>& b0, Vec<V>& b1, Vec<V>& b2) {
+__device__ __forceinline__ void coef_rows(const MatIndex& mi, size_t p, const float2* tab, Vec<V>& a0,
+                                          Vec<V>& a1, Vec<V>& a2, Vec<V>& b0, Vec<V>& b1, Vec<V>& b2) {
     unsigned m[V];
-    if (mat == nullptr) {
+    if (mi.m8 != nullptr) {
+        if constexpr (V == 4) {
+            const unsigned t = __builtin_nontemporal_load(reinterpret_cast<const unsigned*>(mi.m8 + p));
+            m[0] = t & 0xffu; m[1] = (t >> 8) & 0xffu; m[2] = (t >> 16) & 0xffu; m[3] = t >> 24;
+        } else {
+            m[0] = mi.m8[p];
+        }
+    } else if (mi.m16 == nullptr) {
 #pragma unroll
         for (int e = 0; e < V; ++e) m[e] = 0;
     } else if constexpr (V == 4) {
-        const u2v t = __builtin_nontemporal_load(reinterpret_cast<const u2v*>(mat + p));
+        const u2v t = __builtin_nontemporal_load(reinterpret_cast<const u2v*>(mi.m16 + p));
         m[0] = t.x & 0xffffu; m[1] = t.x >> 16; m[2] = t.y & 0xffffu; m[3] = t.y >> 16;
     } else {
-        m[0] = mat[p];
+        m[0] = mi.m16[p];
     }
 #pragma unroll
     for (int e = 0; e < V; ++e) {
@@ -268,7 +281,7 @@ __device__ __forceinline__ void psi_store(const Psi<V>& s) {
 // material table (mat, tab in LDS) instead of the arrays in Cf.
 template <int V, int AX, bool PHYS, bool TAB>
 __device__ __forceinline__ void h_body(const Grid& g, const float* Fin, float* Fout, const float* __restrict__ Cf,
-                                       const uint16_t* __restrict__ mat, const float2* tab,
+                                       const MatIndex mat, const float2* tab,
                                        const CpmlArgs& c, const Tile& t) {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     if (t.k0 >= nz || t.j >= t.jend) return;
@@ -432,7 +445,7 @@ __device__ __forceinline__ void apply_sources(const Sources& src, size_t p, Vec<
 // Mur post step (the host checks it).
 template <int V, int AX, bool PHYS, bool TAB, bool MUR = false>
 __device__ __forceinline__ void e_body(const Grid& g, const float* Fin, float* Fout, const float* __restrict__ Cf,
-                                       const uint16_t* __restrict__ mat, const float2* tab,
+                                       const MatIndex mat, const float2* tab,
                                        const CpmlArgs& c, const Tile& t, const Sources* src = nullptr,
                                        const MurZ& mz = MurZ{}) {
     static_assert(!MUR || V == 4, "the folded Mur needs float4 lanes");
@@ -616,7 +629,8 @@ __global__ __launch_bounds__(256) void k_update_h(YeeArgs a, CpmlArgs c) {
         for (int q = threadIdx.x; q < 3 * a.n_mat_h; q += blockDim.x) lds_rows[q] = src[q];
         __syncthreads();
     }
-    h_body<V, AX, PHYS, TAB>(a.g, a.Fin, a.Fout, a.Cf, a.n_mat_h == 1 ? nullptr : a.mat_h, lds_rows, c, decode<V>(a));
+    h_body<V, AX, PHYS, TAB>(a.g, a.Fin, a.Fout, a.Cf, MatIndex{a.n_mat_h == 1 ? nullptr : a.mat_h, a.mat8_h}, lds_rows, c,
+                             decode<V>(a));
 }
 
 template <int V, int AX, bool PHYS, bool TAB, bool MUR = false>
@@ -627,7 +641,7 @@ __global__ __launch_bounds__(256) void k_update_e(YeeArgs a, CpmlArgs c) {
         for (int q = threadIdx.x; q < 3 * a.n_mat_e; q += blockDim.x) lds_rows[q] = src[q];
         __syncthreads();
     }
-    const uint16_t* mat_e = a.n_mat_e == 1 ? nullptr : a.mat_e;
+    const MatIndex mat_e{a.n_mat_e == 1 ? nullptr : a.mat_e, a.mat8_e};
     if constexpr (MUR) {
         // sources follow the Mur step, so they are never folded with it
         e_body<V, AX, PHYS, TAB, true>(a.g, a.Fin, a.Fout, a.Cf, mat_e, lds_rows, c, decode<V>(a), nullptr, a.mur_z);
@@ -689,7 +703,7 @@ __global__ __launch_bounds__(256) void k_update_h_sweep(SweepArgs a) {
         for (int q = threadIdx.x; q < 3 * S.n_mat_h; q += blockDim.x) lds_rows[q] = src[q];
         __syncthreads();
     }
-    h_body<V, AX, PHYS, TAB>(S.g, S.F, S.F, S.Cf, S.n_mat_h == 1 ? nullptr : S.mat_h, lds_rows, S.c, t);
+    h_body<V, AX, PHYS, TAB>(S.g, S.F, S.F, S.Cf, MatIndex{S.n_mat_h == 1 ? nullptr : S.mat_h, nullptr}, lds_rows, S.c, t);
 }
 
 template <int V, int AX, bool PHYS, bool TAB>
@@ -704,7 +718,7 @@ __global__ __launch_bounds__(256) void k_update_e_sweep(SweepArgs a) {
         for (int q = threadIdx.x; q < 3 * S.n_mat_e; q += blockDim.x) lds_rows[q] = src[q];
         __syncthreads();
     }
-    e_body<V, AX, PHYS, TAB>(S.g, S.F, S.F, S.Cf, S.n_mat_e == 1 ? nullptr : S.mat_e, lds_rows, S.c, t);
+    e_body<V, AX, PHYS, TAB>(S.g, S.F, S.F, S.Cf, MatIndex{S.n_mat_e == 1 ? nullptr : S.mat_e, nullptr}, lds_rows, S.c, t);
 }
 
 template <int V, int AX, bool PHYS>
@@ -851,7 +865,7 @@ __global__ __launch_bounds__((FUSED_ROWS + 1) * 64) void k_fused(FusedArgs a, Cp
         hz = ldn<V, OE_NT_OWN>(Hzo + p);
         Vec<V> a0, a1, a2, b0, b1, b2;
         if constexpr (TAB) {
-            coef_rows<V>(a.n_mat_h == 1 ? nullptr : a.mat_h, p, tabh, a0, a1, a2, b0, b1, b2);
+            coef_rows<V>(MatIndex{a.n_mat_h == 1 ? nullptr : a.mat_h, nullptr}, p, tabh, a0, a1, a2, b0, b1, b2);
         } else {
             a0 = ldn<V, OE_NT_COEF>(a.Cf + 6 * N + p); a1 = ldn<V, OE_NT_COEF>(a.Cf + 7 * N + p);
             a2 = ldn<V, OE_NT_COEF>(a.Cf + 8 * N + p); b0 = ldn<V, OE_NT_COEF>(a.Cf + 9 * N + p);
@@ -918,7 +932,7 @@ __global__ __launch_bounds__((FUSED_ROWS + 1) * 64) void k_fused(FusedArgs a, Cp
         Psi<V> pe;
         auto e_loads = [&]() {
             if constexpr (TAB) {
-                coef_rows<V>(a.n_mat_e == 1 ? nullptr : a.mat_e, p, tabe, s0, s1, s2, c0, c1, c2);
+                coef_rows<V>(MatIndex{a.n_mat_e == 1 ? nullptr : a.mat_e, nullptr}, p, tabe, s0, s1, s2, c0, c1, c2);
             } else {
                 s0 = ldn<V, OE_NT_COEF>(a.Cf + p); s1 = ldn<V, OE_NT_COEF>(a.Cf + N + p);
                 s2 = ldn<V, OE_NT_COEF>(a.Cf + 2 * N + p); c0 = ldn<V, OE_NT_COEF>(a.Cf + 3 * N + p);

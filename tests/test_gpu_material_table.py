@@ -7,7 +7,9 @@ bit-identical with the table on, with it off, and to the FP32 oracle:
     ingredients), table on and off;
   * the drop-in path (coefficient arrays handed over whole), including
     grids with more distinct coefficient sets than the table holds, where
-    one or both half-steps fall back to the arrays.
+    one or both half-steps fall back to the arrays;
+  * the row index read as bytes (tables of 2 to 256 rows, the default) and
+    as 16-bit words (OE_BYTE_INDEX=0, and every table of 257 rows or more).
 """
 import numpy as np
 import pytest
@@ -104,7 +106,7 @@ def _coeff_sets(rng, n_e, n_h, dt):
     return c.reshape((12,) + SHAPE)
 
 
-@pytest.mark.parametrize("n_e,n_h", [(7, 3), (5000, 4), (5000, 5000)])
+@pytest.mark.parametrize("n_e,n_h", [(7, 3), (300, 600), (5000, 4), (5000, 5000)])
 def test_drop_in_arrays(n_e, n_h):
     rng = np.random.default_rng(n_e + n_h)
     sim = O.OracleSim(*SHAPE, DELTA, 32)
@@ -136,3 +138,23 @@ def test_table_toggle_keeps_state():
     ref = HipEngine(op)
     b = _run(ref, [], smp, 60)
     assert np.array_equal(a, b)
+
+
+def test_byte_index_equals_16bit(monkeypatch):
+    rng = np.random.default_rng(11)
+    mats = _boxes(rng, 30)
+    steps = 60
+    outs = []
+    for byte_index in ("1", "0"):
+        monkeypatch.setenv("OE_BYTE_INDEX", byte_index)
+        op = Operator.uniform(*SHAPE, DELTA)
+        for m in mats:
+            op.set_material(*m)
+        eng = HipEngine(op)
+        rows = eng.material_table(-1)
+        say(f"[table] byte index {byte_index}: rows (E, H) = {rows}")
+        assert 2 <= rows[0] <= 256 and 2 <= rows[1] <= 256
+        cp = eng.cpml(CpmlBoundaries(layers=(8,) * 6, mode=MODE_PHYSICAL))
+        outs.append(_run(eng, [cp], _source(op.dt, steps), steps))
+    assert np.all(np.isfinite(outs[0]))
+    assert np.array_equal(outs[0], outs[1])
