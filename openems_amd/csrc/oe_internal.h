@@ -198,6 +198,8 @@ struct SweepStruct {
     // every structure has one
     const uint16_t* mat_h;
     const uint16_t* mat_e;
+    const uint8_t* mat8_h;  // byte-wide indices (2 to 256 rows) or null
+    const uint8_t* mat8_e;
     const float* tab_h;
     const float* tab_e;
     int n_mat_h, n_mat_e;

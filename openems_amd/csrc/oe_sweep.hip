@@ -379,6 +379,10 @@ oe_status oe_sweep_run(oe_sweep* sw, const oe_sweep_config* cfg, oe_sweep_entry*
             st[s].tab_h = it.eng->mtab + (size_t)MAX_MAT_ROWS * 6;
             st[s].n_mat_e = it.eng->n_mat[0];
             st[s].n_mat_h = it.eng->n_mat[1];
+            if (it.eng->mat8) {
+                if (st[s].n_mat_e >= 2 && st[s].n_mat_e <= 256) st[s].mat8_e = it.eng->mat8;
+                if (st[s].n_mat_h >= 2 && st[s].n_mat_h <= 256) st[s].mat8_h = it.eng->mat8 + it.eng->g.N;
+            }
         }
         for (const auto& v : it.samples) total += v.size();
     }

@@ -703,7 +703,7 @@ __global__ __launch_bounds__(256) void k_update_h_sweep(SweepArgs a) {
         for (int q = threadIdx.x; q < 3 * S.n_mat_h; q += blockDim.x) lds_rows[q] = src[q];
         __syncthreads();
     }
-    h_body<V, AX, PHYS, TAB>(S.g, S.F, S.F, S.Cf, MatIndex{S.n_mat_h == 1 ? nullptr : S.mat_h, nullptr}, lds_rows, S.c, t);
+    h_body<V, AX, PHYS, TAB>(S.g, S.F, S.F, S.Cf, MatIndex{S.n_mat_h == 1 ? nullptr : S.mat_h, S.mat8_h}, lds_rows, S.c, t);
 }
 
 template <int V, int AX, bool PHYS, bool TAB>
@@ -718,7 +718,7 @@ __global__ __launch_bounds__(256) void k_update_e_sweep(SweepArgs a) {
         for (int q = threadIdx.x; q < 3 * S.n_mat_e; q += blockDim.x) lds_rows[q] = src[q];
         __syncthreads();
     }
-    e_body<V, AX, PHYS, TAB>(S.g, S.F, S.F, S.Cf, MatIndex{S.n_mat_e == 1 ? nullptr : S.mat_e, nullptr}, lds_rows, S.c, t);
+    e_body<V, AX, PHYS, TAB>(S.g, S.F, S.F, S.Cf, MatIndex{S.n_mat_e == 1 ? nullptr : S.mat_e, S.mat8_e}, lds_rows, S.c, t);
 }
 
 template <int V, int AX, bool PHYS>
