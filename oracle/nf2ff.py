@@ -2,7 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY (see oracle/oe_oracle.cpp header).
 
-Follows matlab/AnalyzeNF2FF.m (the reference's Rust nf2ff is a stub):
+Follows matlab/AnalyzeNF2FF.m (the reference's Rust integrate_surfaces,
+src/nf2ff/mod.rs:157-170, returns zeros: a placeholder):
   outward normals n (:76-81), J = n x H, M = -n x E (:186-191),
   e_fkt = exp(+j k r_hat . r') (:192-193), N_theta, N_phi, L_theta, L_phi by
   2-D trapezoid (:194-197, dbltrapz :214), E_theta = -jk e^{-jkr}/(4 pi r)

@@ -7,7 +7,7 @@ Sources (reference root):
   CalcPort (incident/reflected waves)         python/openEMS/ports.py:115-152
   DFT_time2freq ('pulse': sum * dt * 2)       python/openEMS/utilities.py:21-35
 The reference's own LumpedRlc (src/extensions/lumped_rlc.rs:165-313) has no
-port excitation, probes or S-parameters (SURVEY F12), so openEMS is followed.
+port excitation, probes or S-parameters, so openEMS is followed.
 """
 from __future__ import annotations
 
