@@ -1617,24 +1617,25 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
     if (pp.size() * sizeof(PointProbe) > e->probe_table.bytes) HIP_OK(e->probe_table.alloc(pp.size() * sizeof(PointProbe) + 64));
     if (!pp.empty())
         HIP_OK(hipMemcpyAsync(e->probe_table.p, pp.data(), pp.size() * sizeof(PointProbe), hipMemcpyHostToDevice, e->stream));
-    // Sources are added by the E kernels themselves when nothing acts between
-    // the E update and them (no ADE, Mur or UPML post-E hooks), when every
-    // source cell is one the E update writes (i, j >= 1) and the list is
-    // short; the point kernel then only records probes.
-    bool fold_sources = !px.empty() && px.size() <= (size_t)MAX_FOLD_SOURCES && murs.empty() && ades.empty() && !upml;
-    for (size_t q = 0; q < b->n_excitations && fold_sources; ++q)
-        if (b->excitations[q].i == 0 || b->excitations[q].j == 0) fold_sources = false;
-    if (const char* v = getenv("OE_FOLD_SOURCES")) fold_sources = fold_sources && atoi(v) != 0;
-    const bool point_ops = (!px.empty() && !fold_sources) || !pp.empty();
-
     Tiles tl = e->tiles;
     if (cp && !cp->zvec && tl.vec == 4) tl = auto_tiles(e->g, 1);  // z slabs not 4-aligned: scalar lanes
     const std::vector<Launch> plan = plan_launches(e, tl, cp);
     // Fused steps (one pass per step over the plan's interior box) when no
     // extension hooks in between the half-steps and the grid suits the
     // fused kernel; the x/y CPML slabs keep their two launches around it.
-    const bool fused = !fold_sources && e->fused_mode && !upml && murs.empty() && ades.empty() && tl.vec == 4 &&
+    const bool fused = e->fused_mode && !upml && murs.empty() && ades.empty() && tl.vec == 4 &&
                        fused_supported(e->g) && (e->fused_mode == 2 || e->g.nz >= 128) && (!cp || cp->zvec);
+    // Sources are added by the E kernels themselves when nothing acts between
+    // the E update and them (no ADE, Mur or UPML post-E hooks), when every
+    // source cell is one the E update writes (i, j >= 1), the list is short
+    // and the steps are not fused (the fused kernel leaves sources to the
+    // point kernel); the point kernel then only records probes.
+    bool fold_sources = !fused && !px.empty() && px.size() <= (size_t)MAX_FOLD_SOURCES && murs.empty() &&
+                        ades.empty() && !upml;
+    for (size_t q = 0; q < b->n_excitations && fold_sources; ++q)
+        if (b->excitations[q].i == 0 || b->excitations[q].j == 0) fold_sources = false;
+    if (const char* v = getenv("OE_FOLD_SOURCES")) fold_sources = fold_sources && atoi(v) != 0;
+    const bool point_ops = (!px.empty() && !fold_sources) || !pp.empty();
     FusedArgs fa{};
     std::vector<Launch> slabs;
     if (fused) {

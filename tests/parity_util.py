@@ -59,6 +59,8 @@ def run_pair(sc: Scenario, prec: int = 32, batches: Optional[List[int]] = None):
         eng.set_tiles(**sc.tiles)
     if sc.fused is not None:
         eng.fused_steps(sc.fused)
+        if sc.fused:
+            eng.set_profiling(1)  # launch counts show which scheme ran
     if sc.table is not None:
         eng.material_table(sc.table)
     if sc.graph is not None:

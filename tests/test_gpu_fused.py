@@ -34,6 +34,13 @@ def _check_psi(res):
                 assert g is not None and np.array_equal(g, o.astype(np.float32)), (face, field, which)
 
 
+def _assert_fused(eng):
+    """The fused kernel ran (the engine's launch counts, timing on)."""
+    n = sum(eng.kernel_time(k)[0] for k in ("fused:interior", "fused:grid"))
+    say(f"[fused] fused launches: {n}")
+    assert n > 0
+
+
 def _exc():
     return [((20, 18, 64), 2, True, gauss_ref(10e9)), ((22, 17, 60), 0, True, gauss_ref(8e9))]
 
@@ -44,6 +51,7 @@ def test_fused_no_cpml(table):
                   excitations=_exc(), probes=[(30, 18, 64)], fused=1, table=table)
     res = run_pair(sc)
     assert res["eng"].fused_steps() == 1
+    _assert_fused(res["eng"])
     assert_parity(res, sc.name)
 
 
@@ -53,6 +61,7 @@ def test_fused_cpml_all_faces(mode):
                   cpml=dict(layers=(8, 6, 5, 8, 8, 12), mode=mode), materials=BOXES, pecs=PEC,
                   excitations=_exc(), probes=[(30, 18, 64)], fused=1)
     res = run_pair(sc)
+    _assert_fused(res["eng"])
     assert_parity(res, sc.name)
     _check_psi(res)
 
@@ -62,16 +71,20 @@ def test_fused_cpml_z_faces_only_and_kappa():
                   cpml=dict(layers=(0, 0, 0, 0, 8, 8), kappa_max=4.0, alpha_max=0.03, mode=MODE_PHYSICAL),
                   materials=BOXES[:1], excitations=_exc(), fused=1)
     res = run_pair(sc)
+    _assert_fused(res["eng"])
     assert_parity(res, sc.name)
     _check_psi(res)
 
 
 def test_fused_short_rows_forced():
-    # nz = 44: rows fill 11 of 64 lanes; mode 2 fuses them anyway
-    sc = Scenario("fused_short_rows", (30, 26, 44), 0.5e-3, 160, cpml=dict(layers=(6,) * 6, mode=MODE_PHYSICAL),
+    # nz = 44: rows fill 11 of 64 lanes; mode 2 fuses them anyway (z-face
+    # layers a multiple of 4, which float4 lanes and so the fused pass need)
+    sc = Scenario("fused_short_rows", (30, 26, 44), 0.5e-3, 160,
+                  cpml=dict(layers=(6, 6, 6, 6, 8, 8), mode=MODE_PHYSICAL),
                   materials=[(4.0, 1.0, 0.02, 0.0, (8, 20, 6, 18, 10, 30))],
                   excitations=[((15, 13, 22), 2, True, gauss_ref(10e9))], fused=2)
     res = run_pair(sc)
+    _assert_fused(res["eng"])
     assert_parity(res, sc.name)
     _check_psi(res)
 
@@ -87,6 +100,7 @@ def test_fused_batches_alternate_with_split():
     eng, sim, exts = res["eng"], res["sim"], res["exts"]
     fn = sc.excitations[0][3]
     start = 1
+    eng.set_profiling(1)
     for nb, mode in ((37, 1), (20, 0), (41, 1), (1, 1), (15, 0)):
         eng.fused_steps(mode)
         smp = X.schedule(fn, start, nb, eng.dt)
@@ -99,6 +113,7 @@ def test_fused_batches_alternate_with_split():
         o = sim.fields()
         say(f"[fused] after {start} steps (last batch mode {mode}): equal={np.array_equal(g, o)}")
         assert np.array_equal(g, o)
+    _assert_fused(eng)
     _check_psi(res)
 
 
@@ -111,8 +126,12 @@ def test_fused_port_dft_matches_split():
         ms = W.microstrip(np.random.default_rng(11))
         op, eng, cp, port, exc = W.build_microstrip_hip(ms, 600, record_steps=600)
         eng.fused_steps(mode)
+        if mode:
+            eng.set_profiling(1)
         from openems_amd import EngineBatch
         eng.run_batch(EngineBatch(num_steps=600, excitations=exc, extensions=[cp, port]))
+        if mode:
+            _assert_fused(eng)
         out.append((port.series(), port.dft_values(), np.concatenate(eng.read_fields())))
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
