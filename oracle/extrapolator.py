@@ -217,3 +217,18 @@ def loss_and_grads(p, x, target, d: Dims, use_bf16: bool = False):
             dcols = np.einsum("bot,ok->btk", q(dpre), q(W.reshape(W.shape[0], -1)))
             dh = col2im(dcols, W.shape[1], c[f"in{l}"].shape[2])
     return loss, g
+
+
+def adam_step(w, g, m, v, t: int, lr: float, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8):
+    """One Adam update (Kingma and Ba 2015, Algorithm 1), the optimiser the
+    device training step applies to the float32 master weights.  Returns
+    (w, m, v) after step t (t counts from 1): m = b1 m + (1 - b1) g,
+    v = b2 v + (1 - b2) g^2, w -= lr (m / (1 - b1^t)) / (sqrt(v / (1 - b2^t)) + eps).
+    float64 arithmetic; the caller casts."""
+    w = np.asarray(w, np.float64)
+    g = np.asarray(g, np.float64)
+    m = b1 * np.asarray(m, np.float64) + (1.0 - b1) * g
+    v = b2 * np.asarray(v, np.float64) + (1.0 - b2) * g * g
+    c1 = 1.0 - b1 ** t
+    c2 = 1.0 - b2 ** t
+    return w - lr * (m / c1) / (np.sqrt(v / c2) + eps), m, v

@@ -139,3 +139,32 @@ def test_training_gains_over_stopping_on_damped_modes():
         f"loss {losses[0]:.3e} -> {losses[-1]:.3e}")
     assert after[64] < 0.2 and after[16] < 0.2
     assert after[64] < 0.5 * before[64]
+
+
+def test_adam_update_matches_oracle():
+    """k_adam against the oracle's Adam step (float64) over two training
+    steps.  The device gradients at the current weights are deterministic (no
+    atomics), so gradients() returns the g that train_step then applies.
+    Tolerance: 4 float32 ulps of each weight plus 1e-5 of the learning rate;
+    the first step moves every weight with a nonzero gradient by about lr
+    (1e-3), far above that."""
+    p, x, tgt = _case(4)
+    ex = Extrapolator(SMALL, params=pack(p, SMALL))
+    lr = 1e-3
+    w_dev = ex.params().astype(np.float64)
+    m = np.zeros_like(w_dev)
+    v = np.zeros_like(w_dev)
+    for t in (1, 2):
+        _, g = ex.gradients(x, tgt)
+        _, g_again = ex.gradients(x, tgt)
+        assert np.array_equal(g, g_again)
+        ex.train_step(x, tgt, lr)
+        got = ex.params().astype(np.float64)
+        want, m, v = X.adam_step(w_dev, g.astype(np.float64), m, v, t, lr)
+        tol = 4 * np.spacing(np.abs(want).astype(np.float32)).astype(np.float64) + 1e-5 * lr
+        err = np.abs(got - want)
+        moved = float(np.median(np.abs(got - w_dev)))
+        say(f"[extrap] adam step {t}: max |w_gpu - w_oracle| {err.max():.3e}, median step {moved:.3e}")
+        assert np.all(err <= tol)
+        assert moved > 0.5 * lr
+        w_dev = got  # the next step starts from the device's float32 weights

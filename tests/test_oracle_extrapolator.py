@@ -78,3 +78,27 @@ def test_bf16_rounding():
     assert r[2] == 1.0 + 2 ** -7  # rounds up
     assert r[3] == -2.5
     assert abs(r[4] - 3.140625) < 1e-12
+
+
+def test_adam_closed_forms():
+    """Adam's bias correction makes the first step lr * g / (|g| + eps), and a
+    constant gradient keeps that step size at every t (m-hat = g, v-hat = g^2
+    exactly).  A dropped bias correction, a missing square root or a swapped
+    beta fails one of these."""
+    rng = np.random.default_rng(3)
+    g = rng.standard_normal(1000) * np.logspace(-4, 2, 1000)
+    w0 = rng.standard_normal(1000)
+    lr, eps = 1e-3, 1e-8
+    w, m, v = X.adam_step(w0, g, 0.0, 0.0, 1, lr)
+    np.testing.assert_allclose(w0 - w, lr * g / (np.abs(g) + eps), rtol=1e-12, atol=0)
+    for t in range(2, 40):
+        w_prev = w
+        w, m, v = X.adam_step(w, g, m, v, t, lr)
+        np.testing.assert_allclose(w_prev - w, lr * g / (np.abs(g) + eps), rtol=1e-9, atol=0)
+    # a gradient that flips sign: second step has m-hat = (b1 g1 (1-b1) + (1-b1) g2)/(1-b1^2)
+    g1, g2 = 2.0, -1.0
+    w1, m1, v1 = X.adam_step(0.0, g1, 0.0, 0.0, 1, lr)
+    w2, _, _ = X.adam_step(w1, g2, m1, v1, 2, lr)
+    mh = (0.9 * 0.1 * g1 + 0.1 * g2) / (1 - 0.81)
+    vh = (0.999 * 0.001 * g1 ** 2 + 0.001 * g2 ** 2) / (1 - 0.999 ** 2)
+    assert abs((w1 - w2) - lr * mh / (math.sqrt(vh) + eps)) < 1e-15

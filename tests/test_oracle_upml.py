@@ -3,7 +3,12 @@ region boxes of build_regions for an asymmetric layer set, coefficients of
 init_region_coefficients at chosen cells against closed forms, the
 pass-through cells, and the instability the reference documents for it
 (src/fdtd/engine_testing.rs:2319-2354: energy growing by orders of magnitude
-within tens of steps)."""
+within tens of steps), and the flux update itself: three steps of pre -> Yee
+-> post written out by hand in numpy for both halves, fields and flux arrays
+compared at every step, with a check that the variant running the Yee update
+on the field instead of the flux would be rejected.  Planting either that
+variant or a flipped sign of the vvfo term in the oracle fails all three
+layer sets."""
 import math
 
 import numpy as np
@@ -75,3 +80,115 @@ def test_reference_instability():
         s.run_batch(30, [(n // 2, n // 2, n // 2, 2, True)], smp[None, 30 * b:30 * (b + 1)])
         energy.append(float((s.fields().astype(np.float64) ** 2).sum()))
     assert energy[2] / energy[1] > 1e15
+
+
+# ---------------------------------------------------------------- one-step pin
+# The update of pre_voltage_update / post_voltage_update and
+# pre_current_update / post_current_update (src/extensions/pml.rs:500-659)
+# written out by hand in numpy around a numpy Yee half-step, then compared with
+# the FP64 oracle after each of three steps: fields and both flux arrays.
+
+
+def _yee_h(E, H, da, db):
+    """update_h_basic (src/fdtd/basic_engine.rs:191-254): forward differences,
+    zero past the last index."""
+    def fwd(a, ax):
+        d = np.zeros_like(a)
+        sl = [slice(None)] * 3
+        sl2 = [slice(None)] * 3
+        sl[ax] = slice(0, -1)
+        sl2[ax] = slice(1, None)
+        d[tuple(sl)] = a[tuple(sl2)] - a[tuple(sl)]
+        return d
+    cx = fwd(E[2], 1) - fwd(E[1], 2)
+    cy = fwd(E[0], 2) - fwd(E[2], 0)
+    cz = fwd(E[1], 0) - fwd(E[0], 1)
+    return np.stack([da[0] * H[0] + db[0] * cx, da[1] * H[1] + db[1] * cy, da[2] * H[2] + db[2] * cz])
+
+
+def _yee_e(E, H, ca, cb):
+    """update_e_basic (src/fdtd/basic_engine.rs:262-306): backward differences,
+    the index-0 planes are never written."""
+    out = E.copy()
+    s = (slice(1, None),) * 3
+    sx = (slice(0, -1), slice(1, None), slice(1, None))
+    sy = (slice(1, None), slice(0, -1), slice(1, None))
+    sz = (slice(1, None), slice(1, None), slice(0, -1))
+    cx = (H[2][s] - H[2][sy]) - (H[1][s] - H[1][sz])
+    cy = (H[0][s] - H[0][sz]) - (H[2][s] - H[2][sx])
+    cz = (H[1][s] - H[1][sx]) - (H[0][s] - H[0][sy])
+    out[0][s] = ca[0][s] * E[0][s] + cb[0][s] * cx
+    out[1][s] = ca[1][s] * E[1][s] + cb[1][s] * cy
+    out[2][s] = ca[2][s] * E[2][s] + cb[2][s] * cz
+    return out
+
+
+def _upml_half(F, flux, coef, regions, yee, swap_first=True):
+    """One half-step with the UPML hooks around the Yee update `yee(F) -> F`.
+    coef[q] = (c, cfo, cfn) per component for region q.  With swap_first=False
+    the Yee update acts on the field itself instead of on the flux: the
+    plausible mistake the pin must reject."""
+    F = F.copy()
+    helps = []
+    for q, (st, sz) in enumerate(regions):
+        box = tuple(slice(a, a + b) for a, b in zip(st, sz))
+        c, cfo, _ = coef[q]
+        fh = np.stack([c[n] * F[n][box] - cfo[n] * flux[q][n] for n in range(3)])
+        helps.append(fh)
+        if swap_first:
+            for n in range(3):
+                F[n][box] = flux[q][n]
+    F = yee(F)
+    new_flux = []
+    for q, (st, sz) in enumerate(regions):
+        box = tuple(slice(a, a + b) for a, b in zip(st, sz))
+        _, _, cfn = coef[q]
+        fn = np.stack([F[n][box] for n in range(3)])
+        new_flux.append(fn)
+        for n in range(3):
+            F[n][box] = helps[q][n] + cfn[n] * fn[n]
+    return F, new_flux
+
+
+@pytest.mark.parametrize("layers", [(1, 0, 0, 0, 0, 0), (1, 0, 0, 1, 0, 2), (0, 2, 1, 0, 1, 0)])
+def test_flux_update_hand_expanded(layers):
+    n, d = 6, 1e-3
+    s = O.OracleSim(n, n, n, d, prec=64)
+    s.set_material(3.0, 1.0, 0.02, 0.0, (0, 4, 1, 5, 0, 3))
+    s.add_upml(layers)
+    rng = np.random.default_rng(7)
+    f0 = rng.standard_normal((6, n, n, n))
+    s.set_fields(f0)
+    regions = s.upml_regions()
+    assert regions, "the layer set must make at least one region"
+    C = s.coefficients()
+    ca, cb, da, db = C[0:3], C[3:6], C[6:9], C[9:12]
+    ce = [tuple(np.stack([s.upml_array(q, w, m) for m in range(3)]) for w in ("vv", "vvfo", "vvfn"))
+          for q in range(len(regions))]
+    ch = [tuple(np.stack([s.upml_array(q, w, m) for m in range(3)]) for w in ("ii", "iifo", "iifn"))
+          for q in range(len(regions))]
+    # the graded coefficients must be far from pass-through somewhere, or the pin is vacuous
+    assert min(float(np.min(c[0])) for c in ce) < 0.5
+    E, H = f0[:3].copy(), f0[3:].copy()
+    fe = [np.zeros((3,) + tuple(sz)) for _, sz in regions]
+    fhx = [np.zeros((3,) + tuple(sz)) for _, sz in regions]
+    Ev, Hv = E.copy(), H.copy()
+    fev = [a.copy() for a in fe]
+    fhv = [a.copy() for a in fhx]
+    for step in range(3):
+        H, fhx = _upml_half(H, fhx, ch, regions, lambda X: _yee_h(E, X, da, db))
+        E, fe = _upml_half(E, fe, ce, regions, lambda X: _yee_e(X, H, ca, cb))
+        Hv, fhv = _upml_half(Hv, fhv, ch, regions, lambda X: _yee_h(Ev, X, da, db), swap_first=False)
+        Ev, fev = _upml_half(Ev, fev, ce, regions, lambda X: _yee_e(X, Hv, ca, cb), swap_first=False)
+        s.run_batch(1)
+        got = s.fields()
+        want = np.concatenate([E, H])
+        scale = float(np.abs(want).max())
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * scale)
+        for q in range(len(regions)):
+            for m in range(3):
+                np.testing.assert_allclose(s.upml_array(q, "volt_flux", m), fe[q][m], rtol=0, atol=1e-12 * scale)
+                np.testing.assert_allclose(s.upml_array(q, "curr_flux", m), fhx[q][m], rtol=0, atol=1e-12 * scale)
+        # the Yee-on-the-field variant gives different fields: this check rejects it
+        wrong = np.concatenate([Ev, Hv])
+        assert float(np.abs(wrong - want).max()) > 1e-3 * scale
