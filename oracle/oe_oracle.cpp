@@ -124,6 +124,13 @@ struct SignalSet {
     std::vector<double> series;  // [steps][n]
 };
 
+// Extension hooks run on the host fields (src/extensions/extension_trait.rs:
+// 34-141): hook 0..3 = pre/post_update_h, pre/post_update_e, 4/5 = pre/post
+// batch; fields are the six component arrays (T*).  Test infrastructure for
+// the host-callback extension of the engine.
+typedef int (*oro_hook_fn)(void* user, int hook, uint64_t step, void* const fields[6]);
+typedef int (*oro_term_fn)(void* user);
+
 template <class T>
 struct Sim {
     size_t nx, ny, nz, N;
@@ -140,6 +147,11 @@ struct Sim {
     std::vector<MatOp> ops;
     std::vector<SignalSet> signals;
     int threads = 0;
+    oro_hook_fn hook = nullptr;
+    oro_term_fn term = nullptr;
+    void* hook_user = nullptr;
+    unsigned hook_mask = 0;
+    int hook_status = 0;  // first nonzero hook return
 
     size_t at(size_t i, size_t j, size_t k) const { return (i * ny + j) * nz + k; }
 };
@@ -839,20 +851,33 @@ struct Exc {
 // run_batch (src/fdtd/basic_engine.rs:56-163) with the hook order
 //   pre_h(UPML), H, post_h(UPML, CPML) | pre_e(Mur, ADE, UPML), E,
 //   post_e(UPML, CPML, ADE, Mur) | excitations
+// A host hook runs after the restated extensions of its hook point, as an
+// extension listed after them would.
+template <class T>
+void call_hook(Sim<T>& s, int which) {
+    if (!s.hook || !(s.hook_mask & (1u << which)) || s.hook_status) return;
+    void* f[6] = {s.E[0].data(), s.E[1].data(), s.E[2].data(), s.H[0].data(), s.H[1].data(), s.H[2].data()};
+    s.hook_status = s.hook(s.hook_user, which, s.timestep, f);
+}
+
 template <class T>
 void step(Sim<T>& s) {
     if (!s.upml.empty()) upml_pre(s, false);
+    call_hook(s, 0);
     update_h(s);
     if (!s.upml.empty()) upml_post(s, false);
     if (s.cpml_on) cpml_post_h(s);
+    call_hook(s, 1);
     mur_pre(s);
     ade_pre(s);
     if (!s.upml.empty()) upml_pre(s, true);
+    call_hook(s, 2);
     update_e(s);
     if (!s.upml.empty()) upml_post(s, true);
     if (s.cpml_on) cpml_post_e(s);
     ade_post(s);
     mur_post_apply(s);
+    call_hook(s, 3);
 }
 
 template <class T>
@@ -1167,14 +1192,18 @@ void oro_yee_steps(void* hv, uint64_t n) {
 // EngineImpl::run_batch restated (src/fdtd/basic_engine.rs:56-163)
 int oro_run_batch(void* hv, oro_batch* b) {
     oro_handle* h = (oro_handle*)hv;
+    int rc = 0;
     DISPATCH(h, {
         uint64_t num_steps = b->num_steps ? b->num_steps : (b->max_timesteps ? b->max_timesteps : UINT64_MAX);
         double peak = 0.0;
         b->n_energy = 0; b->steps_executed = 0; b->reason = 0; b->final_decay_db = 0.0;
         uint64_t ci = b->check_interval ? b->check_interval : 100;
-        for (uint64_t st = 0; st < num_steps; ++st) {
+        s.hook_status = 0;
+        call_hook(s, 4);
+        for (uint64_t st = 0; st < num_steps && !s.hook_status; ++st) {
             uint64_t cur = s.timestep;
             step(s);
+            if (s.hook_status) break;
             apply_exc(s, b->exc, b->n_exc, b->samples, b->n_samples, st);
             sample_probes(s);
             record_signals(s);
@@ -1191,6 +1220,8 @@ int oro_run_batch(void* hv, oro_batch* b) {
                 b->n_energy++;
             }
             if (cur > 0 && cur % ci == 0) {
+                // check_termination first, the decay test overrides it (basic_engine.rs:115-147)
+                const bool ext_stop = s.term && s.term(s.hook_user);
                 if (b->energy_decay_db > 0) {
                     double curE = energy_of(s, s.E) + energy_of(s, s.H);
                     if (peak > 0.0 && curE > 0.0) {
@@ -1201,10 +1232,26 @@ int oro_run_batch(void* hv, oro_batch* b) {
                         }
                     }
                 }
+                if (ext_stop) {
+                    b->reason = 2;
+                    break;
+                }
             }
         }
+        call_hook(s, 5);
+        rc = s.hook_status;
     });
-    return 0;
+    return rc;
+}
+
+void oro_set_hook(void* hv, oro_hook_fn fn, oro_term_fn term, void* user, unsigned mask) {
+    oro_handle* h = (oro_handle*)hv;
+    DISPATCH(h, {
+        s.hook = fn;
+        s.term = term;
+        s.hook_user = user;
+        s.hook_mask = mask;
+    });
 }
 
 }  // extern "C"

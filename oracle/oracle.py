@@ -114,10 +114,16 @@ def lib():
         L.oro_energy.restype = d
         L.oro_energy.argtypes = [vp, C.c_int]
         L.oro_yee_steps.argtypes = [vp, u64]
+        L.oro_set_hook.restype = None
+        L.oro_set_hook.argtypes = [vp, _HOOK_FN, _TERM_FN, vp, C.c_uint]
         L.oro_run_batch.restype = C.c_int
         L.oro_run_batch.argtypes = [vp, C.POINTER(_Batch)]
         _lib = L
     return _lib
+
+
+_HOOK_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_uint64, C.POINTER(C.c_void_p))
+_TERM_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
 
 
 def _dptr(a: np.ndarray):
@@ -295,6 +301,27 @@ class OracleSim:
         lib().oro_signal_series(self._h, idx, _dptr(out))
         return out.reshape(steps, n_signals)
 
+    def set_hook(self, hook=None, hooks=(), check_termination=None):
+        """Extension hooks on the oracle's fields, with the engine's host-callback
+        contract: hook(which, step, fields) gets six arrays of the grid's shape
+        (float32 or float64 as the oracle's precision) to change in place."""
+        dt = np.float32 if self.prec == 32 else np.float64
+        ct = C.c_float if self.prec == 32 else C.c_double
+        shape, n = self.shape, self.N
+
+        def _hook(user, which, step, fields):
+            views = [np.ctypeslib.as_array(C.cast(fields[d], C.POINTER(ct)), (n,)).reshape(shape) for d in range(6)]
+            assert views[0].dtype == dt
+            return int(hook(int(which), int(step), views) or 0)
+
+        def _term(user):
+            return int(bool(check_termination()))
+
+        self._hook_keep = (_HOOK_FN(_hook) if hook else _HOOK_FN(),
+                           _TERM_FN(_term) if check_termination else _TERM_FN())
+        lib().oro_set_hook(self._h, self._hook_keep[0], self._hook_keep[1], None,
+                           sum(1 << int(h) for h in hooks))
+
     def energy(self):
         return lib().oro_energy(self._h, 0), lib().oro_energy(self._h, 1)
 
@@ -327,7 +354,9 @@ class OracleSim:
         b.track_peak = int(track_peak)
         b.energy_out = _dptr(eout)
         b.energy_cap = energy_cap
-        lib().oro_run_batch(self._h, C.byref(b))
+        rc = lib().oro_run_batch(self._h, C.byref(b))
+        if rc:
+            raise RuntimeError(f"oracle batch: a hook returned status {rc}")
         n = min(b.n_energy, energy_cap)
         return BatchResult(b.steps_executed, b.reason, b.final_decay_db, eout[:n].copy())
 
