@@ -575,6 +575,44 @@ oe_status oe_extrap_gradients(oe_extrap* x, const float* inputs, const float* ta
 oe_status oe_extrap_train_step(oe_extrap* x, const float* inputs, const float* targets, size_t batch,
                                float lr, double* loss);
 
+/* Host-callback extension: the CPU fallback the reference's GPU engine takes
+ * for an Extension whose gpu_data() is None (requires_cpu_fallback,
+ * src/extensions/extension_trait.rs:143-166), with the hook contract of
+ * :32-141.  For oracle and debug use: each selected hook point synchronises
+ * the engine's stream, copies the six field arrays (Ex, Ey, Ez, Hx, Hy, Hz,
+ * N floats each, z fastest) to a host buffer, calls hook(user, which,
+ * timestep, fields) and uploads what that hook may change:
+ *   which 0 pre_update_h, 1 post_update_h  -> H uploaded,
+ *   which 2 pre_update_e, 3 post_update_e  -> E uploaded,
+ *   which 4 pre_batch,    5 post_batch     -> all six uploaded.
+ * hook_mask bit `which` selects a hook point.  timestep is the engine's count
+ * at the start of the step (current_step, src/fdtd/basic_engine.rs:85-87);
+ * for pre/post_batch the count at that moment.  At each hook point the
+ * batch's device extensions act first (the order listed at
+ * oe_engine_run_batch), then host extensions in list order; sources are
+ * applied after the post_update_e hooks.  A nonzero return from hook ends
+ * run_batch at once with that status (values outside 1..7 become
+ * OE_ERR_NUMERICAL), as the reference's `?` does; post_batch hooks are then
+ * not called.  check_termination (may be NULL) is asked at every check step
+ * (timestep > 0 and a multiple of check_interval, with or without a decay
+ * test); nonzero ends the batch with OE_TERM_EXTENSION_STOP unless the
+ * energy decay test ends it at the same step (src/fdtd/basic_engine.rs:
+ * 115-147).  It may write a NUL-terminated reason into reason[0 ..
+ * reason_capacity), read back with oe_ext_host_callback_reason.  A batch with
+ * a host extension launches every step directly (no graphs, no fused steps). */
+typedef int (*oe_host_hook_fn)(void* user, int which, uint64_t timestep, float* const fields[6]);
+typedef int (*oe_host_check_fn)(void* user, char* reason, size_t reason_capacity);
+typedef struct {
+    oe_host_hook_fn hook;               /* may be NULL */
+    oe_host_check_fn check_termination; /* may be NULL */
+    void* user;
+    uint32_t hook_mask;
+} oe_host_callback_config;
+oe_status oe_ext_host_callback_create(oe_engine* eng, const oe_host_callback_config* cfg, oe_extension** out);
+/* The reason given by the last check_termination that asked to stop ("" if
+ * none); *len (may be NULL) receives its length. */
+oe_status oe_ext_host_callback_reason(oe_extension* ext, char* out, size_t capacity, size_t* len);
+
 /* CPML psi arrays / ADE state for inspection: field 0 = E, 1 = H; which 0 =
  * psi_pos, 1 = psi_neg.  Returns the element count in *n. */
 oe_status oe_ext_cpml_read_psi(oe_extension* ext, int face, int field, int which, float* out,

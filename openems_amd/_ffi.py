@@ -90,6 +90,15 @@ class ExtrapDims(C.Structure):
                 ("hidden", C.c_uint32), ("modes", C.c_uint32), ("batch", C.c_uint32), ("tail", C.c_uint32)]
 
 
+HOST_HOOK_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_uint64, C.POINTER(C.POINTER(C.c_float)))
+HOST_CHECK_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_char), C.c_size_t)
+
+
+class HostCallbackConfig(C.Structure):
+    _fields_ = [("hook", HOST_HOOK_FN), ("check_termination", HOST_CHECK_FN), ("user", C.c_void_p),
+                ("hook_mask", C.c_uint32)]
+
+
 class CpmlConfig(C.Structure):
     _fields_ = [("layers", C.c_size_t * 6), ("grading_order", C.c_double), ("reflection_coeff", C.c_double),
                 ("alpha_max", C.c_double), ("kappa_max", C.c_double), ("mode", C.c_int)]
@@ -171,6 +180,8 @@ SIGNATURES = {
     "oe_ext_upml_read": (_i, [_vp, _i, _i, _i, _fp, _sz, _szp, _szp, _szp]),
     "oe_ext_cpml_read_psi": (_i, [_vp, _i, _i, _i, _fp, _sz, _szp]),
     "oe_ext_ade_read_state": (_i, [_vp, _fp, _sz, _szp]),
+    "oe_ext_host_callback_create": (_i, [_vp, C.POINTER(HostCallbackConfig), _pp]),
+    "oe_ext_host_callback_reason": (_i, [_vp, C.c_char_p, _sz, _szp]),
     "oe_extension_reset": (_i, [_vp]),
     "oe_extension_destroy": (None, [_vp]),
 }

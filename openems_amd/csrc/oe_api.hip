@@ -376,6 +376,7 @@ void engine_free(oe_engine* e) {
     if (e->partials) (void)hipFree(e->partials);
     if (e->energy_dev) (void)hipFree(e->energy_dev);
     if (e->energy_host) (void)hipHostFree(e->energy_host);
+    if (e->host_fields) (void)hipHostFree(e->host_fields);
     if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
     if (e->ev_join) (void)hipEventDestroy(e->ev_join);
     if (e->side) (void)hipStreamDestroy(e->side);
@@ -1051,6 +1052,31 @@ oe_status oe_ext_field_probe_create(oe_engine* e, uint32_t i, uint32_t j, uint32
     return OE_OK;
 }
 
+oe_status oe_ext_host_callback_create(oe_engine* e, const oe_host_callback_config* cfg, oe_extension** out) {
+    if (!e || !cfg || !out) return fail(OE_ERR_CONFIG, "null argument");
+    if (cfg->hook_mask & ~0x3Fu) return fail(OE_ERR_CONFIG, "hook_mask has bits above the six hook points");
+    if (cfg->hook_mask && !cfg->hook) return fail(OE_ERR_CONFIG, "hook_mask selects hooks but hook is null");
+    std::unique_ptr<oe_extension> x(new oe_extension());
+    x->kind = EXT_HOST;
+    x->eng = e;
+    x->host = *cfg;
+    e->exts.push_back(x.get());
+    *out = x.release();
+    return OE_OK;
+}
+
+oe_status oe_ext_host_callback_reason(oe_extension* x, char* out, size_t capacity, size_t* len) {
+    if (!x) return fail(OE_ERR_CONFIG, "null argument");
+    if (x->kind != EXT_HOST) return fail(OE_ERR_CONFIG, "not a host-callback extension");
+    if (len) *len = x->host_reason.size();
+    if (out && capacity) {
+        const size_t n = std::min(capacity - 1, x->host_reason.size());
+        std::memcpy(out, x->host_reason.data(), n);
+        out[n] = 0;
+    }
+    return OE_OK;
+}
+
 oe_status oe_ext_dft_create(oe_engine* e, const oe_dft_config* cfg, oe_extension** out) {
     if (!e || !cfg || !out) return fail(OE_ERR_CONFIG, "null argument");
     const size_t n = cfg->n_signals, nf = cfg->n_freqs;
@@ -1549,7 +1575,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
     const CpmlArgs* cp = nullptr;
     oe_extension* cpx = nullptr;
     oe_extension* upml = nullptr;
-    std::vector<oe_extension*> murs, ades, probes, dfts;
+    std::vector<oe_extension*> murs, ades, probes, dfts, hosts;
     for (size_t q = 0; q < b->n_extensions; ++q) {
         oe_extension* x = b->extensions[q];
         if (!x) return fail(OE_ERR_CONFIG, "extension %zu is null", q);
@@ -1569,6 +1595,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             case EXT_ADE: ades.push_back(x); break;
             case EXT_PROBE: probes.push_back(x); break;
             case EXT_DFT: dfts.push_back(x); break;
+            case EXT_HOST: hosts.push_back(x); break;
             default: return fail(OE_ERR_CONFIG, "unknown extension kind");
         }
     }
@@ -1623,14 +1650,17 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
     // Fused steps (one pass per step over the plan's interior box) when no
     // extension hooks in between the half-steps and the grid suits the
     // fused kernel; the x/y CPML slabs keep their two launches around it.
-    const bool fused = e->fused_mode && !upml && murs.empty() && ades.empty() && tl.vec == 4 &&
+    // Host-callback extensions run between the launches of a step, so their
+    // batches launch every step directly, in the plain two-half-step order.
+    const bool host = !hosts.empty();
+    const bool fused = e->fused_mode && !host && !upml && murs.empty() && ades.empty() && tl.vec == 4 &&
                        fused_supported(e->g) && (e->fused_mode == 2 || e->g.nz >= 128) && (!cp || cp->zvec);
     // Sources are added by the E kernels themselves when nothing acts between
     // the E update and them (no ADE, Mur or UPML post-E hooks), when every
     // source cell is one the E update writes (i, j >= 1), the list is short
     // and the steps are not fused (the fused kernel leaves sources to the
     // point kernel); the point kernel then only records probes.
-    bool fold_sources = !fused && !px.empty() && px.size() <= (size_t)MAX_FOLD_SOURCES && murs.empty() &&
+    bool fold_sources = !fused && !host && !px.empty() && px.size() <= (size_t)MAX_FOLD_SOURCES && murs.empty() &&
                         ades.empty() && !upml;
     for (size_t q = 0; q < b->n_excitations && fold_sources; ++q)
         if (b->excitations[q].i == 0 || b->excitations[q].j == 0) fold_sources = false;
@@ -1719,7 +1749,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
     // UPML, and no ADE cell on a face's line or shift plane (ADE post runs
     // between the E update and the Mur post step).
     MurZ mz{};
-    bool fold_mur = e->fold_mur && murs.size() == 1 && !upml && !fused && tl.vec == 4 && e->g.nz >= 4;
+    bool fold_mur = e->fold_mur && !host && murs.size() == 1 && !upml && !fused && tl.vec == 4 && e->g.nz >= 4;
     if (fold_mur)
         for (const MurDev& m : murs[0]->mur_faces) {
             if (m.ny != 2) {
@@ -1770,7 +1800,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
     // on the side stream beside it: fork after the previous work, join
     // before the next half-step.  Steps whose launches are timed stay on
     // one stream.
-    bool overlap = e->overlap_slabs && !fused && !upml;
+    bool overlap = e->overlap_slabs && !fused && !upml && !host;
     {
         bool any_slab = false, any_inner = false;
         for (const Launch& L : plan) ((L.axes & (AX_X | AX_Y)) ? any_slab : any_inner) = true;
@@ -1802,6 +1832,33 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             if (!(L.axes & (AX_X | AX_Y))) HIP_OK(one(L, e->stream));
         HIP_OK(hipEventRecord(e->ev_join, e->side));
         HIP_OK(hipStreamWaitEvent(e->stream, e->ev_join, 0));
+        return OE_OK;
+    };
+
+    // Host-callback hooks at point `which` (0..5): fields to the host, every
+    // selected hook in list order, then the part the hook point may change
+    // back to HBM (H after 0/1, E after 2/3, all after 4/5).
+    auto run_hooks = [&](int which) -> oe_status {
+        bool any = false;
+        for (auto* x : hosts) any = any || ((x->host.hook_mask >> which) & 1u);
+        if (!any) return OE_OK;
+        const size_t N = e->g.N;
+        if (!e->host_fields) HIP_OK(hipHostMalloc((void**)&e->host_fields, 6 * N * sizeof(float), hipHostMallocDefault));
+        HIP_OK(hipMemcpyAsync(e->host_fields, e->F, 6 * N * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+        HIP_OK(hipStreamSynchronize(e->stream));
+        float* f[6];
+        for (int d = 0; d < 6; ++d) f[d] = e->host_fields + (size_t)d * N;
+        for (auto* x : hosts) {
+            if (!((x->host.hook_mask >> which) & 1u)) continue;
+            const int rc = x->host.hook(x->host.user, which, e->timestep, f);
+            if (rc)
+                return fail(rc >= 1 && rc <= 7 ? (oe_status)rc : OE_ERR_NUMERICAL,
+                            "host extension hook %d returned %d at timestep %llu", which, rc,
+                            (unsigned long long)e->timestep);
+        }
+        const size_t lo = which == 0 || which == 1 ? 3 : 0, cnt = which >= 4 ? 6 : 3;
+        HIP_OK(hipMemcpyAsync(e->F + lo * N, e->host_fields + lo * N, cnt * N * sizeof(float), hipMemcpyHostToDevice,
+                              e->stream));
         return OE_OK;
     };
 
@@ -1855,6 +1912,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             Timed tm(e, "upml");
             HIP_OK(launch_upml_pre(e->F, e->g, upml->upml, false, e->stream));
         }
+        if (host) OE_TRY(run_hooks(0));
         if (!fused && ovl) {
             OE_TRY(overlapped_half(true, st));
         } else if (!fused)
@@ -1866,6 +1924,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             Timed tm(e, "upml");
             HIP_OK(launch_upml_post(e->F, e->g, upml->upml, false, e->stream));
         }
+        if (host) OE_TRY(run_hooks(1));
         for (auto* x : murs) {
             if (fold_mur) break;
             Timed tm(e, "mur_pre");
@@ -1887,6 +1946,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             Timed tm(e, "upml");
             HIP_OK(launch_upml_pre(e->F, e->g, upml->upml, true, e->stream));
         }
+        if (host) OE_TRY(run_hooks(2));
         if (!fused && ovl) {
             OE_TRY(overlapped_half(false, st));
         } else if (!fused)
@@ -1922,6 +1982,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             Timed tm(e, "mur_post");
             HIP_OK(launch_mur_post(e->F, e->g, x->mur_dev.as<MurDev>(), (int)x->mur_faces.size(), e->stream));
         }
+        if (host) OE_TRY(run_hooks(3));
         if (point_ops) {
             Timed tm(e, "point_ops");
             HIP_OK(launch_point_ops(e->F, e->g.N, e->exc_table.as<PointExc>(), fold_sources ? 0 : (int)px.size(), st,
@@ -1945,7 +2006,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
     };
 
     const int LG = e->graph_steps;
-    const bool use_graph = LG > 0 && !fused && dev_dft;
+    const bool use_graph = LG > 0 && !fused && dev_dft && !host;
     struct GraphGuard {
         hipGraph_t g = nullptr;
         hipGraphExec_t x = nullptr;
@@ -1968,6 +2029,7 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
         return OE_OK;
     };
 
+    if (host) OE_TRY(run_hooks(4));  // Extension::pre_batch
     uint64_t st = 0;
     while (st < num_steps) {
         const uint64_t cur = e->timestep;
@@ -2008,22 +2070,42 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             HIP_OK(launch_energy(e->F, e->g.N, e->partials, energy_blocks(), e->energy_dev + 2 * sample_steps.size(), e->stream));
             sample_steps.push_back(cur);
         }
-        if (cur > 0 && cur % check_interval == 0 && need_decay) {
-            const bool sampled_now = !sample_steps.empty() && sample_steps.back() == cur;
-            OE_TRY(drain());
-            double curE = last_total;
-            if (!sampled_now) {  // compute_energy_sample at the check step
-                double ee, he;
-                OE_TRY(oe_engine_energy(e, &ee, &he));
-                curE = ee + he;
-            }
-            if (peak > 0.0 && curE > 0.0) {  // src/fdtd/basic_engine.rs:128-141
-                const double ddb = 10.0 * std::log10(curE / peak);
-                if (ddb < -std::fabs(b->energy_decay_db)) {
-                    reason = OE_TERM_ENERGY_DECAY;
-                    final_db = ddb;
+        if (cur > 0 && cur % check_interval == 0 && (need_decay || host)) {
+            // Extension::check_termination first; the decay test takes
+            // precedence when it ends the batch at the same step
+            // (src/fdtd/basic_engine.rs:115-147).
+            bool ext_stop = false;
+            for (auto* x : hosts) {
+                if (!x->host.check_termination) continue;
+                char why[256] = {0};
+                if (x->host.check_termination(x->host.user, why, sizeof(why))) {
+                    why[sizeof(why) - 1] = 0;
+                    x->host_reason = why;
+                    ext_stop = true;
                     break;
                 }
+            }
+            if (need_decay) {
+                const bool sampled_now = !sample_steps.empty() && sample_steps.back() == cur;
+                OE_TRY(drain());
+                double curE = last_total;
+                if (!sampled_now) {  // compute_energy_sample at the check step
+                    double ee, he;
+                    OE_TRY(oe_engine_energy(e, &ee, &he));
+                    curE = ee + he;
+                }
+                if (peak > 0.0 && curE > 0.0) {  // src/fdtd/basic_engine.rs:128-141
+                    const double ddb = 10.0 * std::log10(curE / peak);
+                    if (ddb < -std::fabs(b->energy_decay_db)) {
+                        reason = OE_TERM_ENERGY_DECAY;
+                        final_db = ddb;
+                        break;
+                    }
+                }
+            }
+            if (ext_stop) {
+                reason = OE_TERM_EXTENSION_STOP;
+                break;
             }
         }
         if (e->profiling && e->pending.size() > 4096) {
@@ -2035,6 +2117,10 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
     HIP_OK(hipStreamSynchronize(e->stream));
     if (e->profiling) collect_timing(e);
     if (b->persistent_peak) e->peak_persistent = peak;
+    if (host) {  // Extension::post_batch
+        OE_TRY(run_hooks(5));
+        HIP_OK(hipStreamSynchronize(e->stream));
+    }
 
     r->timesteps_executed = executed;
     r->termination_reason = reason;

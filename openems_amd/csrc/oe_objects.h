@@ -59,7 +59,7 @@ struct oe_operator {
 };
 
 // ------------------------------------------------------------ extensions
-enum ExtKind { EXT_CPML = 1, EXT_MUR = 2, EXT_ADE = 3, EXT_PROBE = 4, EXT_DFT = 5, EXT_UPML = 6 };
+enum ExtKind { EXT_CPML = 1, EXT_MUR = 2, EXT_ADE = 3, EXT_PROBE = 4, EXT_DFT = 5, EXT_UPML = 6, EXT_HOST = 7 };
 
 struct oe_extension {
     int kind = 0;
@@ -103,6 +103,9 @@ struct oe_extension {
     std::vector<Nf2ffFace> nf_faces;
     std::vector<size_t> nf_sig0;
     Nf2ffCachePtr nf_cache;
+    // host-callback extension: hooks on fields copied to the host
+    oe_host_callback_config host{};
+    std::string host_reason;  // last reason check_termination gave
 };
 
 // ------------------------------------------------------------------ engine
@@ -165,5 +168,8 @@ struct oe_engine {
     std::map<std::string, KernelStat> stats;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::vector<hipEvent_t> event_pool;
+    // host copy of the six field arrays for host-callback hooks (pinned,
+    // allocated on first use)
+    float* host_fields = nullptr;
 };
 

@@ -250,6 +250,15 @@ class Extension:
         return _far_field(lambda th, nt, ph, np_, r, c, out: lib().oe_ext_nf2ff_calculate(
             self._h, int(freq_index), th, nt, ph, np_, r, c, out), self._freqs[freq_index], theta, phi, radius, center)
 
+    # host callback
+    def stop_reason(self) -> str:
+        """The reason the last check_termination gave for stopping ("" if none)."""
+        n = C.c_size_t()
+        check(lib().oe_ext_host_callback_reason(self._h, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        check(lib().oe_ext_host_callback_reason(self._h, buf, n.value + 1, C.byref(n)))
+        return buf.value.decode(errors="replace")
+
     # probes
     def data(self) -> np.ndarray:
         n = C.c_size_t()
@@ -649,6 +658,56 @@ class HipEngine:
         x._freqs = fr
         return x
 
+    HOOKS = ("pre_update_h", "post_update_h", "pre_update_e", "post_update_e", "pre_batch", "post_batch")
+
+    def host_callback(self, hook=None, hooks: Sequence = (), check_termination=None) -> Extension:
+        """An Extension without a device form, run on the host
+        (src/extensions/extension_trait.rs:32-166): hook(which, step, fields)
+        is called at the hook points named in `hooks` (names of HOOKS or their
+        indices) with the six field arrays, each shaped like the grid, to change
+        in place (H at the H hooks, E at the E hooks, all at the batch hooks);
+        it may return a nonzero oe_status to fail the batch.
+        check_termination() returns None or a reason string, as the trait's
+        check_termination does.  An exception in either ends the batch and is
+        raised again by run_batch."""
+        shape, n = tuple(self.shape), self.N
+        mask = 0
+        for h in hooks:
+            mask |= 1 << (self.HOOKS.index(h) if isinstance(h, str) else int(h))
+        x_ref = {}
+
+        def _hook(user, which, step, fields):
+            try:
+                views = [np.ctypeslib.as_array(fields[d], (n,)).reshape(shape) for d in range(6)]
+                return int(hook(int(which), int(step), views) or 0)
+            except BaseException as exc:  # raised again by run_batch
+                x_ref["x"]._host_error = exc
+                return 5
+
+        def _check(user, reason, cap):
+            try:
+                why = check_termination()
+            except BaseException as exc:
+                x_ref["x"]._host_error = exc
+                why = "exception in check_termination"
+            if why is None:
+                return 0
+            b = str(why).encode()[: max(int(cap) - 1, 0)]
+            C.memmove(reason, b, len(b))
+            return 1
+
+        cfg = _ffi.HostCallbackConfig()
+        cfg.hook = _ffi.HOST_HOOK_FN(_hook) if hook else _ffi.HOST_HOOK_FN()
+        cfg.check_termination = _ffi.HOST_CHECK_FN(_check) if check_termination else _ffi.HOST_CHECK_FN()
+        cfg.hook_mask = mask
+        h = C.c_void_p()
+        check(lib().oe_ext_host_callback_create(self._h, C.byref(cfg), C.byref(h)))
+        x = Extension(self, h.value, "host")
+        x._host_keep = (cfg.hook, cfg.check_termination)
+        x._host_error = None
+        x_ref["x"] = x
+        return x
+
     def field_probe(self, pos) -> Extension:
         h = C.c_void_p()
         check(lib().oe_ext_field_probe_create(self._h, *map(int, pos), C.byref(h)))
@@ -691,7 +750,16 @@ class HipEngine:
         d.energy_samples = samples
         d.energy_samples_capacity = cap
         r = _ffi.BatchResultC()
-        check(lib().oe_engine_run_batch(self._h, C.byref(d), C.byref(r)))
+        for x in batch.extensions:
+            if x.kind == "host":
+                x._host_error = None
+        status = lib().oe_engine_run_batch(self._h, C.byref(d), C.byref(r))
+        for x in batch.extensions:
+            err = getattr(x, "_host_error", None)
+            if err is not None:
+                x._host_error = None
+                raise err
+        check(status)
         n = min(r.n_energy_samples, cap)
         es = np.array([(samples[q].timestep, samples[q].e_energy, samples[q].h_energy) for q in range(n)],
                       dtype=np.float64).reshape(-1, 3)

@@ -23,6 +23,7 @@ STRUCTS = {
     "SweepEntry": "oe_sweep_entry",
     "ExtrapDims": "oe_extrap_dims",
     "CpmlConfig": "oe_cpml_config",
+    "HostCallbackConfig": "oe_host_callback_config",
 }
 
 
