@@ -154,6 +154,47 @@ def box_faces(lo, hi, steps, field_fn):
     return faces
 
 
+def _node_terms(ax, q, a1, a2, i1, i2):
+    """The four signals (E_a1, E_a2, H_a1, H_a2) at one node of a face."""
+    p = [0, 0, 0]
+    p[ax], p[a1], p[a2] = q, i1, i2
+    out = []
+    for c in (a1, a2):
+        sig = []
+        for d in (-1, 0):
+            t = list(p)
+            t[c] += d
+            sig.append((c, *t, 0.5))
+        out.append(sig)
+    for c in (a1, a2):
+        o = a2 if c == a1 else a1
+        sig = []
+        for dn in (-1, 0):
+            for dt in (-1, 0):
+                t = list(p)
+                t[ax] += dn
+                t[o] += dt
+                sig.append((3 + c, *t, 0.25))
+        out.append(sig)
+    return out
+
+
+def surface_faces(box, directions: int = 0x3F):
+    """Faces of the recording box, (axis, sign, q_axis, c1 node indices, c2
+    node indices), in the order of surface_terms."""
+    lo = box[0::2]
+    hi = box[1::2]
+    faces = []
+    for ax in range(3):
+        for side in range(2):
+            if not directions >> (2 * ax + side) & 1:
+                continue
+            a1, a2 = (ax + 1) % 3, (ax + 2) % 3
+            faces.append((ax, 1 if side else -1, hi[ax] if side else lo[ax], list(range(lo[a1], hi[a1] + 1)),
+                          list(range(lo[a2], hi[a2] + 1))))
+    return faces
+
+
 def surface_terms(shape, delta, box, directions: int = 0x3F):
     """DFT signal terms of an NF2FF recording box through the mesh nodes
     box = (i0, i1, j0, j1, k0, k1) (inclusive), restating openEMS's
@@ -166,40 +207,29 @@ def surface_terms(shape, delta, box, directions: int = 0x3F):
     (axis+1)%3, (axis+2)%3, per node the signals E_a1, E_a2, H_a1, H_a2.
     Terms are (component 0..5, i, j, k, weight).  Returns (terms, faces) where
     faces are (axis, sign, q_axis, c1 node indices, c2 node indices)."""
-    lo = box[0::2]
-    hi = box[1::2]
-    terms, faces = [], []
-    for ax in range(3):
-        for side in range(2):
-            if not directions >> (2 * ax + side) & 1:
-                continue
-            a1, a2 = (ax + 1) % 3, (ax + 2) % 3
-            q = hi[ax] if side else lo[ax]
-            r1 = list(range(lo[a1], hi[a1] + 1))
-            r2 = list(range(lo[a2], hi[a2] + 1))
-            faces.append((ax, 1 if side else -1, q, r1, r2))
-            for i1 in r1:
-                for i2 in r2:
-                    p = [0, 0, 0]
-                    p[ax], p[a1], p[a2] = q, i1, i2
-                    for c in (a1, a2):
-                        sig = []
-                        for d in (-1, 0):
-                            t = list(p)
-                            t[c] += d
-                            sig.append((c, *t, 0.5))
-                        terms.append(sig)
-                    for c in (a1, a2):
-                        o = a2 if c == a1 else a1
-                        sig = []
-                        for dn in (-1, 0):
-                            for dt in (-1, 0):
-                                t = list(p)
-                                t[ax] += dn
-                                t[o] += dt
-                                sig.append((3 + c, *t, 0.25))
-                        terms.append(sig)
+    faces = surface_faces(box, directions)
+    terms = []
+    for ax, sign, q, r1, r2 in faces:
+        a1, a2 = (ax + 1) % 3, (ax + 2) % 3
+        for i1 in r1:
+            for i2 in r2:
+                terms.extend(_node_terms(ax, q, a1, a2, i1, i2))
     return terms, faces
+
+
+def surface_terms_subset(box, signals, directions: int = 0x3F):
+    """The terms of the given signal indices (positions in surface_terms'
+    order) only, for boxes too large to list every signal."""
+    faces = surface_faces(box, directions)
+    starts = np.cumsum([0] + [4 * len(r1) * len(r2) for _, _, _, r1, r2 in faces])
+    out = []
+    for s in signals:
+        f = int(np.searchsorted(starts, s, side="right")) - 1
+        ax, _, q, r1, r2 = faces[f]
+        node, which = divmod(int(s) - int(starts[f]), 4)
+        n1, n2 = divmod(node, len(r2))
+        out.append(_node_terms(ax, q, (ax + 1) % 3, (ax + 2) % 3, r1[n1], r2[n2])[which])
+    return out
 
 
 def faces_from_phasors(values, delta, faces):

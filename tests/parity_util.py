@@ -141,3 +141,33 @@ def assert_parity(res, name="", bit_exact=True):
 def gauss_ref(center_freq, bw=0.5):
     t0, tau = X.reference_gaussian_params(center_freq, bw)
     return lambda t: X.gaussian(t, t0, tau)
+
+
+def port_spheres_oracle(wl, samples, surf_terms=(), prec: int = 32):
+    """The composite workload (openems_amd.workloads.PortSpheres) on the
+    oracle: Lorentz background material, port resistor, CPML on five faces,
+    Mur on +z, Drude and Lorentz ADE, the port's U/I signals followed by
+    surf_terms, driven by the port's soft sources.  Returns (sim, port_terms)."""
+    from oracle import port as P
+
+    sim = O.OracleSim(*wl.shape, wl.delta, prec)
+    drude, lorentz = [], []
+    for kind, c, r in wl.spheres:
+        cells = wl.sphere_cells(c, r)
+        (drude if kind == "drude" else lorentz).append(cells)
+        if kind == "lorentz":
+            for i, j, k0, k1 in wl.columns(cells):
+                sim.set_material(wl.LORENTZ_EPS_INF, 1.0, 0.0, 0.0, (i, i + 1, j, j + 1, k0, k1))
+    sim.add_lumped(2, wl.port_region, wl.port_R)
+    sim.add_cpml(wl.cpml_layers, mode=O.MODE_PHYSICAL)
+    sim.add_mur(wl.mur_faces)
+    if drude:
+        sim.add_ade(O.ADE_DRUDE, wl.DRUDE, np.concatenate(drude), O.MODE_PHYSICAL)
+    if lorentz:
+        sim.add_ade(O.ADE_LORENTZ, wl.LORENTZ, np.concatenate(lorentz), O.MODE_PHYSICAL)
+    port_terms = P.lumped_port_terms(wl.shape, (wl.delta,) * 3, 2, wl.port_region)
+    sim.add_signals(list(port_terms) + list(surf_terms))
+    exc = P.port_excitations(2, wl.port_region)
+    sim.run_batch(wl.steps, [(p[0], p[1], p[2], d, True) for p, d, a in exc],
+                  np.array([a * samples.astype(np.float64) for _, _, a in exc]))
+    return sim, port_terms

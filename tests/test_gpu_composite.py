@@ -16,35 +16,15 @@ from openems_amd import EngineBatch
 from openems_amd import workloads as W
 from openems_amd.engine import port_excitations
 from oracle import nf2ff as N
-from oracle import oracle as O
 from oracle import port as P
-from tests.parity_util import say
+from tests.parity_util import port_spheres_oracle, say
 
 pytestmark = pytest.mark.gpu
 
 
 def _oracle(wl, samples):
-    sim = O.OracleSim(*wl.shape, wl.delta)
-    drude, lorentz = [], []
-    for kind, c, r in wl.spheres:
-        cells = wl.sphere_cells(c, r)
-        (drude if kind == "drude" else lorentz).append(cells)
-        if kind == "lorentz":
-            for i, j, k0, k1 in wl.columns(cells):
-                sim.set_material(wl.LORENTZ_EPS_INF, 1.0, 0.0, 0.0, (i, i + 1, j, j + 1, k0, k1))
-    sim.add_lumped(2, wl.port_region, wl.port_R)
-    sim.add_cpml(wl.cpml_layers, mode=O.MODE_PHYSICAL)
-    sim.add_mur(wl.mur_faces)
-    if drude:
-        sim.add_ade(O.ADE_DRUDE, wl.DRUDE, np.concatenate(drude), O.MODE_PHYSICAL)
-    if lorentz:
-        sim.add_ade(O.ADE_LORENTZ, wl.LORENTZ, np.concatenate(lorentz), O.MODE_PHYSICAL)
-    port_terms = P.lumped_port_terms(wl.shape, (wl.delta,) * 3, 2, wl.port_region)
     surf_terms, faces = N.surface_terms(wl.shape, (wl.delta,) * 3, wl.nf2ff_box)
-    sim.add_signals(port_terms + surf_terms)
-    exc = P.port_excitations(2, wl.port_region)
-    sim.run_batch(wl.steps, [(p[0], p[1], p[2], d, True) for p, d, a in exc],
-                  np.array([a * samples.astype(np.float64) for _, _, a in exc]))
+    sim, port_terms = port_spheres_oracle(wl, samples, surf_terms)
     return sim, port_terms, surf_terms, faces
 
 

@@ -167,3 +167,12 @@ def test_surface_terms_interpolate_linear_fields_to_nodes():
     built = N.faces_from_phasors(vals.astype(np.complex128), d, faces)
     assert built[0].axis == 0 and built[0].sign == -1 and built[0].c0 == 2.0
     assert np.array_equal(built[5].c1, np.arange(2, 7) * 1.0)
+
+
+def test_surface_terms_subset_equals_full_list():
+    box = (3, 9, 2, 8, 4, 11)
+    terms, faces = N.surface_terms((12, 12, 14), (1e-3,) * 3, box)
+    assert [f[:3] for f in N.surface_faces(box)] == [f[:3] for f in faces]
+    idx = np.random.default_rng(0).choice(len(terms), 60, replace=False)
+    idx = np.concatenate([idx, [0, len(terms) - 1]])
+    assert N.surface_terms_subset(box, idx) == [terms[i] for i in idx]
