@@ -19,13 +19,21 @@ Prints one JSON line:
              inside the timed region, against 8 TB/s
   hbm_frac_step       the whole step's algorithmic bytes (this layout) / wall
                       / 8 TB/s
-  hbm_frac_step_b120  the metric's convention: 120 B per cell-update plus
-                      32 B per CPML slab cell and direction / wall / 8 TB/s
+  array_layout_equivalent_step_gbs
+                      the metric's byte convention (120 B per cell-update,
+                      the reference's 12 coefficient arrays, plus 32 B per
+                      CPML slab cell and direction) / wall: a bandwidth in
+                      that layout's bytes, not a fraction of HBM, since the
+                      material table does not move those coefficient bytes
   stream_copy         a 1 GiB device-to-device copy measured after the timed
                       region, and the step and kernel rates as fractions of it
   cpu_baseline  the oracle's OpenMP restatement of the reference's
              ParallelEngine (same arithmetic, AVX-512 when present) timed on
              the host cores on a bounded sample of the same workload
+  perf_point_512  the same workload at 512^3 (200 timed steps, its own
+             roofline record), after the headline and outside its timing
+  sweep      the 256-structure microstrip sweep to -50 dB or 20000 steps:
+             wall time and cell-updates per second
 """
 from __future__ import annotations
 
@@ -600,6 +608,172 @@ def bench_learned(args):
     print(json.dumps(out), flush=True)
 
 
+def yee_point(wl, mode, steps, warmup, period, tiles="", coef_opt="auto", fused=False, mode_name="physical"):
+    """Build the workload's engine, run `warmup` untimed steps, then time
+    `steps` steps (HIP events around every launch of every `period`-th step)
+    and return the measurements: MCells/s, the dominant Yee launch against
+    the HBM roofline, the whole step's algorithmic bytes per second."""
+    import numpy as np
+    import torch
+
+    log(f"[bench] building {wl.shape} with {len(wl.boxes)} boxes, CPML {wl.cpml_layers} ({mode_name})")
+    op, eng, exts = build_engine(wl, mode)
+    if tiles:
+        bx, by, ci, vec = (int(x) for x in tiles.split(","))
+        eng.set_tiles(bx, by, ci, vec)
+    rows = eng.material_table(0 if coef_opt == "arrays" else -1)
+    eng.fused_steps(1 if fused else 0)
+    coef = "table" if rows[0] and rows[1] else "arrays"
+    if coef_opt == "table" and coef != "table":
+        raise SystemExit(f"material table unavailable for this grid (rows {rows})")
+    dt = op.dt
+    log(f"[bench] tiles {eng.tiles()}; coefficients: {coef} (rows E/H {rows}); warmup {warmup} steps; "
+        f"launches timed every {period} steps")
+    # the warm-up runs with the timed region's launch timing, so that the
+    # timing events' first use falls outside the timed region
+    eng.set_profiling(period)
+    if warmup:
+        run_steps(eng, exts, wl, dt, 0, warmup)
+    kernel_names = ("update_h:grid", "update_e:grid", "update_h:interior", "update_e:interior", "update_h:slabs",
+                    "update_e:slabs", "fused:interior", "fused:grid", "point_ops", "energy")
+    before = {k: eng.kernel_time(k) for k in kernel_names}
+    with quiet_gc():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = run_steps(eng, exts, wl, dt, warmup, steps)
+        t1 = time.perf_counter()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    eng.set_profiling(0)
+    assert res.timesteps_executed == steps
+    log(f"[bench] timed {steps} steps in {wall:.3f} s (batch {res.elapsed_seconds:.3f} s, "
+        f"call {t1 - t0:.3f} s, device sync {wall - (t1 - t0):.4f} s)")
+
+    kern = {}
+    for name in kernel_names:  # launches timed inside the timed region only
+        n, ms = eng.kernel_time(name)
+        n, ms = n - before[name][0], ms - before[name][1]
+        if n:
+            kern[name] = (n, ms / n)
+    yee = [k for k in kern if k.startswith("update_") or k.startswith("fused:")]
+    dom = max(yee, key=lambda k: kern[k][0] * kern[k][1])
+    avg_ms = kern[dom][1]
+    kb = launch_bytes(dom, wl.shape, wl.cpml_layers, coef, rows)
+    # the reference's layout and scheme: 12 coefficient arrays, two in-place
+    # half-steps (a fused launch counts as both halves of its box)
+    kb_ref = launch_bytes(dom.replace("fused:", "update_h:"), wl.shape, wl.cpml_layers, "arrays")
+    if dom.startswith("fused:"):
+        kb_ref *= 2
+    achieved = kb / (avg_ms * 1e-3) / 1e9
+    for k, (n, ms) in kern.items():
+        log(f"[bench] {k}: {n} launches, mean {ms * 1e3:.1f} us")
+    dom_key = f"{dom}@{wl.name}{wl.shape[0]}" + ("" if coef == "arrays" else "/table")
+    E, H = eng.read_fields()
+    finite = bool(np.isfinite(E).all() and np.isfinite(H).all())
+    del E, H
+    # the metric's own convention: 120 B per cell-update (the reference's 12
+    # coefficient arrays, two half-steps) plus 32 B per CPML slab cell and
+    # slab direction: a bandwidth in that layout's bytes, not a fraction of
+    # HBM (the material table moves far fewer coefficient bytes)
+    nx, ny, nz = wl.shape
+    L = wl.cpml_layers
+    conv_bytes = 120 * wl.cells + 32 * 2 * L * (ny * nz + nx * nz + nx * ny)
+    step_bytes = (kernel_bytes(wl.shape, wl.cpml_layers, half_coef(coef, rows, True))
+                  + kernel_bytes(wl.shape, wl.cpml_layers, half_coef(coef, rows, False)))
+    if "fused:interior" in kern:
+        step_bytes = (launch_bytes("fused:interior", wl.shape, wl.cpml_layers, coef, rows)
+                      + launch_bytes("update_h:slabs", wl.shape, wl.cpml_layers, coef, rows)
+                      + launch_bytes("update_e:slabs", wl.shape, wl.cpml_layers, coef, rows))
+    elif "fused:grid" in kern:
+        step_bytes = launch_bytes("fused:grid", wl.shape, wl.cpml_layers, coef, rows)
+    step_gbs = step_bytes * steps / wall / 1e9
+    config = {
+        "workload": (f"plain_cavity: {wl.shape[0]}^3 FP32 Yee, 1 mm cells, vacuum, the reference's PEC faces "
+                     f"(no boundary extension), soft Ez Gaussian fc=15 GHz at the centre, energy sampled every "
+                     f"100 steps" if wl.name == "plain_cavity" else
+                     f"cpml_boxes: {wl.shape[0]}x{wl.shape[1]}x{wl.shape[2]} FP32 Yee, 0.5 mm cells, "
+                     f"{wl.cpml_layers}-cell CPML ({mode_name}) on all faces, {len(wl.boxes)} seeded dielectric "
+                     f"boxes, soft Ez Gaussian fc=15 GHz, energy sampled every 100 steps"),
+        "grid": list(wl.shape),
+        "tiles": eng.tiles(),
+        "coefficients": coef,
+        "step_scheme": "fused" if any(k.startswith("fused:") for k in kern) else "two half-steps",
+    }
+    roofline = {
+        "bound": "hbm",
+        "kernel": {"update_h:interior": "k_update_h<4,AX_Z,phys> (interior box, z-face CPML)",
+                   "update_e:interior": "k_update_e<4,AX_Z,phys> (interior box, z-face CPML)",
+                   "fused:interior": "k_fused<AX_Z,phys> (H and E of the interior box in one pass)"}.get(dom, dom),
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": measured_traffic(dom_key),
+        "bytes_per_launch": kb,
+        "mean_launch_us": round(avg_ms * 1e3, 2),
+        "launches_timed": kern[dom][0],
+        # the same launch priced at the reference's 12-array layout (60 B per
+        # cell): the bandwidth that layout would need for this launch time
+        "bytes_per_launch_array_layout": kb_ref,
+        "array_layout_equivalent_gbs": round(kb_ref / (avg_ms * 1e-3) / 1e9, 1),
+    }
+    eng.close()
+    return {"config": config, "steps": steps, "ms_per_step": round(wall / steps * 1e3, 4),
+            "mcells": wl.cells * steps / wall / 1e6, "hbm_frac_step": round(step_gbs / HBM_PEAK_GBS, 4),
+            "step_gbs": step_gbs,
+            "array_layout_equivalent_step_gbs": round(conv_bytes * steps / wall / 1e9, 1),
+            "roofline": roofline, "fields_finite": finite}
+
+
+def sweep_record(count, max_steps, autotune=True):
+    """The microstrip sweep (see bench_sweep): wall time and throughput."""
+    import numpy as np
+    import torch
+
+    from openems_amd import Sweep
+    from openems_amd import workloads as W
+
+    structs = W.microstrip_sweep(count, seed=4)
+    log(f"[bench] sweep: building {len(structs)} structures, {sum(m.cells for m in structs) / 1e6:.1f} M cells")
+    t_build = time.perf_counter()
+    sw = Sweep()
+    keep = []
+    for ms in structs:
+        op, eng, cp, port, exc = W.build_microstrip_hip(ms, max_steps)
+        sw.add(eng, cp, port, exc)
+        keep.append((ms, eng, port))
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t_build
+    log(f"[bench] sweep: built in {t_build:.1f} s; running")
+    with quiet_gc():
+        t0 = time.perf_counter()
+        res = sw.run(max_steps, energy_decay_db=50.0, check_interval=100, use_graph=True, autotune=autotune)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    steps = np.array([r.timesteps_executed for r in res])
+    cells = np.array([ms.cells for ms, _, _ in keep])
+    updates = float((steps * cells).sum())
+    s11 = [port.s11(50.0) for _, _, port in keep]
+    rec = {
+        "workload": f"microstrip sweep: {len(structs)} structures (seed 4), {cells.sum() / 1e6:.1f} M cells, "
+                    f"max {max_steps} steps, -50 dB decay (whole-run peak), CPML 8, 64-frequency port DFT",
+        "wall_s": round(wall, 3),
+        "build_s": round(t_build, 1),
+        "cell_updates_per_s": round(updates / wall / 1e6, 1),
+        "cell_updates_unit": "MCells/s",
+        "steps_min_median_max": [int(steps.min()), int(np.median(steps)), int(steps.max())],
+        "steps_total": int(steps.sum()),
+        "decayed": int(sum(r.termination_reason == 1 for r in res)),
+        "s11_finite": bool(all(np.isfinite(x).all() for x in s11)),
+        "tiles": sw.tuning(),
+    }
+    log(f"[bench] sweep: {rec}")
+    del sw
+    for _, eng, _ in keep:
+        eng.close()
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -608,8 +782,12 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="grid edge of the CPML/boxes workload")
     ap.add_argument("--mode", choices=["physical", "compat"], default="physical")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--prof-period", type=int, default=8,
-                    help="HIP events around every launch of every n-th timed step (1 = all)")
+    ap.add_argument("--prof-period", type=int, default=0,
+                    help="HIP events around every launch of every n-th timed step (1 = all; 0: every step up to "
+                         "64 timed steps, then at most every 8th)")
+    ap.add_argument("--no-subrecords", action="store_true",
+                    help="skip the 512^3 perf point and the sweep that the default line carries beside the headline")
+    ap.add_argument("--perf-point-steps", type=int, default=200)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--tiles", type=str, default="", help="bx,by,ci,vec override")
     ap.add_argument("--fused", action="store_true",
@@ -677,62 +855,8 @@ def main():
 
     wl = W.plain_cavity(steps=args.steps) if args.cavity else W.cpml_boxes(n=args.n, steps=args.steps)
     mode = MODE_PHYSICAL if args.mode == "physical" else MODE_COMPAT
-    log(f"[bench] building {wl.shape} with {len(wl.boxes)} boxes, CPML {wl.cpml_layers} ({args.mode})")
-    op, eng, exts = build_engine(wl, mode)
-    if args.tiles:
-        bx, by, ci, vec = (int(x) for x in args.tiles.split(","))
-        eng.set_tiles(bx, by, ci, vec)
-    rows = eng.material_table(0 if args.coef == "arrays" else -1)
-    eng.fused_steps(1 if args.fused else 0)
-    coef = "table" if rows[0] and rows[1] else "arrays"
-    if args.coef == "table" and coef != "table":
-        raise SystemExit(f"material table unavailable for this grid (rows {rows})")
-    dt = op.dt
-    log(f"[bench] tiles {eng.tiles()}; coefficients: {coef} (rows E/H {rows}); warmup {args.warmup} steps")
-    # the warm-up runs with the timed region's launch timing, so that the
-    # timing events' first use falls outside the timed region
-    eng.set_profiling(args.prof_period)
-    if args.warmup:
-        run_steps(eng, exts, wl, dt, 0, args.warmup)
-    kernel_names = ("update_h:grid", "update_e:grid", "update_h:interior", "update_e:interior", "update_h:slabs",
-                    "update_e:slabs", "fused:interior", "fused:grid", "point_ops", "energy")
-    before = {k: eng.kernel_time(k) for k in kernel_names}
-    with quiet_gc():
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        res = run_steps(eng, exts, wl, dt, args.warmup, args.steps)
-        t1 = time.perf_counter()
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-    eng.set_profiling(0)
-    assert res.timesteps_executed == args.steps
-    log(f"[bench] timed {args.steps} steps in {wall:.3f} s (batch {res.elapsed_seconds:.3f} s, "
-        f"call {t1 - t0:.3f} s, device sync {wall - (t1 - t0):.4f} s)")
-
-    kern = {}
-    for name in kernel_names:  # launches timed inside the timed region only
-        n, ms = eng.kernel_time(name)
-        n, ms = n - before[name][0], ms - before[name][1]
-        if n:
-            kern[name] = (n, ms / n)
-    yee = [k for k in kern if k.startswith("update_") or k.startswith("fused:")]
-    dom = max(yee, key=lambda k: kern[k][0] * kern[k][1])
-    avg_ms = kern[dom][1]
-    kb = launch_bytes(dom, wl.shape, wl.cpml_layers, coef, rows)
-    # the reference's layout and scheme: 12 coefficient arrays, two in-place
-    # half-steps (a fused launch counts as both halves of its box)
-    kb_ref = launch_bytes(dom.replace("fused:", "update_h:"), wl.shape, wl.cpml_layers, "arrays")
-    if dom.startswith("fused:"):
-        kb_ref *= 2
-    achieved = kb / (avg_ms * 1e-3) / 1e9
-    for k, (n, ms) in kern.items():
-        log(f"[bench] {k}: {n} launches, mean {ms * 1e3:.1f} us")
-    dom_key = f"{dom}@{wl.name}{wl.shape[0]}" + ("" if coef == "arrays" else "/table")
-    traffic = measured_traffic(dom_key)
-    E, H = eng.read_fields()
-    import numpy as np
-
-    finite = bool(np.isfinite(E).all() and np.isfinite(H).all())
+    period = args.prof_period if args.prof_period > 0 else max(1, min(8, args.steps // 64))
+    pt = yee_point(wl, mode, args.steps, args.warmup, period, args.tiles, args.coef, args.fused, args.mode)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -742,74 +866,44 @@ def main():
 
     copy_gbs = stream_copy_gbs()
     log(f"[bench] stream copy (1 GiB device-to-device): {copy_gbs:.1f} GB/s")
-    mcells = wl.cells * args.steps / wall / 1e6
-    # the metric's own convention: 120 B per cell-update (the reference's 12
-    # coefficient arrays, two half-steps) plus 32 B per CPML slab cell and
-    # slab direction, against 8 TB/s
-    nx, ny, nz = wl.shape
-    L = wl.cpml_layers
-    conv_bytes = 120 * wl.cells + 32 * 2 * L * (ny * nz + nx * nz + nx * ny)
-    step_bytes = (kernel_bytes(wl.shape, wl.cpml_layers, half_coef(coef, rows, True))
-                  + kernel_bytes(wl.shape, wl.cpml_layers, half_coef(coef, rows, False)))
-    if "fused:interior" in kern:
-        step_bytes = (launch_bytes("fused:interior", wl.shape, wl.cpml_layers, coef, rows)
-                      + launch_bytes("update_h:slabs", wl.shape, wl.cpml_layers, coef, rows)
-                      + launch_bytes("update_e:slabs", wl.shape, wl.cpml_layers, coef, rows))
-    elif "fused:grid" in kern:
-        step_bytes = launch_bytes("fused:grid", wl.shape, wl.cpml_layers, coef, rows)
     out = {
         "metric": METRIC,
-        "value": round(mcells, 1),
+        "value": round(pt["mcells"], 1),
         "unit": "MCells/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "ms_per_step": pt["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         # the reference's published 100^3 plain-Yee figure (its WGPU engine,
         # 1.39 ms per step = 718 Mcells/s, BENCHMARK_RESULTS.md:14-17): the
         # cavity workload only
-        "vs_baseline": round(mcells / REF_MCELLS_100, 2) if args.cavity else None,
+        "vs_baseline": round(pt["mcells"] / REF_MCELLS_100, 2) if args.cavity else None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {
-            "workload": (f"plain_cavity: {wl.shape[0]}^3 FP32 Yee, 1 mm cells, vacuum, the reference's PEC faces "
-                         f"(no boundary extension), soft Ez Gaussian fc=15 GHz at the centre, energy sampled every "
-                         f"100 steps" if args.cavity else
-                         f"cpml_boxes: {wl.shape[0]}x{wl.shape[1]}x{wl.shape[2]} FP32 Yee, 0.5 mm cells, "
-                         f"{wl.cpml_layers}-cell CPML ({args.mode}) on all faces, {len(wl.boxes)} seeded dielectric "
-                         f"boxes, soft Ez Gaussian fc=15 GHz, energy sampled every 100 steps"),
-            "grid": list(wl.shape),
-            "tiles": eng.tiles(),
-            "coefficients": coef,
-            "step_scheme": "fused" if any(k.startswith("fused:") for k in kern) else "two half-steps",
-        },
-        "hbm_frac_step": round(step_bytes * args.steps / wall / 1e9 / HBM_PEAK_GBS, 4),
-        "hbm_frac_step_b120": round(conv_bytes * args.steps / wall / 1e9 / HBM_PEAK_GBS, 4),
+        "config": pt["config"],
+        "hbm_frac_step": pt["hbm_frac_step"],
+        "array_layout_equivalent_step_gbs": pt["array_layout_equivalent_step_gbs"],
         "stream_copy": {"gbs": round(copy_gbs, 1), "bytes": 2 << 30,
-                        "step_frac_of_copy": round(step_bytes * args.steps / wall / 1e9 / copy_gbs, 4),
-                        "kernel_frac_of_copy": round(achieved / copy_gbs, 4)},
-        "roofline": {
-            "bound": "hbm",
-            "kernel": {"update_h:interior": "k_update_h<4,AX_Z,phys> (interior box, z-face CPML)",
-                       "update_e:interior": "k_update_e<4,AX_Z,phys> (interior box, z-face CPML)",
-                       "fused:interior": "k_fused<AX_Z,phys> (H and E of the interior box in one pass)"}.get(dom, dom),
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "bytes_per_launch": kb,
-            "mean_launch_us": round(avg_ms * 1e3, 2),
-            # the same launch priced at the reference's 12-array layout (60 B per
-            # cell): the bandwidth that layout would need for this launch time
-            "bytes_per_launch_array_layout": kb_ref,
-            "array_layout_equivalent_gbs": round(kb_ref / (avg_ms * 1e-3) / 1e9, 1),
-        },
-        "fields_finite": finite,
+                        "step_frac_of_copy": round(pt["step_gbs"] / copy_gbs, 4),
+                        "kernel_frac_of_copy": round(pt["roofline"]["achieved"] / copy_gbs, 4)},
+        "roofline": pt["roofline"],
+        "fields_finite": pt["fields_finite"],
         "cpu_baseline": cpu,
     }
+    if not args.no_subrecords and not args.cavity and args.n == 256:
+        # the north-star figures beside the headline: the 512^3 perf point of
+        # the same workload, and the 256-structure sweep
+        big = W.cpml_boxes(n=512, steps=args.perf_point_steps)
+        p5 = yee_point(big, mode, args.perf_point_steps, 20, 4, "", "auto", False, args.mode)
+        out["perf_point_512"] = {k: p5[k] for k in ("config", "steps", "ms_per_step", "mcells", "hbm_frac_step",
+                                                    "array_layout_equivalent_step_gbs", "roofline",
+                                                    "fields_finite")}
+        out["perf_point_512"]["mcells"] = round(p5["mcells"], 1)
+        del p5
+        gc.collect()
+        out["sweep"] = sweep_record(args.sweep_count, args.sweep_max_steps)
     print(json.dumps(out), flush=True)
 
 

@@ -529,6 +529,12 @@ oe_status oe_sweep_run(oe_sweep* sw, const oe_sweep_config* cfg, oe_sweep_entry*
  * step of all structures' Yee updates) for its four candidates. */
 oe_status oe_sweep_tuning(const oe_sweep* sw, int* ci, double ms_per_step[4], int candidates[4]);
 oe_status oe_sweep_set_ci(oe_sweep* sw, int ci);
+/* The energies the last run's checks saw for structure `index` (E + H sums
+ * of squares, Field3D::energy, src/arrays/field.rs:219-221), with their
+ * timesteps, in order: up to capacity entries (either array may be NULL);
+ * *n receives the count. */
+oe_status oe_sweep_energy_history(const oe_sweep* sw, size_t index, uint64_t* timesteps, double* energies,
+                                  size_t capacity, size_t* n);
 
 /* ----------------------------------------------------- learned extrapolator
  * No counterpart in the reference (whose only stop rule is the energy decay

@@ -168,6 +168,7 @@ SIGNATURES = {
     "oe_sweep_run": (_i, [_vp, C.POINTER(SweepConfig), C.POINTER(SweepEntry)]),
     "oe_sweep_tuning": (_i, [_vp, C.POINTER(_i), _dp, C.POINTER(_i)]),
     "oe_sweep_set_ci": (_i, [_vp, _i]),
+    "oe_sweep_energy_history": (_i, [_vp, _sz, C.POINTER(_u64), _dp, _sz, _szp]),
     "oe_extrap_create": (_i, [_i, C.POINTER(ExtrapDims), _fp, _u64, _pp]),
     "oe_extrap_destroy": (None, [_vp]),
     "oe_extrap_param_count": (_sz, [_vp]),

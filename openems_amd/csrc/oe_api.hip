@@ -150,7 +150,11 @@ std::vector<Launch> plan_launches(const oe_engine* e, const Tiles& t, const Cpml
     // slower (3 x 19 us against 31 us per half-step); the sweep, whose slab
     // launches carry every structure, does split them.
     std::vector<Box> slabs;
-    const Tiles& ts = t;  // the interior's depth measured best for the slabs too (ci 1/2/4/8 at 256^3)
+    Tiles ts = t;  // the interior's depth measured best for the slabs too (ci 1/2/4/8 at 256^3)
+    if (e->slab_tiles.bx > 0) {
+        ts = e->slab_tiles;
+        if (ts.vec == 4 && g.nz % 4 != 0) ts.vec = 1;
+    }
     if (ix0 > 0) slabs.push_back(make_box(ts, 0, ix0, 0, g.ny));
     if (ix1 < g.nx) slabs.push_back(make_box(ts, ix1, g.nx, 0, g.ny));
     if (ix1 > ix0) {
@@ -307,6 +311,12 @@ oe_status engine_alloc(oe_engine* e, size_t nx, size_t ny, size_t nz, int device
     if (const char* v = getenv("OE_FOLD_MUR")) e->fold_mur = atoi(v) != 0;
     if (const char* v = getenv("OE_MERGE_ADE")) e->merge_ade = atoi(v) != 0;
     if (const char* v = getenv("OE_BYTE_INDEX")) e->byte_index = atoi(v) != 0;
+    if (const char* v = getenv("OE_SLAB_TILES")) {  // "bx,by,ci,vec" of the CPML x/y slab launches
+        Tiles t{};
+        if (sscanf(v, "%d,%d,%d,%d", &t.bx, &t.by, &t.ci, &t.vec) == 4 && t.bx > 0 && t.bx <= 64 &&
+            (t.bx & (t.bx - 1)) == 0 && t.bx * t.by == 256 && t.ci >= 1 && (t.vec == 1 || t.vec == 4))
+            e->slab_tiles = t;
+    }
     return OE_OK;
 }
 
@@ -1785,6 +1795,9 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
             if (x->ade.mode == OE_MODE_PHYSICAL) ade_pre_m.a[ade_pre_m.n++] = x->ade;
             ade_post_m.a[ade_post_m.n++] = x->ade;
         }
+    for (const Launch& L : plan)
+        if (L.a.t.vec != 4) fold_mur = false;  // the folded Mur step needs float4 lanes
+    if (!fold_mur) mz = MurZ{};
     std::vector<Launch> plan_e = plan;  // the E launches carry the folded Mur faces
     for (Launch& L : plan_e) L.a.mur_z = mz;
     // Two-half-step batches advance the device step counter in their first H

@@ -350,6 +350,16 @@ __global__ __launch_bounds__(256) void k_phase(float* __restrict__ A, int ld, si
     }
 }
 
+// Zero rows [np, K) of B (the padding the contraction's K tiles cover), so
+// that whatever an earlier face or call left there -- a non-finite value
+// from a diverged run included -- cannot reach a product: A's padding
+// columns are zeros, and 0 * NaN would still be NaN.
+__global__ __launch_bounds__(256) void k_nf2ff_pad_rows(float* __restrict__ B, int ldb, int np, int K) {
+    const size_t n = (size_t)(K - np) * ldb;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256)
+        B[(size_t)np * ldb + e] = 0.f;
+}
+
 // B = [Br | Bi] for one face: row ip, columns 4 iq + c and 4 nq + 4 iq + c
 // hold w_p[ip] times the equivalent current c = (J_a1, J_a2, M_a1, M_a2) at
 // node (i1, i2) (ip = i1, iq = i2, or swapped when swap = 1), with J = n x H
@@ -798,10 +808,10 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
     double* d_part = d_coord + coord_len;
     float* Bm = reinterpret_cast<float*>(reinterpret_cast<char*>(C.work.p) + round_up(n_dbl * sizeof(double), 256));
     // B is zeroed only when its region moves or grows.  Its padding rows
-    // (p >= np) are never cleared per face: they keep finite values (zeros
-    // or an earlier face's entries), and A's padding columns are exact zeros
-    // (k_phase), so they add signed zeros to every product; B's padding
-    // columns only reach G columns the reduction never reads.
+    // (p >= np) are cleared per face after the pack (k_nf2ff_pad_rows), and
+    // A's padding columns are exact zeros (k_phase), so they add signed zeros
+    // to every product; B's padding columns only reach G columns the
+    // reduction never reads.
     {
         const size_t off = round_up(n_dbl * sizeof(double), 256), len = Kmax * Nmax * sizeof(float);
         if (off != C.bm_off || len > C.bm_len) {
@@ -873,6 +883,12 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
             hipLaunchKernelGGL(k_nf2ff_pack<1>, dim3(nb), dim3(256), 0, s, f.src, f.scale, n1, n2, pl.swap, f.sign,
                                wp, wq, Bm, pl.N, d_part + bo, d_pmax + bo);
         NF_OK(hipGetLastError());
+        if (pl.K > pl.np) {
+            const size_t npad = (size_t)(pl.K - pl.np) * pl.N;
+            hipLaunchKernelGGL(k_nf2ff_pad_rows, dim3((unsigned)std::min<size_t>((npad + 255) / 256, 1024)), dim3(256),
+                               0, s, Bm, pl.N, pl.np, pl.K);
+            NF_OK(hipGetLastError());
+        }
         const float* face_pmax = d_pmax + bo;
         bo += nb;
         const int uchunk = (int)(rows / 2);

@@ -131,6 +131,7 @@ struct oe_engine {
     uint8_t* mat8 = nullptr;  // [E: N][H: N] the indices as bytes, when a half has 2 to 256 rows
     uint64_t timestep = 0;
     Tiles tiles{};
+    Tiles slab_tiles{};  // tiles of the CPML x/y slab launches (bx = 0: the interior's)
     int xcd_remap = 1;
     double peak_persistent = 0.0;
     oe_extension* cpml_owner = nullptr;

@@ -65,6 +65,8 @@ struct oe_sweep {
     double* energy_host = nullptr;
     size_t energy_host_n = 0;
     std::vector<Group> groups;
+    // per structure, the (timestep, E + H energy) of every check of the last run
+    std::vector<std::vector<std::pair<uint64_t, double>>> history;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_steps = 0;
@@ -415,6 +417,7 @@ oe_status oe_sweep_run(oe_sweep* sw, const oe_sweep_config* cfg, oe_sweep_entry*
     for (int s = 0; s < n; ++s) active[s] = s;
     std::vector<double> peak(n, 0.0);
     for (int s = 0; s < n; ++s) peak[s] = sw->items[s].eng->peak_persistent;
+    sw->history.assign(n, {});
     for (int s = 0; s < n; ++s) {
         results[s] = oe_sweep_entry{};
         results[s].termination_reason = OE_TERM_STEPS_COMPLETED;
@@ -495,6 +498,7 @@ oe_status oe_sweep_run(oe_sweep* sw, const oe_sweep_config* cfg, oe_sweep_entry*
             const int s = active[q];
             const double E = sw->energy_host[2 * q] + sw->energy_host[2 * q + 1];
             peak[s] = std::max(peak[s], E);
+            sw->history[s].push_back({cur, E});
             results[s].final_energy = E;
             results[s].peak_energy = peak[s];
             bool stop = false;
@@ -543,6 +547,20 @@ oe_status oe_sweep_tuning(const oe_sweep* sw, int* ci, double ms_per_step[4], in
     for (int q = 0; q < 4; ++q) {
         if (ms_per_step) ms_per_step[q] = sw->tune_ms[q];
         if (candidates) candidates[q] = sw->tune_ci[q];
+    }
+    return OE_OK;
+}
+
+oe_status oe_sweep_energy_history(const oe_sweep* sw, size_t index, uint64_t* timesteps, double* energies,
+                                  size_t capacity, size_t* n) {
+    if (!sw) return fail(OE_ERR_CONFIG, "null argument");
+    if (index >= sw->history.size()) return fail(OE_ERR_CONFIG, "structure %zu has no history (run the sweep first)", index);
+    const auto& h = sw->history[index];
+    if (n) *n = h.size();
+    const size_t m = std::min(capacity, h.size());
+    for (size_t q = 0; q < m; ++q) {
+        if (timesteps) timesteps[q] = h[q].first;
+        if (energies) energies[q] = h[q].second;
     }
     return OE_OK;
 }

@@ -417,6 +417,17 @@ class Sweep:
     def set_ci(self, ci: int):
         check(lib().oe_sweep_set_ci(self._h, int(ci)))
 
+    def energy_history(self, index: int):
+        """(timesteps, energies) of structure `index` at every check of the last run."""
+        n = C.c_size_t()
+        check(lib().oe_sweep_energy_history(self._h, int(index), None, None, 0, C.byref(n)))
+        ts = np.zeros(n.value, np.uint64)
+        en = np.zeros(n.value, np.float64)
+        if n.value:
+            check(lib().oe_sweep_energy_history(self._h, int(index), ts.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                en.ctypes.data_as(C.POINTER(C.c_double)), n.value, C.byref(n)))
+        return ts.astype(np.int64), en
+
 
 def port_excitations(direction: int, region, samples: np.ndarray, sign: int = 1, amplitude: float = 1.0):
     """The excitation box of a lumped port: a soft source of

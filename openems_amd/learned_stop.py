@@ -108,6 +108,31 @@ def tail_dft(s, a, start: int, k0: int, k1: int, decimate: int, dt: float, offse
     return out
 
 
+def refit_residues(x: np.ndarray, s: np.ndarray, k0: int, k1: int, ref: int, decimate: int, stride: int = 1):
+    """Least-squares residues for given poles: the complex a [M] minimising
+    sum_k (x[k] - Re sum_m a_m exp(s_m (k - ref) / decimate))^2 over
+    k = k0, k0 + stride, ... < k1 (s per decimated sample).  Columns are
+    scaled to unit norm before the solve (poles of very different damping
+    make them span many decades); returns (a, relative residual)."""
+    k = np.arange(k0, k1, stride)
+    A = np.exp(np.outer((k - ref) / decimate, s))               # [K, M] complex
+    D = np.concatenate([A.real, -A.imag], axis=1)               # x = D [Re a; Im a]
+    sc = np.linalg.norm(D, axis=0)
+    sc[sc == 0] = 1.0
+    y = x[k0:k1:stride].astype(np.float64)
+    sol, *_ = np.linalg.lstsq(D / sc, y, rcond=1e-12)
+    sol = sol / sc
+    M = len(s)
+    a = sol[:M] + 1j * sol[M:]
+    r = float(np.sum((D @ sol - y) ** 2) / max(np.sum(y * y), 1e-300))
+    return a, r
+
+
+def modes_eval(s: np.ndarray, a: np.ndarray, k: np.ndarray, ref: int, decimate: int) -> np.ndarray:
+    """Re sum_m a_m exp(s_m (k - ref) / decimate) at the steps k."""
+    return np.real(np.exp(np.outer((np.asarray(k) - ref) / decimate, s)) @ a)
+
+
 def calc_s11(U, I, z_ref=50.0):
     inc = 0.5 * (U + I * z_ref)
     return (U - inc) / inc

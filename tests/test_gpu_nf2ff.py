@@ -237,3 +237,52 @@ def test_split_fp16_contraction_against_oracle_and_fp32(monkeypatch):
     assert err <= PATTERN_TOL
     assert d32 <= 1e-6
     assert abs(g.prad - prad) <= 1e-12 * abs(prad)
+
+
+def test_fp32_contraction_after_non_finite_phasors(monkeypatch):
+    """A recording box whose phasors were non-finite (a diverged run) and
+    then finite again (engine and extension reset, run again) gives the far
+    field of a fresh box, bit for bit: the transform's reused buffers keep
+    nothing from the earlier call (the FP32 contraction's padding rows are
+    cleared per face)."""
+    monkeypatch.setenv("OE_NF2FF_SPLIT16", "0")
+    shape, steps, freqs = (40, 36, 32), 160, (20e9,)
+    box = (10, 30, 9, 27, 8, 24)
+    th = np.linspace(0, math.pi, 19)
+    ph = np.linspace(0, 2 * math.pi, 25)[:-1]
+    t0, tau = X.openems_gaussian_params(30e9)
+
+    def build():
+        op = Operator.uniform(*shape, 0.5e-3)
+        eng = HipEngine(op)
+        cp = eng.cpml(CpmlBoundaries(layers=(6,) * 6, mode=MODE_PHYSICAL))
+        nf = eng.nf2ff_box(box, freqs)
+        return eng, cp, nf, X.schedule(lambda t: X.gaussian(t, t0, tau), 0, steps, op.dt)
+
+    def run(eng, cp, nf, smp, extra=()):
+        eng.run_batch(EngineBatch(num_steps=steps, excitations=[ScheduledExcitation((20, 18, 16), 2, smp, True)],
+                                  extensions=[cp, nf, *extra]))
+
+    eng, cp, nf, smp = build()
+
+    def poison(which, step, f):
+        if step == 40:
+            for c in range(6):
+                f[c][...] = np.nan
+
+    run(eng, cp, nf, smp, [eng.host_callback(poison, hooks=("post_update_e",))])
+    bad = nf.far_field(0, th, ph)
+    assert not np.all(np.isfinite(bad.e_theta))
+    eng.reset()
+    cp.reset()
+    nf.reset()
+    run(eng, cp, nf, smp)
+    again = nf.far_field(0, th, ph)
+    e2, cp2, nf2, _ = build()
+    run(e2, cp2, nf2, smp)
+    fresh = nf2.far_field(0, th, ph)
+    say(f"[nf2ff] after non-finite phasors: finite {bool(np.all(np.isfinite(again.e_theta)))}, "
+        f"max |E_theta| {np.abs(fresh.e_theta).max():.3e}")
+    assert np.array_equal(nf.dft_values(), nf2.dft_values())
+    assert np.array_equal(again.e_theta, fresh.e_theta) and np.array_equal(again.e_phi, fresh.e_phi)
+    assert again.prad == fresh.prad and np.abs(fresh.e_theta).max() > 0
