@@ -625,6 +625,24 @@ oe_status oe_ext_cpml_read_psi(oe_extension* ext, int face, int field, int which
                                size_t capacity, size_t* n);
 oe_status oe_ext_ade_read_state(oe_extension* ext, float* out, size_t capacity, size_t* n);
 
+/* ------------------------------------------------------- no allocation
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the float64 sums of squares of a caller-owned field
+ * block, the computation of oe_engine_energy (Field3D::energy,
+ * src/arrays/field.rs:219-221), which allocates nothing.  fields is a device
+ * pointer to 6 n floats on `device` (Ex, Ey, Ez, Hx, Hy, Hz, n each: the
+ * engine's layout); out is a device pointer to two doubles that receive the E
+ * and H sums, written on `stream` (a hipStream_t; NULL is the null stream)
+ * without a host synchronisation, so the call can be captured into a graph.
+ * Results equal oe_engine_energy's for the same values bit for bit.  The
+ * partial sums live in a 16 KB array in the library's device memory, one per
+ * GPU, held until the process ends; every call on a GPU shares it, so calls
+ * whose launches overlap on one GPU (on different streams, or from several
+ * threads) race.  1 <= n <= OE_NOALLOC_MAX_CELLS, otherwise OE_ERR_GRID. */
+#define OE_NOALLOC_MAX_CELLS (((uint64_t)1) << 34)
+oe_status oe_fields_energy_noalloc(const float* fields, size_t n, double* out, int device, void* stream);
+
 /* Extension::pre_batch/post_batch reset counterpart: zero the state. */
 oe_status oe_extension_reset(oe_extension* ext);
 void oe_extension_destroy(oe_extension* ext);

@@ -183,6 +183,7 @@ SIGNATURES = {
     "oe_ext_ade_read_state": (_i, [_vp, _fp, _sz, _szp]),
     "oe_ext_host_callback_create": (_i, [_vp, C.POINTER(HostCallbackConfig), _pp]),
     "oe_ext_host_callback_reason": (_i, [_vp, C.c_char_p, _sz, _szp]),
+    "oe_fields_energy_noalloc": (_i, [_vp, _sz, _vp, _i, _vp]),
     "oe_extension_reset": (_i, [_vp]),
     "oe_extension_destroy": (None, [_vp]),
 }
