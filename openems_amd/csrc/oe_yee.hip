@@ -156,8 +156,12 @@ __device__ __forceinline__ void coef_rows(const MatIndex& mi, size_t p, const fl
     }
 }
 
+// kfirst / klast: the lane's left / right k neighbour along the row is not
+// the adjacent lane of its wave (row ends, wave ends), so the neighbour value
+// is loaded instead of taken from that lane.
 struct Tile {
     int k0, j, i0, i1, jend;
+    bool kfirst, klast;
 };
 
 // Block id -> (box, k-block, j-block, i-chunk).  Hardware hands consecutive
@@ -190,6 +194,8 @@ __device__ __forceinline__ Tile decode(const YeeArgs& a) {
     r.i0 = B.i0 + ib * a.t.ci;
     r.i1 = min(r.i0 + a.t.ci, B.i1);
     r.jend = B.j1;
+    r.kfirst = kx == 0 || (lane & 63) == 0;
+    r.klast = kx == a.t.bx - 1 || (lane & 63) == 63;
     return r;
 }
 
@@ -332,10 +338,14 @@ __device__ __forceinline__ void h_body(const Grid& g, const float* Fin, float* F
             ez_j = ldv<V>(Ez + p + nz);
             ex_j = ldv<V>(Ex + p + nz);
         }
+        // the k + V neighbour: the next lane's first cell, or a load where
+        // that lane holds another row (or sits in another wave)
+        const float ey_sh = __shfl_down(ey_c.v[0], 1, 64);
+        const float ex_sh = __shfl_down(ex_c.v[0], 1, 64);
         float ey_kl = 0.f, ex_kl = 0.f;
         if (hk_last) {
-            ey_kl = Ey[p + V];
-            ex_kl = Ex[p + V];
+            ey_kl = t.klast ? Ey[p + V] : ey_sh;
+            ex_kl = t.klast ? Ex[p + V] : ex_sh;
         }
         Vec<V> hx = ldn<V, OE_NT_OWN>(Hx + p), hy = ldn<V, OE_NT_OWN>(Hy + p), hz = ldn<V, OE_NT_OWN>(Hz + p);
         Vec<V> a0, a1, a2, b0, b1, b2;
@@ -497,10 +507,14 @@ __device__ __forceinline__ void e_body(const Grid& g, const float* Fin, float* F
             hz_j = ldv<V>(Hz + p - nz);
             hx_j = ldv<V>(Hx + p - nz);
         }
+        // the k - 1 neighbour: the previous lane's last cell, or a load where
+        // that lane holds another row (or sits in another wave)
+        const float hy_sh = __shfl_up(hy_c.v[V - 1], 1, 64);
+        const float hx_sh = __shfl_up(hx_c.v[V - 1], 1, 64);
         float hy_kf = 0.f, hx_kf = 0.f;
         if (gk_first) {
-            hy_kf = Hy[p - 1];
-            hx_kf = Hx[p - 1];
+            hy_kf = t.kfirst ? Hy[p - 1] : hy_sh;
+            hx_kf = t.kfirst ? Hx[p - 1] : hx_sh;
         }
         Vec<V> ex, ey, ez, c0, c1, c2, s0, s1, s2;
         if (row) {
@@ -683,6 +697,8 @@ __device__ __forceinline__ bool decode_sweep(const SweepArgs& a, Tile& r, int& s
     // surplus lanes (beyond ns chunks, or a chunk past the box) get an empty
     // row range and return before any load
     r.jend = (sub < B.ns && r.i0 < B.i1) ? B.j1 : r.j;
+    r.kfirst = kx == 0 || (threadIdx.x & 63) == 0;
+    r.klast = kx == B.bx - 1 || (threadIdx.x & 63) == 63;
     sid = B.sid;
     return true;
 }

@@ -1,22 +1,24 @@
 """Learned-stop sweeps: stop each structure early and continue its port
-signals with the extrapolator's modes instead of running to -50 dB.
+signals with damped modes whose poles the extrapolator predicts, instead of
+running to -50 dB.
 
   plain run    every structure runs until its energy is 50 dB below its
                whole-run peak (or max_steps); S11 from the port's running DFT
   learned run  all structures run together to a checkpoint P (first
                window + check, then every `stride` steps).  At P the network
                maps the decimated V and I of [P - check - window, P - check)
-               to damped modes; their prediction over [P - check, P) is
-               compared with the simulated samples there: if the squared
-               error is at most `max_residual` times the energy of the window
-               and the energy the modes predict after P is at most
-               `max_tail_energy` of the energy recorded so far, the
-               structure stops and its S11 combines the simulated DFT
-               over [0, P) with the closed-form DFT of the predicted modes
-               from P to max_steps.  The others continue from their state at
-               P (same fields, DFT sums, energy peak and decay-check steps as
-               the plain run), so a structure that is never accepted ends
-               exactly as in the plain run, with the same S11.
+               to damped modes; their poles are kept and the residues of V
+               and I refitted by least squares to the simulated samples.  A
+               structure stops when the refitted modes reproduce the held-out
+               steps [P - check, P), the energy they predict after P is small,
+               and S11 hardly moves when the predicted stop moves; its S11 is
+               the simulated DFT over [0, P) plus the closed-form DFT of the
+               modes from P to the step at which the plain run's decay test
+               is predicted to stop it (the plain run's own horizon).  The
+               others continue from their state at P (same fields, DFT sums,
+               energy peak and decay-check steps as the plain run), so a
+               structure that is never accepted ends exactly as in the plain
+               run, with the same S11.
 
 The DFT of a mode sum is a geometric series per mode (U sampled at (k + 1)
 dt, I at (k + 1/2) dt, as the engine samples E and H), so no tail samples
@@ -115,16 +117,24 @@ def refit_residues(x: np.ndarray, s: np.ndarray, k0: int, k1: int, ref: int, dec
     scaled to unit norm before the solve (poles of very different damping
     make them span many decades); returns (a, relative residual)."""
     k = np.arange(k0, k1, stride)
-    A = np.exp(np.outer((k - ref) / decimate, s))               # [K, M] complex
+    t = (k - ref) / decimate
+    # each column divided by its largest modulus first (exp of the largest
+    # real exponent over the range), so no column overflows
+    shift = np.maximum(s.real * t[0], s.real * t[-1])
+    A = np.exp(np.outer(t, s) - shift)                            # [K, M] complex
     D = np.concatenate([A.real, -A.imag], axis=1)               # x = D [Re a; Im a]
     sc = np.linalg.norm(D, axis=0)
     sc[sc == 0] = 1.0
     y = x[k0:k1:stride].astype(np.float64)
     sol, *_ = np.linalg.lstsq(D / sc, y, rcond=1e-12)
     sol = sol / sc
+    fit = D @ sol
     M = len(s)
-    a = sol[:M] + 1j * sol[M:]
-    r = float(np.sum((D @ sol - y) ** 2) / max(np.sum(y * y), 1e-300))
+    with np.errstate(over="ignore", under="ignore"):
+        g = np.exp(-shift)
+    a = (sol[:M] + 1j * sol[M:]) * g
+    a[~np.isfinite(a)] = 0.0
+    r = float(np.sum((fit - y) ** 2) / max(np.sum(y * y), 1e-300))
     return a, r
 
 
@@ -162,14 +172,54 @@ def _window(u: np.ndarray, i: np.ndarray, a: int, window: int, decimate: int, n_
     return x.astype(np.float32), chk, (sv, si)
 
 
+def predict_stop(timesteps: np.ndarray, energies: np.ndarray, pos: int, max_steps: int, decay_db: float = 50.0,
+                 check: int = 100, last: int = 10) -> int:
+    """The step count at which the plain run's decay test
+    (src/fdtd/basic_engine.rs:128-141, whole-run peak) is predicted to stop:
+    log-energy over the last `last` checks before `pos` extrapolated as a
+    line to decay_db below the peak, rounded up to the next check (the run
+    then holds that check's step, hence + 1); max_steps if the energy does
+    not fall."""
+    m = timesteps < pos
+    ts, en = timesteps[m].astype(np.float64), energies[m]
+    if len(ts) < 3 or not np.all(en > 0):
+        return max_steps
+    thr = en.max() * 10.0 ** (-decay_db / 10.0)
+    b, a = np.polyfit(ts[-last:], np.log(en[-last:]), 1)
+    if not b < 0:
+        return max_steps
+    t_cross = (math.log(thr) - a) / b
+    c = max(int(math.ceil(t_cross / check)) * check, (pos // check + 1) * check)
+    return int(min(c + 1, max_steps))
+
+
+def _passes(value: float, bar: float) -> bool:
+    return bar == np.inf or (np.isfinite(value) and value <= bar)
+
+
 def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, decimate: int, check: int,
-                  max_residual: float = 1e-4, freqs=None, decay_db: float = 50.0, stride: Optional[int] = None,
-                  decay_check: int = 100, max_tail_energy: float = 1e-4) -> LearnedResult:
-    """A structure stops at a checkpoint when (1) the modes reproduce the
-    check steps to max_residual of the window energy and (2) the energy the
-    modes predict from the checkpoint to max_steps is at most
-    max_tail_energy of the energy recorded so far (both V and I, decimated,
-    in units of the window peaks)."""
+                  max_residual: float = 1e-6, freqs=None, decay_db: float = 50.0, stride: Optional[int] = None,
+                  decay_check: int = 100, max_tail_energy: float = 1e-4, horizon_window: int = 300,
+                  max_horizon_change: float = 3e-4) -> LearnedResult:
+    """All structures run together from checkpoint to checkpoint (the first
+    at window + check steps, then every `stride`).  At a checkpoint P the
+    network maps the decimated V and I of [P - check - window, P - check) to
+    damped modes; their poles are kept and the residues of V and I refitted
+    by least squares to the simulated samples of that range
+    (refit_residues).  A structure stops when
+      (1) the refitted modes reproduce the held-out steps [P - check, P) to
+          max_residual of the window's energy (V and I each),
+      (2) the energy they predict from P to the predicted stop is at most
+          max_tail_energy of the energy recorded so far, and
+      (3) S11 moves by at most max_horizon_change when the predicted stop
+          moves by horizon_window steps either way;
+    the predicted stop is where the plain run's decay test would end it
+    (predict_stop on the energies of every check so far).  Its S11 is the
+    simulated DFT over [0, P) plus the closed-form DFT of the modes
+    (residues refitted over [P - window, P)) from P to the predicted stop,
+    the plain run's own horizon.  The other structures continue from their
+    state at P with the plain run's fields, DFT sums, energy peak and check
+    steps, so one that never stops ends exactly as the plain run does."""
     d = model.dims
     early = window + check
     stride = stride or window
@@ -182,7 +232,8 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
     resid = np.full(n, np.nan)
     tail_frac = np.full(n, np.nan)
     steps = np.zeros(n, np.int64)
-    n_chk = check // decimate
+    hist_t = [[] for _ in range(n)]
+    hist_e = [[] for _ in range(n)]
     live = list(range(n))
     pos = 0
     ci = None
@@ -206,6 +257,9 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
         still = []
         for k, q in enumerate(live):
             steps[q] += res[k].timesteps_executed
+            ts, en = sw.energy_history(k)
+            hist_t[q].append(ts)
+            hist_e[q].append(en)
             if res[k].termination_reason != 0 or pos >= max_steps:
                 s11[q] = built[q][3].s11(50.0)
             else:
@@ -215,47 +269,102 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
         checkpoints.append(pos)
         t1 = time.perf_counter()
         X = np.zeros((len(still), 2, d.L0), np.float32)
-        chk, scales, dfts, sers = [], [], [], []
+        sers, dfts = [], []
         for k, q in enumerate(still):
             port = built[q][3]
             ser = port.series()
-            X[k], c, sc = _window(ser[:, 0], ser[:, 1], pos - early, window, decimate, n_chk)
-            chk.append(c)
-            scales.append(sc)
+            X[k], _, _ = _window(ser[:, 0], ser[:, 1], pos - early, window, decimate, 0)
             sers.append(ser)
             dfts.append(port.dft_values())
         modes = np.zeros((len(still), 6 * d.modes), np.float32)
         for b0 in range(0, len(still), d.batch):
             m, _ = model.forward(X[b0:b0 + d.batch], tails=False)
             modes[b0:b0 + d.batch] = m
-        s, aV, aI = mode_params(modes, d.modes)
-        # the modes over the check steps and from the checkpoint to max_steps,
-        # all structures at once: x[k, n] = Re sum_m a[k, m] exp(s[k, m] n)
-        n_all = np.arange(n_chk + (max_steps - pos) // decimate)
-        zn = np.exp(s[:, :, None] * n_all)                       # [K, M, n]
-        pv = np.real(np.einsum("km,kmn->kn", aV, zn))
-        pi = np.real(np.einsum("km,kmn->kn", aI, zn))
+        s_all, _, _ = mode_params(modes, d.modes)
         rejected = []
+        a0, a1 = pos - early, pos - check
         for k, q in enumerate(still):
-            sv, si = scales[k]
-            e_win = (X[k].astype(np.float64) ** 2).sum()
-            nc = len(chk[k][0])
-            resid[q] = float(((pv[k, :nc] - chk[k][0]) ** 2 + (pi[k, :nc] - chk[k][1]) ** 2).sum() / max(e_win, 1e-30))
-            e_tail = float((pv[k, n_chk:] ** 2).sum() + (pi[k, n_chk:] ** 2).sum())
-            e_run = float((sers[k][:pos:decimate, 0] / sv) @ (sers[k][:pos:decimate, 0] / sv)
-                          + (sers[k][:pos:decimate, 1] / si) @ (sers[k][:pos:decimate, 1] / si))
-            tail_frac[q] = e_tail / max(e_run, 1e-300)
-            if resid[q] <= max_residual and tail_frac[q] <= max_tail_energy:
-                accepted[q] = True
-                a = pos - check
-                U = dfts[k][:, 0] + sv * tail_dft(s[k], aV[k], a, pos, max_steps, decimate, dt, 1.0, fr)
-                I = dfts[k][:, 1] + si * tail_dft(s[k], aI[k], a, pos, max_steps, decimate, dt, 0.5, fr)
-                s11[q] = calc_s11(U, I)
-            else:
+            s = s_all[k]
+            u, i = sers[k][:, 0], sers[k][:, 1]
+            aU, _ = refit_residues(u, s, a0, a1, a1, decimate)
+            aI, _ = refit_residues(i, s, a0, a1, a1, decimate)
+            kk = np.arange(a1, pos)
+            eu = float(np.sum(u[a0:a1] ** 2))
+            ei = float(np.sum(i[a0:a1] ** 2))
+            ru = float(np.sum((modes_eval(s, aU, kk, a1, decimate) - u[a1:pos]) ** 2)) / max(eu, 1e-300)
+            ri = float(np.sum((modes_eval(s, aI, kk, a1, decimate) - i[a1:pos]) ** 2)) / max(ei, 1e-300)
+            resid[q] = max(ru, ri)
+            ok = _passes(resid[q], max_residual)
+            if ok:
+                aU, _ = refit_residues(u, s, pos - window, pos, pos, decimate)
+                aI, _ = refit_residues(i, s, pos - window, pos, pos, decimate)
+                stop = predict_stop(np.concatenate(hist_t[q]), np.concatenate(hist_e[q]), pos, max_steps, decay_db,
+                                    decay_check)
+                sv = max(float(np.abs(u[a0:pos]).max()), 1e-300)
+                si = max(float(np.abs(i[a0:pos]).max()), 1e-300)
+                kt = np.arange(pos, stop, decimate)
+                e_tail = float(np.sum((modes_eval(s, aU, kt, pos, decimate) / sv) ** 2)
+                               + np.sum((modes_eval(s, aI, kt, pos, decimate) / si) ** 2))
+                e_run = float(np.sum((u[:pos:decimate] / sv) ** 2) + np.sum((i[:pos:decimate] / si) ** 2))
+                tail_frac[q] = e_tail / max(e_run, 1e-300)
+                ok = _passes(tail_frac[q], max_tail_energy)
+
+                def s11_to(end):
+                    U = dfts[k][:, 0] + tail_dft(s, aU, pos, pos, end, decimate, dt, 1.0, fr)
+                    I = dfts[k][:, 1] + tail_dft(s, aI, pos, pos, end, decimate, dt, 0.5, fr)
+                    return calc_s11(U, I)
+
+                if ok:
+                    s_pred = s11_to(stop)
+                    change = max(float(np.abs(s11_to(max(pos, stop - horizon_window)) - s_pred).max()),
+                                 float(np.abs(s11_to(min(max_steps, stop + horizon_window)) - s_pred).max()))
+                    ok = _passes(change, max_horizon_change)
+                if ok:
+                    accepted[q] = True
+                    s11[q] = s_pred
+            if not ok:
                 rejected.append(q)
         t_inf += time.perf_counter() - t1
         live = rejected
     return LearnedResult(s11, accepted, resid, wall + t_inf, t_inf, steps, checkpoints, tail_frac)
+
+
+def refit_tail_error(model: Extrapolator, X: np.ndarray, T: np.ndarray, poles: Optional[np.ndarray] = None):
+    """Tail error of modes whose residues are refitted to the window
+    (refit_residues on the decimated window samples), divided by that of
+    predicting zero, over the whole tail.  The poles are the network's for
+    each window, or the given [M] poles for every window (for instance the
+    untrained frequency-spread start).  X [B, 2, L0] and T [B, 2, tail] in
+    the network's normalised units."""
+    B = model.dims.batch
+    err = zero = 0.0
+    L0, tail = X.shape[2], T.shape[2]
+    n_in = np.arange(L0)
+    n_out = np.arange(L0, L0 + tail)
+    for b0 in range(0, len(X), B):
+        xb = X[b0:b0 + B]
+        nb = len(xb)
+        if poles is None:
+            pad = xb if nb == B else np.concatenate([xb, np.zeros((B - nb,) + xb.shape[1:], xb.dtype)])
+            m, _ = model.forward(pad, tails=False)
+            s_all, _, _ = mode_params(m[:nb], model.dims.modes)
+        else:
+            s_all = np.broadcast_to(poles, (nb, len(poles)))
+        for k in range(nb):
+            for ch in range(2):
+                a, _ = refit_residues(xb[k, ch].astype(np.float64), s_all[k], 0, L0, L0, 1)
+                pred = modes_eval(s_all[k], a, n_out, L0, 1)
+                err += float(np.sum((pred - T[b0 + k, ch]) ** 2))
+                zero += float(np.sum(T[b0 + k, ch].astype(np.float64) ** 2))
+    return err / max(zero, 1e-300)
+
+
+def spread_poles(model: Extrapolator) -> np.ndarray:
+    """The poles the output layer starts from (spread_modes): the network's
+    modes for a zero input, before training."""
+    d = model.dims
+    m, _ = model.forward(np.zeros((d.batch, 2, d.L0), np.float32), tails=False)
+    return mode_params(m[:1], d.modes)[0][0]
 
 
 def dataset(run: SweepRun, window: int, decimate: int, tail: int, offsets=(0,)):

@@ -57,3 +57,30 @@ def test_tail_schedule():
     assert set(s[:7]) <= {16, 32, 64, 128, 256}
     assert tail_schedule(4, 10) == [10] * 4          # shorter than the first stage
     assert tail_schedule(3, 64, first=16, share=0.0) == [64] * 3
+
+
+def test_refit_residues_recovers_modes_and_ignores_huge_damping():
+    from openems_amd.learned_stop import modes_eval, refit_residues
+    rng = np.random.default_rng(1)
+    s = np.array([-0.01 + 2j * math.pi * 0.05, -0.003 + 2j * math.pi * 0.11, -0.02 + 2j * math.pi * 0.3])
+    a = rng.standard_normal(3) + 1j * rng.standard_normal(3)
+    k = np.arange(3000)
+    x = modes_eval(s, a, k, 2000, 8)
+    # an extra pole decaying by e^-50 per sample overflows a naive design matrix
+    got, r = refit_residues(x, np.append(s, -50 + 1j), 0, 2000, 2000, 8)
+    assert np.abs(got[:3] - a).max() < 1e-10 and abs(got[3]) < 1e-10 and r < 1e-20
+    # held-out continuation
+    assert np.abs(modes_eval(s, got[:3], k[2000:], 2000, 8) - x[2000:]).max() < 1e-10
+
+
+def test_predict_stop_on_exponential_decay():
+    from openems_amd.learned_stop import predict_stop
+    # energy exp(-t / 400) from a peak of 1: -50 dB at t = 400 * 5 ln 10 = 4605.2,
+    # the first check below is 4700, the run then holds 4701 steps
+    ts = np.arange(0, 3000, 100)
+    en = np.exp(-ts / 400.0)
+    assert predict_stop(ts, en, 3000, 20000) == 4701
+    assert predict_stop(ts, en, 3000, 4000) == 4000           # capped
+    assert predict_stop(ts, np.ones_like(en), 3000, 20000) == 20000  # no decay
+    # only checks before pos count
+    assert predict_stop(ts, en, 1000, 20000) == 4701
