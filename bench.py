@@ -576,12 +576,23 @@ def bench_learned(args):
                 "untrained_poles_refit": LS.refit_tail_error(model, Xh[sub], Yh[sub], poles=start_poles),
                 "windows": int(len(Xh[sub]))}
     log(f"[bench] learned: held-out tail error / zero-prediction error {gain}")
-    learned = LS.learned_sweep(model, test_s, T, window, dec, check, stride=args.learned_stride)
+    learned = LS.learned_sweep(model, test_s, T, window, dec, check, stride=args.learned_stride,
+                               log=lambda m: log(f"[bench] learned (network): {m}"))
     ds11 = np.abs(learned.s11 - plain.s11).max(axis=1)
     acc = learned.accepted
     log(f"[bench] learned: checkpoints {learned.checkpoints}; accepted {int(acc.sum())}; "
         f"steps plain {int(plain.steps.sum())} learned {int(learned.steps_learned.sum())}")
     qs = (lambda v: [float(np.median(v)), float(np.percentile(v, 90)), float(v.max())] if len(v) else None)
+    # the same gates and horizon with classical matrix-pencil poles: what an
+    # exact-structure extrapolator reaches against the -50 dB judge
+    pen = LS.learned_sweep(model, test_s, T, window, dec, check, stride=args.learned_stride, poles="pencil",
+                           log=lambda m: log(f"[bench] learned (pencil): {m}"))
+    dp = np.abs(pen.s11 - plain.s11).max(axis=1)
+    pencil = {"wall_ratio": round(pen.wall_learned / plain.wall, 4), "accepted": int(pen.accepted.sum()),
+              "within_1e-3": int((dp[pen.accepted] <= 1e-3).sum()),
+              "dS11_accepted_median_p90_max": qs(dp[pen.accepted]),
+              "steps_learned": int(pen.steps_learned.sum())}
+    log(f"[bench] learned: pencil poles {pencil}")
     out = {
         "metric": "sweep time-to-|dS11|<=1e-3, learned-stop vs plain",
         "value": round(learned.wall_learned / plain.wall, 4),
@@ -613,6 +624,7 @@ def bench_learned(args):
         "continued_equal_plain": bool(np.array_equal(learned.s11[~acc], plain.s11[~acc])),
         "train_loss_first_last": [losses[0], losses[-1]],
         "heldout_tail_error_vs_zero": gain,
+        "pencil_poles_same_gates": pencil,
         "data_s": round(t_data, 1),
         "train_s": round(t_train, 1),
         "roofline": None,

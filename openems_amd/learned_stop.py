@@ -193,6 +193,28 @@ def predict_stop(timesteps: np.ndarray, energies: np.ndarray, pos: int, max_step
     return int(min(c + 1, max_steps))
 
 
+def pencil_poles(u: np.ndarray, i: np.ndarray, k0: int, k1: int, decimate: int, order: int) -> np.ndarray:
+    """Poles per decimated sample of the decimated V and I over [k0, k1)
+    by the matrix pencil method (Hua and Sarkar, IEEE Trans. ASSP 38(5),
+    1990): a Hankel matrix of the samples with pencil parameter N/3, its
+    `order` leading right singular vectors, and the eigenvalues z of the
+    shifted pencil; s = log z, decaying ones with Im s >= 0 kept.  A
+    classical stand-in for the network's poles, for comparison."""
+    out = []
+    for x in (u, i):
+        y = x[k0:k1:decimate].astype(np.float64)
+        y = y / max(np.abs(y).max(), 1e-300)
+        N = len(y)
+        L = max(N // 3, order + 1)
+        Y = np.lib.stride_tricks.sliding_window_view(y, L + 1)
+        _, _, Vh = np.linalg.svd(Y, full_matrices=False)
+        V = Vh[:min(order, Vh.shape[0])].conj().T
+        z = np.linalg.eigvals(np.linalg.pinv(V[:-1]) @ V[1:])
+        out.append(np.log(z.astype(np.complex128)))
+    s = np.concatenate(out)
+    return s[(s.real < 0) & (s.real > -1.0) & (s.imag >= 0)]
+
+
 def _passes(value: float, bar: float) -> bool:
     return bar == np.inf or (np.isfinite(value) and value <= bar)
 
@@ -200,7 +222,8 @@ def _passes(value: float, bar: float) -> bool:
 def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, decimate: int, check: int,
                   max_residual: float = 1e-6, freqs=None, decay_db: float = 50.0, stride: Optional[int] = None,
                   decay_check: int = 100, max_tail_energy: float = 1e-4, horizon_window: int = 300,
-                  max_horizon_change: float = 3e-4) -> LearnedResult:
+                  max_horizon_change: float = 3e-4, poles: str = "network", pencil_order: int = 24,
+                  min_decay_db: float = 20.0, fit_stride: int = 4, log=None) -> LearnedResult:
     """All structures run together from checkpoint to checkpoint (the first
     at window + check steps, then every `stride`).  At a checkpoint P the
     network maps the decimated V and I of [P - check - window, P - check) to
@@ -219,7 +242,13 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
     (residues refitted over [P - window, P)) from P to the predicted stop,
     the plain run's own horizon.  The other structures continue from their
     state at P with the plain run's fields, DFT sums, energy peak and check
-    steps, so one that never stops ends exactly as the plain run does."""
+    steps, so one that never stops ends exactly as the plain run does.
+    poles="pencil" takes the poles from the matrix pencil of the same window
+    instead of the network (pencil_poles), the classical comparison.  Only
+    structures whose energy has already fallen min_decay_db below its peak
+    are evaluated at a checkpoint (a tail of at most max_tail_energy of the
+    signal energy needs most of the decay behind it), and the least-squares
+    fits use every fit_stride-th step."""
     d = model.dims
     early = window + check
     stride = stride or window
@@ -254,7 +283,7 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
         if ci is None:
             ci = sw.tuning()["ci"]
         pos = target
-        still = []
+        still, waiting = [], []
         for k, q in enumerate(live):
             steps[q] += res[k].timesteps_executed
             ts, en = sw.energy_history(k)
@@ -262,10 +291,17 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
             hist_e[q].append(en)
             if res[k].termination_reason != 0 or pos >= max_steps:
                 s11[q] = built[q][3].s11(50.0)
-            else:
+                continue
+            e_all = np.concatenate(hist_e[q])
+            if len(e_all) and e_all.max() > 0 and e_all[-1] <= e_all.max() * 10.0 ** (-min_decay_db / 10.0):
                 still.append(q)
+            else:
+                waiting.append(q)
         if not still:
-            break
+            live = waiting
+            if live:
+                checkpoints.append(pos)
+            continue
         checkpoints.append(pos)
         t1 = time.perf_counter()
         X = np.zeros((len(still), 2, d.L0), np.float32)
@@ -276,18 +312,29 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
             X[k], _, _ = _window(ser[:, 0], ser[:, 1], pos - early, window, decimate, 0)
             sers.append(ser)
             dfts.append(port.dft_values())
-        modes = np.zeros((len(still), 6 * d.modes), np.float32)
-        for b0 in range(0, len(still), d.batch):
-            m, _ = model.forward(X[b0:b0 + d.batch], tails=False)
-            modes[b0:b0 + d.batch] = m
-        s_all, _, _ = mode_params(modes, d.modes)
+        if poles == "network":
+            modes = np.zeros((len(still), 6 * d.modes), np.float32)
+            for b0 in range(0, len(still), d.batch):
+                m, _ = model.forward(X[b0:b0 + d.batch], tails=False)
+                modes[b0:b0 + d.batch] = m
+            s_all, _, _ = mode_params(modes, d.modes)
         rejected = []
         a0, a1 = pos - early, pos - check
         for k, q in enumerate(still):
-            s = s_all[k]
+            if poles == "network":
+                s = s_all[k]
+            else:
+                try:
+                    s = pencil_poles(sers[k][:, 0], sers[k][:, 1], a0, a1, decimate, pencil_order)
+                except np.linalg.LinAlgError:
+                    rejected.append(q)
+                    continue
+                if len(s) == 0:
+                    rejected.append(q)
+                    continue
             u, i = sers[k][:, 0], sers[k][:, 1]
-            aU, _ = refit_residues(u, s, a0, a1, a1, decimate)
-            aI, _ = refit_residues(i, s, a0, a1, a1, decimate)
+            aU, _ = refit_residues(u, s, a0, a1, a1, decimate, fit_stride)
+            aI, _ = refit_residues(i, s, a0, a1, a1, decimate, fit_stride)
             kk = np.arange(a1, pos)
             eu = float(np.sum(u[a0:a1] ** 2))
             ei = float(np.sum(i[a0:a1] ** 2))
@@ -296,8 +343,8 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
             resid[q] = max(ru, ri)
             ok = _passes(resid[q], max_residual)
             if ok:
-                aU, _ = refit_residues(u, s, pos - window, pos, pos, decimate)
-                aI, _ = refit_residues(i, s, pos - window, pos, pos, decimate)
+                aU, _ = refit_residues(u, s, pos - window, pos, pos, decimate, fit_stride)
+                aI, _ = refit_residues(i, s, pos - window, pos, pos, decimate, fit_stride)
                 stop = predict_stop(np.concatenate(hist_t[q]), np.concatenate(hist_e[q]), pos, max_steps, decay_db,
                                     decay_check)
                 sv = max(float(np.abs(u[a0:pos]).max()), 1e-300)
@@ -325,7 +372,9 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
             if not ok:
                 rejected.append(q)
         t_inf += time.perf_counter() - t1
-        live = rejected
+        live = rejected + waiting
+        if log:
+            log(f"checkpoint {pos}: {int(accepted.sum())} accepted, {len(live)} continue")
     return LearnedResult(s11, accepted, resid, wall + t_inf, t_inf, steps, checkpoints, tail_frac)
 
 

@@ -49,7 +49,7 @@ def test_rejected_structures_continue_to_the_plain_result():
 def test_open_gates_stop_at_the_first_checkpoint():
     structs = _structs()
     res = LS.learned_sweep(_model(), structs, MAX_STEPS, WINDOW, DEC, CHECK, max_residual=np.inf,
-                           max_tail_energy=np.inf, max_horizon_change=np.inf, stride=448)
+                           max_tail_energy=np.inf, max_horizon_change=np.inf, min_decay_db=0.0, stride=448)
     say(f"[learned] open gates: steps {res.steps_learned.tolist()}, checkpoints {res.checkpoints}")
     early = WINDOW + CHECK
     # a structure stops at the first checkpoint unless it decayed before it

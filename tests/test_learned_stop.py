@@ -84,3 +84,14 @@ def test_predict_stop_on_exponential_decay():
     assert predict_stop(ts, np.ones_like(en), 3000, 20000) == 20000  # no decay
     # only checks before pos count
     assert predict_stop(ts, en, 1000, 20000) == 4701
+
+
+def test_pencil_poles_recover_damped_modes():
+    from openems_amd.learned_stop import modes_eval, pencil_poles
+    s = np.array([-0.02 + 2j * math.pi * 0.04, -0.005 + 2j * math.pi * 0.13])
+    k = np.arange(4096)
+    u = modes_eval(s, np.array([1.0 + 0.5j, -0.3 + 0.2j]), k, 0, 8)
+    i = modes_eval(s, np.array([0.2 - 0.1j, 0.7 + 0.0j]), k, 0, 8)
+    got = pencil_poles(u, i, 0, 4096, 8, 4)
+    for sm in s:
+        assert np.min(np.abs(got - sm)) < 1e-8
