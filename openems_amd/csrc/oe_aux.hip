@@ -325,29 +325,18 @@ __global__ __launch_bounds__(256) void k_dft(const float* __restrict__ F, size_t
 
 // One thread per structure applies that structure's excitations in list order
 // (the order of k_point_ops), with the sample of the current step.
-__global__ void k_sweep_exc(const SweepExc* __restrict__ e, const int* __restrict__ first, int n_struct,
-                            const uint64_t* __restrict__ step) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n_struct) return;
-    const uint64_t st = *step;
-    for (int q = first[s]; q < first[s + 1]; ++q) {
-        const float v = st < e[q].n_samples ? e[q].samples[st] : 0.f;
-        if (e[q].soft) *e[q].target += v;
-        else *e[q].target = v;
-    }
-}
-
-// One work-group per DFT extension: the signal values go to LDS, the phasors
-// of a chunk of frequencies too, then the threads update the (frequency,
-// signal) accumulators.  Same arithmetic as k_dft.
-__global__ __launch_bounds__(256) void k_sweep_dft(const SweepDft* __restrict__ list, const uint64_t* __restrict__ step,
-                                                   double dt) {
+// The sweep's per-step point work, one work-group per structure: its
+// excitations (BasicEngine::apply_excitations, src/fdtd/basic_engine.rs:
+// 313-395: one thread per source when the structure's source cells are
+// distinct, else one thread in list order), then -- after the barrier, so the
+// sources are in the fields it samples -- its port DFT (same arithmetic as
+// k_dft: signal values to LDS, the phasors of a chunk of frequencies too,
+// then the (frequency, signal) accumulators).
+__device__ void sweep_dft_body(const SweepDft& x, uint64_t st, double dt) {
     __shared__ double sv[256];
     __shared__ uint8_t hv[256];
     __shared__ double ph[2][DFT_FCHUNK][2];
-    const SweepDft& x = list[blockIdx.x];
     const DftDev& d = x.d;
-    const uint64_t st = *step;
     const uint64_t sample = x.sample0 + st;
     const double cur = x.t0 + (double)st;
     const double tE = (cur + 1.0) * dt, tH = (cur + 0.5) * dt;
@@ -392,6 +381,33 @@ __global__ __launch_bounds__(256) void k_sweep_dft(const SweepDft* __restrict__ 
             }
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_sweep_point(const SweepExc* __restrict__ e, const int* __restrict__ first,
+                                                     const int* __restrict__ serial,
+                                                     const SweepDft* __restrict__ dfts,
+                                                     const int* __restrict__ dft_of, const uint64_t* __restrict__ step,
+                                                     double dt) {
+    const int s = blockIdx.x;
+    const uint64_t st = *step;
+    const int q0 = first[s], q1 = first[s + 1];
+    if (serial[s]) {
+        if (threadIdx.x == 0)
+            for (int q = q0; q < q1; ++q) {
+                const float v = st < e[q].n_samples ? e[q].samples[st] : 0.f;
+                if (e[q].soft) *e[q].target += v;
+                else *e[q].target = v;
+            }
+    } else {
+        for (int q = q0 + (int)threadIdx.x; q < q1; q += blockDim.x) {
+            const float v = st < e[q].n_samples ? e[q].samples[st] : 0.f;
+            if (e[q].soft) *e[q].target += v;
+            else *e[q].target = v;
+        }
+    }
+    if (dft_of[s] < 0) return;  // uniform over the work-group
+    __syncthreads();
+    sweep_dft_body(dfts[dft_of[s]], st, dt);
 }
 
 __global__ void k_sweep_step(uint64_t* step) {
@@ -542,15 +558,10 @@ hipError_t launch_mur_post(float* F, const Grid& g, const MurDev* faces, int n, 
     return hipGetLastError();
 }
 
-hipError_t launch_sweep_exc(const SweepExc* e, const int* first, int n_struct, const uint64_t* step, hipStream_t s) {
+hipError_t launch_sweep_point(const SweepExc* e, const int* first, const int* serial, const SweepDft* dfts,
+                              const int* dft_of, int n_struct, const uint64_t* step, double dt, hipStream_t s) {
     if (n_struct <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sweep_exc, dim3((n_struct + 63) / 64), dim3(64), 0, s, e, first, n_struct, step);
-    return hipGetLastError();
-}
-
-hipError_t launch_sweep_dft(const SweepDft* list, int n, const uint64_t* step, double dt, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sweep_dft, dim3(n), dim3(256), 0, s, list, step, dt);
+    hipLaunchKernelGGL(k_sweep_point, dim3(n_struct), dim3(256), 0, s, e, first, serial, dfts, dft_of, step, dt);
     return hipGetLastError();
 }
 

@@ -338,8 +338,11 @@ struct SweepDft {
     uint64_t sample0;        // sample index of sweep step 0
     double t0;               // timestep of sweep step 0
 };
-hipError_t launch_sweep_exc(const SweepExc* e, const int* first, int n_struct, const uint64_t* step, hipStream_t s);
-hipError_t launch_sweep_dft(const SweepDft* list, int n, const uint64_t* step, double dt, hipStream_t s);
+// Excitations and port DFT of every active structure after its E update: one
+// work-group per structure (first: excitation ranges, serial: 1 where the
+// structure's source cells repeat, dft_of: index into dfts or -1).
+hipError_t launch_sweep_point(const SweepExc* e, const int* first, const int* serial, const SweepDft* dfts,
+                              const int* dft_of, int n_struct, const uint64_t* step, double dt, hipStream_t s);
 hipError_t launch_sweep_step(uint64_t* step, hipStream_t s);
 // Energies [n][2] of structures st[ids[0..n)] (partials: n * 2 * energy_blocks() doubles).
 hipError_t launch_sweep_energy(const SweepStruct* st, const int* ids, int n, double* partials, double* out,

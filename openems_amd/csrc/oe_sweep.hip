@@ -61,7 +61,7 @@ struct oe_sweep {
     double tune_ms[4] = {0, 0, 0, 0};
     int tune_ci[4] = {1, 2, 4, 8};
     // device state of a run
-    DevBuf structs, tables, excs, exc_first, exc_samples, dfts, ids, partials, energy, step;
+    DevBuf structs, tables, excs, exc_first, exc_serial, exc_samples, dfts, dft_of, ids, partials, energy, step;
     double* energy_host = nullptr;
     size_t energy_host_n = 0;
     std::vector<Group> groups;
@@ -208,11 +208,18 @@ struct StepTables {
 oe_status build_step_tables(oe_sweep* sw, const std::vector<int>& active, StepTables& t) {
     std::vector<SweepExc> ex;
     std::vector<int> first{0};
+    std::vector<int> serial, dft_of;
     std::vector<SweepDft> dft;
     for (int sid : active) {
         const Item& it = sw->items[sid];
         for (const auto& e : it.exc) ex.push_back(e);
         first.push_back((int)ex.size());
+        // sources on one cell apply in list order: then one thread does them all
+        std::vector<const float*> cells;
+        for (const auto& e : it.exc) cells.push_back(e.target);
+        std::sort(cells.begin(), cells.end());
+        serial.push_back(std::adjacent_find(cells.begin(), cells.end()) != cells.end() ? 1 : 0);
+        dft_of.push_back(it.port ? (int)dft.size() : -1);
         if (it.port) {
             SweepDft d{};
             d.F = it.eng->F;
@@ -226,6 +233,15 @@ oe_status build_step_tables(oe_sweep* sw, const std::vector<int>& active, StepTa
     if (sw->excs.bytes < std::max<size_t>(ex.size(), 1) * sizeof(SweepExc))
         HIP_OK(sw->excs.alloc(std::max<size_t>(ex.size(), 1) * sizeof(SweepExc)));
     if (sw->exc_first.bytes < first.size() * sizeof(int)) HIP_OK(sw->exc_first.alloc(first.size() * sizeof(int)));
+    const size_t nb_struct = std::max<size_t>(active.size(), 1) * sizeof(int);
+    if (sw->exc_serial.bytes < nb_struct) HIP_OK(sw->exc_serial.alloc(nb_struct));
+    if (sw->dft_of.bytes < nb_struct) HIP_OK(sw->dft_of.alloc(nb_struct));
+    if (!active.empty()) {
+        HIP_OK(hipMemcpyAsync(sw->exc_serial.p, serial.data(), serial.size() * sizeof(int), hipMemcpyHostToDevice,
+                              sw->stream));
+        HIP_OK(hipMemcpyAsync(sw->dft_of.p, dft_of.data(), dft_of.size() * sizeof(int), hipMemcpyHostToDevice,
+                              sw->stream));
+    }
     if (sw->dfts.bytes < std::max<size_t>(dft.size(), 1) * sizeof(SweepDft))
         HIP_OK(sw->dfts.alloc(std::max<size_t>(dft.size(), 1) * sizeof(SweepDft)));
     if (!ex.empty()) HIP_OK(hipMemcpyAsync(sw->excs.p, ex.data(), ex.size() * sizeof(SweepExc), hipMemcpyHostToDevice, sw->stream));
@@ -247,10 +263,9 @@ hipError_t sweep_step(oe_sweep* sw, const StepTables& t, double dt) {
     // the first H launch advances the step counter (from -1: step st reads st)
     if ((e = yee_half(sw, true, sw->step.as<uint64_t>())) != hipSuccess) return e;
     if ((e = yee_half(sw, false)) != hipSuccess) return e;
-    if ((e = launch_sweep_exc(sw->excs.as<SweepExc>(), sw->exc_first.as<int>(), t.n_struct, sw->step.as<uint64_t>(),
-                              sw->stream)) != hipSuccess)
-        return e;
-    return launch_sweep_dft(sw->dfts.as<SweepDft>(), t.n_dft, sw->step.as<uint64_t>(), dt, sw->stream);
+    return launch_sweep_point(sw->excs.as<SweepExc>(), sw->exc_first.as<int>(), sw->exc_serial.as<int>(),
+                              sw->dfts.as<SweepDft>(), sw->dft_of.as<int>(), t.n_struct, sw->step.as<uint64_t>(), dt,
+                              sw->stream);
 }
 
 oe_status capture(oe_sweep* sw, const StepTables& t, double dt, int steps) {

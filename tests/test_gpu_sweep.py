@@ -153,3 +153,33 @@ def test_sweep_rejects_mixed_timesteps():
         sw2.add(eng, cp, port, exc)
     with pytest.raises(Exception, match="one timestep"):
         sw2.run(5, check_interval=CHECK, autotune=False)
+
+
+def test_sweep_sources_on_repeated_cells_keep_list_order():
+    """Sources of one structure that share a cell (a soft and then a hard
+    source on one port edge) are applied in list order by the sweep's point
+    kernel, as a single engine applies them; the other structure's distinct
+    cells take the parallel path."""
+    from openems_amd import ScheduledExcitation
+    structs = _structures()[:2]
+    steps = 300
+
+    def sources(exc):
+        e0, e1 = exc[0], exc[1]
+        return exc + [ScheduledExcitation(e0.position, e0.direction, 0.5 * e0.samples, True),
+                      ScheduledExcitation(e1.position, e1.direction, 0.25 * e1.samples, False)]
+
+    sw = Sweep()
+    keep = []
+    for q, ms in enumerate(structs):
+        op, eng, cp, port, exc = W.build_microstrip_hip(ms, steps, record_steps=steps)
+        sw.add(eng, cp, port, sources(exc) if q == 0 else exc)
+        keep.append((eng, port))
+    sw.run(steps, check_interval=CHECK, autotune=False)
+    for q, ms in enumerate(structs):
+        op, eng1, cp1, port1, exc1 = W.build_microstrip_hip(ms, steps, record_steps=steps)
+        eng1.run_batch(EngineBatch(num_steps=steps, excitations=sources(exc1) if q == 0 else exc1,
+                                   extensions=[cp1, port1]))
+        eng2, port2 = keep[q]
+        assert np.array_equal(np.concatenate(eng1.read_fields()), np.concatenate(eng2.read_fields()))
+        assert np.array_equal(port1.series(), port2.series())
