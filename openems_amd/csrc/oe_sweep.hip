@@ -496,11 +496,18 @@ oe_status oe_sweep_run(oe_sweep* sw, const oe_sweep_config* cfg, oe_sweep_entry*
         const uint64_t next_check = (ts + check - 1) / check * check;  // first check at or after it
         const uint64_t chunk = std::min<uint64_t>(next_check - ts + 1, cfg->max_steps - done);
         if (cfg->use_graph && chunk == check && done > 0) {
-            if (!sw->exec || dirty || sw->graph_steps != (int)chunk) {
-                OE_TRY(capture(sw, tab, dt, (int)chunk));
+            // the graph holds a divisor of the check interval (at most 25
+            // steps) and is launched check / gsteps times: the active set
+            // changes at many checks, and instantiating a short graph costs
+            // a fraction of a long one
+            int gsteps = 1;
+            for (int q = 1; q <= 25; ++q)
+                if (check % q == 0) gsteps = q;
+            if (!sw->exec || dirty || sw->graph_steps != gsteps) {
+                OE_TRY(capture(sw, tab, dt, gsteps));
                 dirty = false;
             }
-            HIP_OK(hipGraphLaunch(sw->exec, sw->stream));
+            for (uint64_t q = 0; q < chunk / gsteps; ++q) HIP_OK(hipGraphLaunch(sw->exec, sw->stream));
         } else {
             for (uint64_t q = 0; q < chunk; ++q) HIP_OK(sweep_step(sw, tab, dt));
         }
