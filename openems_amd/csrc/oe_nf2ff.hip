@@ -544,6 +544,118 @@ __global__ __launch_bounds__(REDUCE_THREADS) void k_nf2ff_reduce(ReduceArgs r) {
     }
 }
 
+// Second contraction over mirror groups: the angles of one G row (one |kappa_p|)
+// whose |kappa_q| also agree, e.g. (theta, pi - theta) x (phi, -phi, pi - phi,
+// pi + phi) on a regular grid.  With sigma = sign kappa_p and s = sign kappa_q,
+// a member's sum
+//   sum_q w_q e^{j s k |kappa_q| x_q} (GC_q + j sigma GS_q)
+//     = P1 + j sigma P2 + j s P3 - s sigma P4,
+//   P1 = sum w cos GC, P2 = sum w cos GS, P3 = sum w sin GC, P4 = sum w sin GS,
+// needs only the group's four partial sums (phase k |kappa_q| x_q of the
+// group's first member; members agree to 1e-12 in |kappa_q|).  Each G row is
+// read once per group, and the float64 work per q is that of one angle for
+// the whole group.
+//
+// One wave per group: lane = 4 ql + c takes component c at q = ql, ql + MQ,
+// ... (the four lanes of one ql read 16 consecutive bytes, the wave 256), with
+// eight float64 accumulators, and issues the loads of four q steps before
+// using them (the reduction is bound by the latency of these loads).  An xor
+// butterfly over the MQ lanes of each component gives every lane its
+// component's sums; lane (ql, c) then finishes component c of member ql (a
+// group holds at most MQ members).
+struct MirrorArgs {
+    const int4* groups;   // (u, first index into members, count, 0)
+    const double* gkq;    // per group: |kappa_q| of its first member
+    const int* members;   // angles, grouped
+    int m0, m1;           // this chunk's range of groups[]
+};
+constexpr int MQ = 16;
+constexpr int MIRROR_THREADS = 256;
+constexpr int MIRROR_GROUPS = MIRROR_THREADS / (4 * MQ);
+
+// UNI: uniform q grid (phases by recurrence, r.uniform set)
+template <bool UNI>
+__global__ __launch_bounds__(MIRROR_THREADS) void k_nf2ff_reduce_mirror(ReduceArgs r, MirrorArgs m) {
+    const int gi = m.m0 + blockIdx.x * MIRROR_GROUPS + (int)(threadIdx.x / (4 * MQ));
+    if (gi >= m.m1) return;  // whole waves: no lane of another group waits on this one
+    const int c = threadIdx.x & 3;
+    const int ql = (threadIdx.x >> 2) & (MQ - 1);
+    const int4 grp = m.groups[gi];
+    const double kk = r.k * m.gkq[gi];
+    double pr = 1.0, pi = 0.0, str = 1.0, sti = 0.0;
+    if (UNI && ql < r.nq) {
+        sincos(kk * r.xq[ql], &pi, &pr);
+        sincos(kk * (MQ * r.dxq), &sti, &str);
+    }
+    // P1..P4 of component c, real and imaginary
+    double p1r = 0.0, p1i = 0.0, p2r = 0.0, p2i = 0.0, p3r = 0.0, p3i = 0.0, p4r = 0.0, p4i = 0.0;
+    const float* gc = r.G + (size_t)(2 * (grp.x - r.u0)) * r.ldg + c;
+    const float* gs = gc + r.ldg;
+    const int n4 = 4 * r.nq;
+    auto step = [&](int q, float a, float b, float e, float f, double w) {
+        double cs = pr, sn = pi;
+        if (UNI) {
+            const double t = pr * str - pi * sti;
+            pi = pr * sti + pi * str;
+            pr = t;
+        } else {
+            sincos(kk * r.xq[q], &sn, &cs);
+        }
+        const double wc = w * cs, ws = w * sn;
+        p1r = fma(wc, (double)a, p1r);
+        p1i = fma(wc, (double)b, p1i);
+        p2r = fma(wc, (double)e, p2r);
+        p2i = fma(wc, (double)f, p2i);
+        p3r = fma(ws, (double)a, p3r);
+        p3i = fma(ws, (double)b, p3i);
+        p4r = fma(ws, (double)e, p4r);
+        p4i = fma(ws, (double)f, p4i);
+    };
+    int iq = ql;
+    for (; iq + 3 * MQ < r.nq; iq += 4 * MQ) {
+        float va[4], vb[4], ve[4], vf[4];
+        double vw[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = iq + u * MQ;
+            va[u] = gc[4 * q];
+            vb[u] = gc[n4 + 4 * q];
+            ve[u] = gs[4 * q];
+            vf[u] = gs[n4 + 4 * q];
+            vw[u] = r.wq[q];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) step(iq + u * MQ, va[u], vb[u], ve[u], vf[u], vw[u]);
+    }
+    for (; iq < r.nq; iq += MQ) step(iq, gc[4 * iq], gc[n4 + 4 * iq], gs[4 * iq], gs[n4 + 4 * iq], r.wq[iq]);
+#pragma unroll
+    for (int o = 4; o < 4 * MQ; o <<= 1) {
+        p1r += __shfl_xor(p1r, o);
+        p1i += __shfl_xor(p1i, o);
+        p2r += __shfl_xor(p2r, o);
+        p2i += __shfl_xor(p2i, o);
+        p3r += __shfl_xor(p3r, o);
+        p3i += __shfl_xor(p3i, o);
+        p4r += __shfl_xor(p4r, o);
+        p4i += __shfl_xor(p4i, o);
+    }
+    const int slot = c == 0 ? r.slot[0] : c == 1 ? r.slot[1] : c == 2 ? r.slot[2] : r.slot[3];
+    if (ql < grp.z) {
+        const int a = m.members[grp.y + ql];
+        const double sg = (double)r.usgn[a];
+        const double s = r.kapq[a] < 0.0 ? -1.0 : 1.0;
+        const double ss = s * sg;
+        double sn, cs;
+        sincos(r.k * r.kap0[a] * r.x0, &sn, &cs);
+        // P1 + j sigma P2 + j s P3 - s sigma P4
+        const double tr = p1r - sg * p2i - s * p3i - ss * p4r;
+        const double ti = p1i + sg * p2r + s * p3r - ss * p4i;
+        double* o = r.NL + ((size_t)a * 6 + slot) * 2;
+        o[0] += cs * tr - sn * ti;
+        o[1] += cs * ti + sn * tr;
+    }
+}
+
 // E_theta = -fac (L_phi + Z0 N_theta), E_phi = fac (L_theta - Z0 N_phi) with
 // fac = jk e^{-jkr}/(4 pi r) (matlab/AnalyzeNF2FF.m:98-105) and
 // U = r^2/(2 Z0)(|E_theta|^2 + |E_phi|^2) (:112): out = [4][n_ang]
@@ -655,7 +767,17 @@ struct Nf2ffCache {
         DevBuf uidx;             // n_ang ints
         DevBuf usgn;             // n_ang floats
         DevBuf order;            // n_ang ints, angles sorted by u
+        std::vector<int> horder; // host copy of order
     } ax[3];
+    // Mirror groups of the faces normal to each axis (k_nf2ff_reduce_mirror):
+    // the angles of every row u of that face's first-contraction axis,
+    // grouped by equal |kappa| along its second axis, at most MQ per group.
+    struct Mirror {
+        std::vector<int> mstart;  // U + 1 offsets into groups
+        DevBuf groups;            // int4 (u, first index into members, count, 0)
+        DevBuf gkq;               // double per group
+        DevBuf members;           // n_ang ints
+    } mir[3];
     DevBuf work;
     // timing events, created on first use and kept: [0] brackets the whole
     // transform, the others one GEMM launch each
@@ -666,6 +788,15 @@ struct Nf2ffCache {
 
 Nf2ffCache* nf2ff_cache_new() { return new Nf2ffCache(); }
 void nf2ff_cache_free(Nf2ffCache* c) { delete c; }
+
+// The faces normal to axis f contract first over the in-plane axis p with
+// fewer distinct |kappa| (fewer GEMM rows), then over the other, q.
+static void face_axes(const Nf2ffCache& c, int f, int* p, int* q) {
+    const int a1 = (f + 1) % 3, a2 = (f + 2) % 3;
+    const bool swap = c.ax[a2].U < c.ax[a1].U;
+    *p = swap ? a2 : a1;
+    *q = swap ? a1 : a2;
+}
 
 static hipError_t plan_angles(Nf2ffCache& c, const double* theta, int nt, const double* phi, int np, hipStream_t s) {
     if (c.n_ang && c.theta.size() == (size_t)nt && c.phi.size() == (size_t)np &&
@@ -730,7 +861,45 @@ static hipError_t plan_angles(Nf2ffCache& c, const double* theta, int nt, const 
         NF_OK(hipMemcpyAsync(A.uidx.p, uidx.data(), n * 4, hipMemcpyHostToDevice, s));
         NF_OK(hipMemcpyAsync(A.usgn.p, sgn.data(), n * 4, hipMemcpyHostToDevice, s));
         NF_OK(hipMemcpyAsync(A.order.p, order.data(), n * 4, hipMemcpyHostToDevice, s));
+        A.horder = order;
         NF_OK(hipStreamSynchronize(s));  // host vectors are reused for the next axis
+    }
+    for (int f = 0; f < 3; ++f) {
+        int p, q;
+        face_axes(c, f, &p, &q);
+        const auto& A = c.ax[p];
+        const double* kq = kap.data() + (size_t)q * n;
+        std::vector<int> members(n);
+        std::vector<int4> groups;
+        std::vector<double> gkq;
+        auto& M = c.mir[f];
+        M.mstart.assign(1, 0);
+        for (int u = 0; u < A.U; ++u) {
+            const int b = A.start[u], e = A.start[u + 1];
+            for (int i = b; i < e; ++i) members[i] = A.horder[i];
+            std::sort(members.begin() + b, members.begin() + e, [&](int x, int y) {
+                const double a1 = std::fabs(kq[x]), a2 = std::fabs(kq[y]);
+                return a1 < a2 || (a1 == a2 && x < y);
+            });
+            for (int i = b; i < e;) {
+                const double v0 = std::fabs(kq[members[i]]);
+                int j = i + 1;
+                // at most MQ members: one finishing pass per lane (the poles
+                // alone would otherwise make one group of every phi)
+                while (j < e && j - i < MQ && std::fabs(kq[members[j]]) - v0 <= 1e-12) ++j;
+                groups.push_back(make_int4(u, i, j - i, 0));
+                gkq.push_back(v0);
+                i = j;
+            }
+            M.mstart.push_back((int)groups.size());
+        }
+        NF_OK(M.groups.alloc(std::max<size_t>(groups.size(), 1) * sizeof(int4)));
+        NF_OK(M.gkq.alloc(std::max<size_t>(gkq.size(), 1) * sizeof(double)));
+        NF_OK(M.members.alloc(n * 4));
+        NF_OK(hipMemcpyAsync(M.groups.p, groups.data(), groups.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+        NF_OK(hipMemcpyAsync(M.gkq.p, gkq.data(), gkq.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        NF_OK(hipMemcpyAsync(M.members.p, members.data(), n * 4, hipMemcpyHostToDevice, s));
+        NF_OK(hipStreamSynchronize(s));
     }
     c.theta.assign(theta, theta + nt);
     c.phi.assign(phi, phi + np);
@@ -831,6 +1000,10 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
     // halves, k_hgemm3) unless OE_NF2FF_SPLIT16=0
     bool split16 = true;
     if (const char* v = getenv("OE_NF2FF_SPLIT16")) split16 = atoi(v) != 0;
+    // the second contraction by mirror groups where they hold two or more
+    // angles on average, unless OE_NF2FF_MIRROR=0
+    bool mirror = true;
+    if (const char* v = getenv("OE_NF2FF_MIRROR")) mirror = atoi(v) != 0;
 
     std::vector<double> coords;
     coords.reserve(coord_len);
@@ -960,9 +1133,22 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
             ra.g0 = A.gstart[u0];
             ra.g1 = A.gstart[u1];
             ra.u0 = u0;
-            const int ng = ra.g1 - ra.g0;
-            hipLaunchKernelGGL(k_nf2ff_reduce, dim3((ng + REDUCE_GROUPS - 1) / REDUCE_GROUPS), dim3(REDUCE_THREADS), 0,
-                               s, ra);
+            const auto& M = C.mir[f.axis];
+            const int nm = M.mstart[u1] - M.mstart[u0];
+            if (mirror && 2 * (size_t)nm <= (size_t)(A.start[u1] - A.start[u0])) {
+                // two or more angles per mirror group on average
+                MirrorArgs ma{M.groups.as<int4>(), M.gkq.as<double>(), M.members.as<int>(), M.mstart[u0],
+                              M.mstart[u1]};
+                const dim3 grid((nm + MIRROR_GROUPS - 1) / MIRROR_GROUPS);
+                if (ra.uniform)
+                    hipLaunchKernelGGL(k_nf2ff_reduce_mirror<true>, grid, dim3(MIRROR_THREADS), 0, s, ra, ma);
+                else
+                    hipLaunchKernelGGL(k_nf2ff_reduce_mirror<false>, grid, dim3(MIRROR_THREADS), 0, s, ra, ma);
+            } else {
+                const int ng = ra.g1 - ra.g0;
+                hipLaunchKernelGGL(k_nf2ff_reduce, dim3((ng + REDUCE_GROUPS - 1) / REDUCE_GROUPS),
+                                   dim3(REDUCE_THREADS), 0, s, ra);
+            }
             NF_OK(hipGetLastError());
         }
     }

@@ -123,6 +123,39 @@ def test_angle_chunks_and_wide_faces():
     assert np.array_equal(g3.e_theta, g.e_theta) and np.array_equal(g3.e_phi, g.e_phi)
 
 
+def test_mirror_groups_against_per_angle_reduction(monkeypatch):
+    """181 x 360 regular angles with the poles: the second contraction runs
+    over mirror groups (theta, pi - theta) x (phi, -phi, pi - phi, pi + phi)
+    sharing |kappa_p| and |kappa_q|, and at the poles (720 angles with
+    kappa_q = 0) over groups of 16.  It must agree with the per-angle reduction
+    (OE_NF2FF_MIRROR=0, which rounds GC -/+ j GS to float32 before the
+    float64 sum) to 1e-6 and with the float64 oracle to the pattern
+    tolerance."""
+    f = 9e9
+    lam = N.C0 / f
+    faces = N.box_faces([-0.6 * lam, -0.5 * lam, -0.7 * lam], [0.6 * lam, 0.5 * lam, 0.7 * lam], (33, 29, 41),
+                        lambda X_, Y, Z: N.hertzian_dipole_fields(1e-3, f, X_ + 0.001, Y - 0.002, Z))
+    th = np.linspace(0, math.pi, 181)
+    ph = np.linspace(0, 2 * math.pi, 361)[:-1]
+    out = {}
+    for mirror in ("1", "0"):
+        monkeypatch.setenv("OE_NF2FF_MIRROR", mirror)
+        out[mirror] = _gpu_far_field(faces, f, th, ph)
+    g, g0 = out["1"], out["0"]
+    d = math.sqrt((np.linalg.norm(g.e_theta - g0.e_theta) ** 2 + np.linalg.norm(g.e_phi - g0.e_phi) ** 2)
+                  / (np.linalg.norm(g0.e_theta) ** 2 + np.linalg.norm(g0.e_phi) ** 2))
+    sub = slice(0, 181, 15)
+    Et, Ep, prad, _ = N.far_field(faces, f, th[sub], ph)
+    err = math.sqrt((np.linalg.norm(g.e_theta[sub] - Et) ** 2 + np.linalg.norm(g.e_phi[sub] - Ep) ** 2)
+                    / (np.linalg.norm(Et) ** 2 + np.linalg.norm(Ep) ** 2))
+    say(f"[nf2ff] mirror groups, 181x360 angles, 33x29x41 box: rel L2 {d:.2e} vs per-angle, {err:.2e} vs oracle "
+        f"(every 15th theta); device {g.total_ms:.3f} ms vs {g0.total_ms:.3f} ms")
+    assert d <= 1e-6
+    assert err <= PATTERN_TOL
+    assert abs(g.prad - prad) <= 1e-12 * abs(prad)
+    assert g.prad == g0.prad
+
+
 def _fdtd_dipole(precision, n=48, steps=900, stride=3):
     d = 1e-3
     op = Operator.uniform(n, n, n, d)
