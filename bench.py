@@ -163,6 +163,8 @@ def summarize_rocprof(paths):
                 for r in rows[:12]:
                     out.append(f"| `{r[0][:90]}` | {r[1]} | {r[2]:.1f} | {r[3]:.2f} | {r[4]:.2f} |")
                 out.append("")
+        if "kernels" in views:
+            out.append(busy_summary(con.execute("select start, end from kernels order by start").fetchall()))
         if "counters_collection" in views:
             rows = con.execute("select kernel_name, counter_name, count(*), avg(value), min(value), max(value) "
                                "from counters_collection group by kernel_name, counter_name "
@@ -176,6 +178,33 @@ def summarize_rocprof(paths):
             if {"SQ_VALU_MFMA_BUSY_CYCLES", "GRBM_GUI_ACTIVE"} <= names:
                 out.append(mfma_summary(con))
     return "\n".join(out)
+
+
+def busy_summary(intervals, host_gap_ns: float = 1e6):
+    """Device busy time of a kernel trace: the union of the dispatches'
+    [start, end) intervals (ns) against the span from the first start to the
+    last end.  Gaps longer than host_gap_ns (host phases) are reported
+    apart from the short gaps between launches."""
+    if not intervals:
+        return ""
+    busy = 0
+    gaps = []
+    cur_s, cur_e = intervals[0]
+    for s, e in intervals[1:]:
+        if s > cur_e:
+            busy += cur_e - cur_s
+            gaps.append(s - cur_e)
+            cur_s, cur_e = s, e
+        else:
+            cur_e = max(cur_e, e)
+    busy += cur_e - cur_s
+    span = max(e for _, e in intervals) - intervals[0][0]
+    big = sum(g for g in gaps if g > host_gap_ns)
+    small = [g for g in gaps if g <= host_gap_ns]
+    return (f"Dispatches {len(intervals)}, span {span / 1e9:.3f} s, busy {busy / 1e9:.3f} s "
+            f"({busy / span:.3f}); gaps over {host_gap_ns / 1e6:g} ms {big / 1e9:.3f} s; "
+            f"busy / (span - those gaps) {busy / (span - big):.3f}; "
+            f"{len(small)} shorter gaps, {sum(small) / 1e9:.3f} s.\n")
 
 
 def mfma_summary(con, xcds: int = 8, simds: int = 1024):

@@ -47,6 +47,12 @@ struct Group {      // one launch: all boxes of one (vec, axes) kind
 
 inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
+#define HIP_RET(expr)                          \
+    do {                                       \
+        const hipError_t e_ = (expr);          \
+        if (e_ != hipSuccess) return e_;       \
+    } while (0)
+
 }  // namespace
 
 struct oe_sweep {
@@ -70,6 +76,13 @@ struct oe_sweep {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_steps = 0;
+    // The CPML slab launches of a half-step run beside the interior launch,
+    // each on a side stream of its own (forked and joined by events; graph
+    // branches under capture).  OE_SWEEP_OVERLAP=1: all slab launches on one
+    // side stream; 0: every launch on the sweep stream.
+    int overlap = 2;
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
 
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -79,12 +92,22 @@ struct oe_sweep {
         graph_steps = 0;
     }
     ~oe_sweep() {
+        for (hipStream_t q : side)
+            if (q) {
+                (void)hipSetDevice(device);
+                (void)hipStreamSynchronize(q);
+            }
         if (stream) {
             (void)hipSetDevice(device);
             (void)hipStreamSynchronize(stream);
         }
         drop_graph();
         if (energy_host) (void)hipHostFree(energy_host);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        for (hipEvent_t q : ev_join)
+            if (q) (void)hipEventDestroy(q);
+        for (hipStream_t q : side)
+            if (q) (void)hipStreamDestroy(q);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -182,9 +205,17 @@ oe_status build_tables(oe_sweep* sw, const std::vector<int>& active) {
     return OE_OK;
 }
 
-// step_inc (H half only): the first launch advances this step counter.
+// step_inc (H half only): the first launch advances this step counter.  The
+// launches of one half write disjoint cells and read only the other field,
+// so with sw->overlap the slab groups (x/y CPML faces) go to side streams,
+// forked from and joined back into the sweep stream around its interior
+// launch.
 hipError_t yee_half(oe_sweep* sw, bool h, uint64_t* step_inc = nullptr) {
     bool first = true;
+    const bool ov = sw->overlap && sw->groups.size() > 1;
+    const int n_side = sw->overlap == 2 ? 3 : 1;
+    int used = 0;  // side streams forked so far
+    if (ov) HIP_RET(hipEventRecord(sw->ev_fork, sw->stream));
     for (const auto& G : sw->groups) {
         SweepArgs a{};
         a.st = sw->structs.as<SweepStruct>();
@@ -193,8 +224,22 @@ hipError_t yee_half(oe_sweep* sw, bool h, uint64_t* step_inc = nullptr) {
         a.xcd_remap = sw->xcd_remap;
         a.step_inc = (h && first) ? step_inc : nullptr;
         first = false;
-        hipError_t e = launch_sweep(h, a, G.n_wg, G.axes, G.vec, h ? sw->rows_h : sw->rows_e, sw->stream);
+        hipStream_t st = sw->stream;
+        if (ov && (G.axes & (AX_X | AX_Y)) != 0) {
+            // slab group: the next side stream (or the one side stream)
+            const int q = std::min(used, n_side - 1);
+            if (q == used) {
+                HIP_RET(hipStreamWaitEvent(sw->side[q], sw->ev_fork, 0));
+                ++used;
+            }
+            st = sw->side[q];
+        }
+        hipError_t e = launch_sweep(h, a, G.n_wg, G.axes, G.vec, h ? sw->rows_h : sw->rows_e, st);
         if (e != hipSuccess) return e;
+    }
+    for (int q = 0; q < used; ++q) {
+        HIP_RET(hipEventRecord(sw->ev_join[q], sw->side[q]));
+        HIP_RET(hipStreamWaitEvent(sw->stream, sw->ev_join[q], 0));
     }
     return hipSuccess;
 }
@@ -316,6 +361,14 @@ oe_status oe_sweep_create(int device, oe_sweep** out) {
     sw->device = device;
     HIP_OK(hipSetDevice(device));
     HIP_OK(hipStreamCreateWithFlags(&sw->stream, hipStreamNonBlocking));
+    if (const char* v = getenv("OE_SWEEP_OVERLAP")) sw->overlap = std::min(std::max(atoi(v), 0), 2);
+    if (sw->overlap) {
+        HIP_OK(hipEventCreateWithFlags(&sw->ev_fork, hipEventDisableTiming));
+        for (int q = 0; q < (sw->overlap == 2 ? 3 : 1); ++q) {
+            HIP_OK(hipStreamCreateWithFlags(&sw->side[q], hipStreamNonBlocking));
+            HIP_OK(hipEventCreateWithFlags(&sw->ev_join[q], hipEventDisableTiming));
+        }
+    }
     HIP_OK(sw->step.alloc(sizeof(uint64_t)));
     *out = sw.release();
     return OE_OK;

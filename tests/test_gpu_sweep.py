@@ -58,10 +58,14 @@ def _sweep(structs, use_graph, autotune):
 
 
 @pytest.mark.parametrize("use_graph", [True, False])
-def test_sweep_matches_single_engines(use_graph):
+@pytest.mark.parametrize("overlap", ["0", "1", "2"])
+def test_sweep_matches_single_engines(use_graph, overlap, monkeypatch):
+    """overlap 1: the CPML slab launches of each half-step run on a second
+    stream beside the interior launch; 2: on three (OE_SWEEP_OVERLAP)."""
+    monkeypatch.setenv("OE_SWEEP_OVERLAP", overlap)
     structs = _structures()
     sw, keep, res = _sweep(structs, use_graph, autotune=use_graph)
-    say(f"[sweep] graph={use_graph} tuning {sw.tuning()}")
+    say(f"[sweep] graph={use_graph} overlap={overlap} tuning {sw.tuning()}")
     for q, ms in enumerate(structs):
         eng1, port1, r1 = _single(ms)
         eng2, port2 = keep[q]
