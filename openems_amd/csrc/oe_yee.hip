@@ -127,6 +127,53 @@ struct MatIndex {
     const uint8_t* m8;
 };
 
+// A lane's V row indices at cell p, as loaded (unpacked by coef_from): the E
+// update loads them one plane ahead.
+struct MatIdx {
+    unsigned a = 0, b = 0;
+};
+
+template <int V>
+__device__ __forceinline__ MatIdx coef_index(const MatIndex& mi, size_t p) {
+    MatIdx r;
+    if (mi.m8 != nullptr) {
+        if constexpr (V == 4) r.a = __builtin_nontemporal_load(reinterpret_cast<const unsigned*>(mi.m8 + p));
+        else r.a = mi.m8[p];
+    } else if (mi.m16 != nullptr) {
+        if constexpr (V == 4) {
+            const u2v t = __builtin_nontemporal_load(reinterpret_cast<const u2v*>(mi.m16 + p));
+            r.a = t.x;
+            r.b = t.y;
+        } else {
+            r.a = mi.m16[p];
+        }
+    }
+    return r;
+}
+
+template <int V>
+__device__ __forceinline__ void coef_from(const MatIndex& mi, const MatIdx& x, const float2* tab, Vec<V>& a0,
+                                          Vec<V>& a1, Vec<V>& a2, Vec<V>& b0, Vec<V>& b1, Vec<V>& b2) {
+    unsigned m[V];
+    if constexpr (V == 4) {
+        if (mi.m8 != nullptr) {
+            m[0] = x.a & 0xffu; m[1] = (x.a >> 8) & 0xffu; m[2] = (x.a >> 16) & 0xffu; m[3] = x.a >> 24;
+        } else {
+            m[0] = x.a & 0xffffu; m[1] = x.a >> 16; m[2] = x.b & 0xffffu; m[3] = x.b >> 16;
+        }
+    } else {
+        m[0] = x.a;
+    }
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        const float2 cx = tab[3 * m[e]], cy = tab[3 * m[e] + 1], cz = tab[3 * m[e] + 2];
+        a0.v[e] = cx.x; b0.v[e] = cx.y;
+        a1.v[e] = cy.x; b1.v[e] = cy.y;
+        a2.v[e] = cz.x; b2.v[e] = cz.y;
+    }
+}
+
+// The index load and the row reads at once (the H update's form).
 template <int V>
 __device__ __forceinline__ void coef_rows(const MatIndex& mi, size_t p, const float2* tab, Vec<V>& a0,
                                           Vec<V>& a1, Vec<V>& a2, Vec<V>& b0, Vec<V>& b1, Vec<V>& b2) {
@@ -497,10 +544,20 @@ __device__ __forceinline__ void e_body(const Grid& g, const float* Fin, float* F
     // lanes of the folded Mur faces held by this thread
     const bool mur_lo = MUR && mz.on[0] && k0 == 0;
     const bool mur_hi = MUR && mz.on[1] && k0 + V == nz;
+    // material-table row indices one plane ahead: the LDS row reads of a
+    // plane need not wait for that plane's field loads
+    MatIdx mi_c;
+    if constexpr (TAB) {
+        if (gj && t.i0 > 0) mi_c = coef_index<V>(mat, p);
+    }
 
     for (int i = t.i0; i < t.i1; ++i, p += plane) {
         const bool gi = i > 0;
         const bool row = gi && gj;
+        MatIdx mi_n;
+        if constexpr (TAB) {
+            if (gj && i + 1 < t.i1) mi_n = coef_index<V>(mat, p + plane);
+        }
         const Vec<V> hx_c = ldv<V>(Hx + p), hy_c = ldv<V>(Hy + p), hz_c = ldv<V>(Hz + p);
         Vec<V> hz_j = hz_c, hx_j = hx_c;
         if (gj) {
@@ -520,7 +577,7 @@ __device__ __forceinline__ void e_body(const Grid& g, const float* Fin, float* F
         if (row) {
             ex = ldn<V, OE_NT_OWN>(Ex + p); ey = ldn<V, OE_NT_OWN>(Ey + p); ez = ldn<V, OE_NT_OWN>(Ez + p);
             if constexpr (TAB) {
-                coef_rows<V>(mat, p, tab, s0, s1, s2, c0, c1, c2);
+                coef_from<V>(mat, mi_c, tab, s0, s1, s2, c0, c1, c2);
             } else {
                 s0 = ldn<V, OE_NT_COEF>(ca0 + p); s1 = ldn<V, OE_NT_COEF>(ca1 + p); s2 = ldn<V, OE_NT_COEF>(ca2 + p);
                 c0 = ldn<V, OE_NT_COEF>(cb0 + p); c1 = ldn<V, OE_NT_COEF>(cb1 + p); c2 = ldn<V, OE_NT_COEF>(cb2 + p);
@@ -627,6 +684,7 @@ __device__ __forceinline__ void e_body(const Grid& g, const float* Fin, float* F
         }
         hy_p = hy_c;
         hz_p = hz_c;
+        mi_c = mi_n;
     }
 }
 
