@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import sys
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence, Tuple
 
@@ -65,6 +66,10 @@ class Operator:
         return cls(h.value, tuple(max(l.size - 1, 1) for l in ls), delta)
 
     def __del__(self):
+        # at interpreter exit the library's module state may already be
+        # gone; the process releases the device memory then
+        if sys.is_finalizing():
+            return
         if getattr(self, "_h", None) and self._h.value:
             lib().oe_operator_destroy(self._h)
             self._h = C.c_void_p()
@@ -175,6 +180,10 @@ class Extension:
         self.kind = kind
 
     def __del__(self):
+        # at interpreter exit the library's module state may already be
+        # gone; the process releases the device memory then
+        if sys.is_finalizing():
+            return
         if getattr(self, "_h", None) and self._h.value:
             lib().oe_extension_destroy(self._h)
             self._h = C.c_void_p()
@@ -332,6 +341,10 @@ class Nf2ff:
         self.frequency = float(frequency)
 
     def __del__(self):
+        # at interpreter exit the library's module state may already be
+        # gone; the process releases the device memory then
+        if sys.is_finalizing():
+            return
         if getattr(self, "_h", None) and self._h.value:
             lib().oe_nf2ff_destroy(self._h)
             self._h = C.c_void_p()
@@ -377,6 +390,10 @@ class Sweep:
         self._keep = []
 
     def __del__(self):
+        # at interpreter exit the library's module state may already be
+        # gone; the process releases the device memory then
+        if sys.is_finalizing():
+            return
         if getattr(self, "_h", None) and self._h.value:
             lib().oe_sweep_destroy(self._h)
             self._h = C.c_void_p()
@@ -504,7 +521,8 @@ class HipEngine:
             self._h = C.c_void_p()
 
     def __del__(self):
-        self.close()
+        if not sys.is_finalizing():  # see Operator.__del__
+            self.close()
 
     @property
     def timestep(self) -> int:

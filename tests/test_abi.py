@@ -128,3 +128,22 @@ def test_null_arguments_rejected():
     assert L.oe_engine_run_batch(None, None, None) == 1
     assert L.oe_engine_read_fields(None, None) == 1
     assert L.oe_ext_probe_read(None, None, 0, None) == 1
+
+
+def test_destructors_skip_the_library_at_interpreter_exit(monkeypatch):
+    """A handle still alive at interpreter exit is left to the process's end:
+    the destructor must not reach for the library, whose module state may
+    already be torn down."""
+    from openems_amd import engine as E
+
+    op = Operator.uniform(6, 6, 6, 1e-3)
+
+    def gone():
+        raise AssertionError("library called during interpreter exit")
+
+    monkeypatch.setattr(E.sys, "is_finalizing", lambda: True)
+    monkeypatch.setattr(E, "lib", gone)
+    op.__del__()  # returns without calling lib()
+    monkeypatch.undo()
+    op.__del__()  # a normal release afterwards
+    assert not op._h.value
