@@ -660,7 +660,8 @@ __global__ __launch_bounds__(MIRROR_THREADS) void k_nf2ff_reduce_mirror(ReduceAr
 // fac = jk e^{-jkr}/(4 pi r) (matlab/AnalyzeNF2FF.m:98-105) and
 // U = r^2/(2 Z0)(|E_theta|^2 + |E_phi|^2) (:112): out = [4][n_ang]
 // (E_theta re, im, E_phi re, im); the block maximum of U goes to umax[block].
-__global__ __launch_bounds__(256) void k_nf2ff_project(const double* __restrict__ NL, const double* __restrict__ theta,
+__global__ __launch_bounds__(256) void k_nf2ff_project(const double* __restrict__ NL, const double* __restrict__ NL2,
+                                                       const double* __restrict__ theta,
                                                        const double* __restrict__ phi, int n_phi, size_t n_ang,
                                                        double fr, double fi, double r, double* __restrict__ out,
                                                        double* __restrict__ umax) {
@@ -672,7 +673,10 @@ __global__ __launch_bounds__(256) void k_nf2ff_project(const double* __restrict_
         double st, ct, sp, cp;
         sincos(theta[it], &st, &ct);
         sincos(phi[ip], &sp, &cp);
-        const double* v = NL + 12 * a;
+        // N, L summed over the faces (NL2: the side stream's faces, or null)
+        double v[12];
+#pragma unroll
+        for (int c = 0; c < 12; ++c) v[c] = NL[12 * a + c] + (NL2 ? NL2[12 * a + c] : 0.0);
         const double ntr = v[0] * (ct * cp) + v[2] * (ct * sp) - v[4] * st;
         const double nti = v[1] * (ct * cp) + v[3] * (ct * sp) - v[5] * st;
         const double npr = -v[0] * sp + v[2] * cp, npi = -v[1] * sp + v[3] * cp;
@@ -783,7 +787,19 @@ struct Nf2ffCache {
     // transform, the others one GEMM launch each
     std::vector<std::unique_ptr<EventPair>> events;
     // byte offset and length of the B region last zeroed in `work`
-    size_t bm_off = SIZE_MAX, bm_len = 0;
+    size_t bm_off[2] = {SIZE_MAX, SIZE_MAX}, bm_len[2] = {0, 0};
+    // the side stream of the x/y faces, created on first use
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    Nf2ffCache() = default;
+    Nf2ffCache(const Nf2ffCache&) = delete;
+    Nf2ffCache& operator=(const Nf2ffCache&) = delete;
+    ~Nf2ffCache() {
+        if (side) (void)hipStreamSynchronize(side);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (side) (void)hipStreamDestroy(side);
+    }
 };
 
 Nf2ffCache* nf2ff_cache_new() { return new Nf2ffCache(); }
@@ -934,7 +950,7 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
         std::vector<double> xp, xq;
     };
     std::vector<Plan> plans;
-    size_t Kmax = KPAD, Nmax = GBN, coord_len = 0, blocks = 0;
+    size_t coord_len = 0, blocks = 0;
     for (const auto& f : faces) {
         const int a1 = (f.axis + 1) % 3, a2 = (f.axis + 2) % 3;
         Plan pl{};
@@ -949,53 +965,96 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
         pl.N = (int)round_up(8 * (size_t)pl.nq, GBN);
         for (double v : cp) pl.xp.push_back(v - center[pl.p]);
         for (double v : cq) pl.xq.push_back(v - center[pl.q]);
-        Kmax = std::max(Kmax, (size_t)pl.K);
-        Nmax = std::max(Nmax, (size_t)pl.N);
         coord_len += 2 * ((size_t)pl.np + pl.nq);
         blocks += (cp.size() * cq.size() + 255) / 256;
         plans.push_back(std::move(pl));
     }
+    // Faces run in up to two lanes: the z faces (the large contractions) on
+    // the caller's stream, the x and y faces on a side stream with buffers
+    // and an N/L accumulator of their own, so that their small,
+    // latency-bound launches overlap the large ones (OE_NF2FF_STREAMS=1: all
+    // on the caller's stream).
+    int n_streams = 2;
+    if (const char* v = getenv("OE_NF2FF_STREAMS")) n_streams = atoi(v) == 1 ? 1 : 2;
+    std::vector<int> lane_of(faces.size(), 0);
+    bool in_lane[2] = {false, false};
+    for (size_t fi = 0; fi < faces.size(); ++fi) {
+        lane_of[fi] = (n_streams == 2 && faces[fi].axis != 2) ? 1 : 0;
+        in_lane[lane_of[fi]] = true;
+    }
+    if (!in_lane[0] || !in_lane[1]) std::fill(lane_of.begin(), lane_of.end(), 0);
+    const int L = (in_lane[0] && in_lane[1]) ? 2 : 1;
+    struct LaneBuf {
+        size_t Kmax = KPAD, Nmax = GBN, rows = GBM, off = 0, bt_off = 0;
+        float *Bm = nullptr, *Am = nullptr, *G = nullptr;
+        _Float16* bt = nullptr;
+        double* nl = nullptr;
+    } lb[2];
     const size_t budget = (size_t)64 << 20;
-    int Umax = 0;
-    for (const auto& pl : plans) Umax = std::max(Umax, C.ax[pl.p].U);
-    size_t rows = std::max<size_t>(GBM, budget / Nmax / GBM * GBM);
-    rows = std::min(rows, round_up(2 * (size_t)Umax, GBM));
+    for (int l = 0; l < L; ++l) {
+        int Umax = 0;
+        for (size_t fi = 0; fi < faces.size(); ++fi) {
+            if (lane_of[fi] != l) continue;
+            lb[l].Kmax = std::max(lb[l].Kmax, (size_t)plans[fi].K);
+            lb[l].Nmax = std::max(lb[l].Nmax, (size_t)plans[fi].N);
+            Umax = std::max(Umax, C.ax[plans[fi].p].U);
+        }
+        const size_t r0 = std::max<size_t>(GBM, budget / lb[l].Nmax / GBM * GBM);
+        lb[l].rows = std::min(r0, round_up(2 * (size_t)Umax, GBM));
+    }
 
     const size_t pblocks = (n_ang + 255) / 256;
-    const size_t n_dbl = 12 * n_ang + 4 * n_ang + pblocks + coord_len + blocks;
-    const size_t f_off = round_up(n_dbl * sizeof(double), 256);
-    const size_t bt_off = round_up(f_off + (Kmax * Nmax + rows * Kmax + rows * Nmax + blocks) * sizeof(float), 256);
-    const size_t bytes = bt_off + Kmax * Nmax * sizeof(float) + 2 * faces.size() * sizeof(float);
+    const size_t n_dbl = 12 * n_ang * L + 4 * n_ang + pblocks + coord_len + blocks;
+    size_t off = round_up(n_dbl * sizeof(double), 256);
+    for (int l = 0; l < L; ++l) {
+        lb[l].off = off;
+        off = round_up(off + (lb[l].Kmax * lb[l].Nmax + lb[l].rows * lb[l].Kmax + lb[l].rows * lb[l].Nmax) *
+                                 sizeof(float), 256);
+        lb[l].bt_off = off;
+        off = round_up(off + lb[l].Kmax * lb[l].Nmax * sizeof(float), 256);
+    }
+    const size_t pmax_off = off;
+    off = round_up(off + blocks * sizeof(float), 256);
+    const size_t bsc_off = off;
+    const size_t bytes = off + 2 * faces.size() * sizeof(float);
     if (C.work.bytes < bytes) {
         NF_OK(C.work.alloc(bytes));
-        C.bm_off = SIZE_MAX;
+        C.bm_off[0] = C.bm_off[1] = SIZE_MAX;
     }
+    char* wk = reinterpret_cast<char*>(C.work.p);
     double* d_nl = C.work.as<double>();
-    double* d_ff = d_nl + 12 * n_ang;
+    double* d_ff = d_nl + 12 * n_ang * L;
     double* d_umax = d_ff + 4 * n_ang;
     double* d_coord = d_umax + pblocks;
     double* d_part = d_coord + coord_len;
-    float* Bm = reinterpret_cast<float*>(reinterpret_cast<char*>(C.work.p) + round_up(n_dbl * sizeof(double), 256));
     // B is zeroed only when its region moves or grows.  Its padding rows
     // (p >= np) are cleared per face after the pack (k_nf2ff_pad_rows), and
     // A's padding columns are exact zeros (k_phase), so they add signed zeros
     // to every product; B's padding columns only reach G columns the
     // reduction never reads.
-    {
-        const size_t off = round_up(n_dbl * sizeof(double), 256), len = Kmax * Nmax * sizeof(float);
-        if (off != C.bm_off || len > C.bm_len) {
-            NF_OK(hipMemsetAsync(Bm, 0, len, s));
-            C.bm_off = off;
-            C.bm_len = len;
+    for (int l = 0; l < L; ++l) {
+        LaneBuf& b = lb[l];
+        b.Bm = reinterpret_cast<float*>(wk + b.off);
+        b.Am = b.Bm + b.Kmax * b.Nmax;
+        b.G = b.Am + b.rows * b.Kmax;
+        // split fp16 halves of B (transposed, [N][K] each)
+        b.bt = reinterpret_cast<_Float16*>(wk + b.bt_off);
+        b.nl = d_nl + 12 * n_ang * l;
+        const size_t len = b.Kmax * b.Nmax * sizeof(float);
+        if (b.off != C.bm_off[l] || len > C.bm_len[l]) {
+            NF_OK(hipMemsetAsync(b.Bm, 0, len, s));
+            C.bm_off[l] = b.off;
+            C.bm_len[l] = len;
         }
     }
-    float* Am = Bm + Kmax * Nmax;
-    float* G = Am + rows * Kmax;
-    float* d_pmax = G + rows * Nmax;  // the pack's block maxima of |B|, per face as d_part
-    // split fp16 halves of B (transposed, [N][K] each) and per face the scale
-    // of B and its inverse
-    _Float16* d_bt = reinterpret_cast<_Float16*>(reinterpret_cast<char*>(C.work.p) + bt_off);
-    float* d_bsc = reinterpret_cast<float*>(reinterpret_cast<char*>(C.work.p) + bt_off + Kmax * Nmax * sizeof(float));
+    float* d_pmax = reinterpret_cast<float*>(wk + pmax_off);  // the pack's block maxima of |B|, per face as d_part
+    // per face the scale of B and its inverse
+    float* d_bsc = reinterpret_cast<float*>(wk + bsc_off);
+    if (L == 2 && !C.side) {
+        NF_OK(hipStreamCreateWithFlags(&C.side, hipStreamNonBlocking));
+        NF_OK(hipEventCreateWithFlags(&C.ev_fork, hipEventDisableTiming));
+        NF_OK(hipEventCreateWithFlags(&C.ev_join, hipEventDisableTiming));
+    }
     // the large contractions on the FP16 matrix cores (operands split in two
     // halves, k_hgemm3) unless OE_NF2FF_SPLIT16=0
     bool split16 = true;
@@ -1016,7 +1075,7 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
         coords.insert(coords.end(), wq.begin(), wq.end());
     }
     NF_OK(hipMemcpyAsync(d_coord, coords.data(), coords.size() * 8, hipMemcpyHostToDevice, s));
-    NF_OK(hipMemsetAsync(d_nl, 0, 12 * n_ang * sizeof(double), s));
+    NF_OK(hipMemsetAsync(d_nl, 0, 12 * n_ang * L * sizeof(double), s));
 
     size_t n_ev = 0;
     auto next_events = [&](EventPair** out) -> hipError_t {
@@ -1035,35 +1094,59 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
     std::vector<bool> gemm_split;  // launch ran the split FP16 kernel
     double flop_split = 0.0;
     NF_OK(hipEventRecord(all->a, s));
+    if (L == 2) {
+        NF_OK(hipEventRecord(C.ev_fork, s));
+        NF_OK(hipStreamWaitEvent(C.side, C.ev_fork, 0));
+    }
     double flop = 0.0;
-    size_t co = 0, bo = 0;
-    for (size_t fi = 0; fi < faces.size(); ++fi) {
+    // per face, its offsets into the coordinates and the pack's block sums
+    std::vector<size_t> co_of(faces.size()), bo_of(faces.size());
+    for (size_t fi = 0, co = 0, bo = 0; fi < faces.size(); ++fi) {
+        co_of[fi] = co;
+        bo_of[fi] = bo;
+        co += 2 * ((size_t)plans[fi].np + plans[fi].nq);
+        bo += (faces[fi].c1.size() * faces[fi].c2.size() + 255) / 256;
+    }
+    // the caller's stream first: its long contractions then start while
+    // the host still issues the side stream's faces
+    std::vector<size_t> order;
+    for (int l = 0; l < L; ++l)
+        for (size_t fi = 0; fi < faces.size(); ++fi)
+            if (lane_of[fi] == l) order.push_back(fi);
+    for (size_t fi : order) {
         const Nf2ffFace& f = faces[fi];
+        // this face's lane: its stream and buffers
+        const int ln = lane_of[fi];
+        const hipStream_t fs = ln ? C.side : s;
+        float* Bm = lb[ln].Bm;
+        float* Am = lb[ln].Am;
+        float* G = lb[ln].G;
+        _Float16* d_bt = lb[ln].bt;
+        const size_t rows = lb[ln].rows;
         const Plan& pl = plans[fi];
         const auto& A = C.ax[pl.p];
         const int n1 = (int)f.c1.size(), n2 = (int)f.c2.size();
         const int a1 = (f.axis + 1) % 3, a2 = (f.axis + 2) % 3;
-        const double* xp = d_coord + co;
+        const double* xp = d_coord + co_of[fi];
         const double* wp = xp + pl.np;
         const double* xq = wp + pl.np;
         const double* wq = xq + pl.nq;
-        co += 2 * ((size_t)pl.np + pl.nq);
+        const size_t bo = bo_of[fi];
         const int nb = (n1 * n2 + 255) / 256;
         if (f.src_kind == 0)
-            hipLaunchKernelGGL(k_nf2ff_pack<0>, dim3(nb), dim3(256), 0, s, f.src, f.scale, n1, n2, pl.swap, f.sign,
+            hipLaunchKernelGGL(k_nf2ff_pack<0>, dim3(nb), dim3(256), 0, fs, f.src, f.scale, n1, n2, pl.swap, f.sign,
                                wp, wq, Bm, pl.N, d_part + bo, d_pmax + bo);
         else
-            hipLaunchKernelGGL(k_nf2ff_pack<1>, dim3(nb), dim3(256), 0, s, f.src, f.scale, n1, n2, pl.swap, f.sign,
+            hipLaunchKernelGGL(k_nf2ff_pack<1>, dim3(nb), dim3(256), 0, fs, f.src, f.scale, n1, n2, pl.swap, f.sign,
                                wp, wq, Bm, pl.N, d_part + bo, d_pmax + bo);
         NF_OK(hipGetLastError());
         if (pl.K > pl.np) {
             const size_t npad = (size_t)(pl.K - pl.np) * pl.N;
             hipLaunchKernelGGL(k_nf2ff_pad_rows, dim3((unsigned)std::min<size_t>((npad + 255) / 256, 1024)), dim3(256),
-                               0, s, Bm, pl.N, pl.np, pl.K);
+                               0, fs, Bm, pl.N, pl.np, pl.K);
             NF_OK(hipGetLastError());
         }
         const float* face_pmax = d_pmax + bo;
-        bo += nb;
         const int uchunk = (int)(rows / 2);
         bool face16 = false;  // some contraction of this face runs on the FP16 matrix cores
         for (int u0 = 0; u0 < A.U; u0 += uchunk) {
@@ -1074,7 +1157,7 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
         _Float16* Blo = d_bt + (size_t)pl.N * pl.K;
         float* bsc = d_bsc + 2 * fi;
         if (face16) {
-            hipLaunchKernelGGL(k_bsplit, dim3(pl.N / 64 + (pl.N % 64 != 0), (pl.K + 63) / 64), dim3(256), 0, s, Bm, pl.K,
+            hipLaunchKernelGGL(k_bsplit, dim3(pl.N / 64 + (pl.N % 64 != 0), (pl.K + 63) / 64), dim3(256), 0, fs, Bm, pl.K,
                                pl.N, pl.np, 8 * pl.nq, face_pmax, nb, Bhi, Blo, bsc);
             NF_OK(hipGetLastError());
         }
@@ -1098,7 +1181,7 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
         ra.x0 = f.c0 - center[f.axis];
         ra.k = k;
         ra.slot[0] = a1; ra.slot[1] = a2; ra.slot[2] = 3 + a1; ra.slot[3] = 3 + a2;
-        ra.NL = d_nl;
+        ra.NL = lb[ln].nl;
         for (int u0 = 0; u0 < A.U; u0 += uchunk) {
             const int u1 = std::min(A.U, u0 + uchunk);
             const int mp = (int)round_up(2 * (size_t)(u1 - u0), GBM);
@@ -1108,24 +1191,24 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
             _Float16* Ahi = reinterpret_cast<_Float16*>(Am);
             _Float16* Alo = Ahi + total;
             if (fp16)
-                hipLaunchKernelGGL(k_phase16, dim3(std::max(1, pb / 8)), dim3(256), 0, s, Ahi, Alo, pl.K, total, u0, u1,
+                hipLaunchKernelGGL(k_phase16, dim3(std::max(1, pb / 8)), dim3(256), 0, fs, Ahi, Alo, pl.K, total, u0, u1,
                                    pl.np, A.uval.as<double>(), xp, k, uni_p ? 1 : 0, dxp);
             else
-                hipLaunchKernelGGL(k_phase, dim3(pb), dim3(256), 0, s, Am, pl.K, total, u0, u1, pl.np,
+                hipLaunchKernelGGL(k_phase, dim3(pb), dim3(256), 0, fs, Am, pl.K, total, u0, u1, pl.np,
                                    A.uval.as<double>(), xp, k);
             NF_OK(hipGetLastError());
             EventPair* ev = nullptr;
             NF_OK(next_events(&ev));
             gemm_ev.push_back(ev);
-            NF_OK(hipEventRecord(ev->a, s));
+            NF_OK(hipEventRecord(ev->a, fs));
             if (fp16) {
-                hipLaunchKernelGGL(k_hgemm3, dim3(pl.N / 128, mp / 128), dim3(256), 0, s, Ahi, Alo, Bhi, Blo, G, mp, pl.N,
+                hipLaunchKernelGGL(k_hgemm3, dim3(pl.N / 128, mp / 128), dim3(256), 0, fs, Ahi, Alo, Bhi, Blo, G, mp, pl.N,
                                    pl.K, bsc);
                 NF_OK(hipGetLastError());
             } else {
-                NF_OK(launch_gemm(Am, Bm, G, mp, pl.N, pl.K, s));
+                NF_OK(launch_gemm(Am, Bm, G, mp, pl.N, pl.K, fs));
             }
-            NF_OK(hipEventRecord(gemm_ev.back()->b, s));
+            NF_OK(hipEventRecord(gemm_ev.back()->b, fs));
             gemm_split.push_back(fp16);
             const double fl = 2.0 * (2.0 * (u1 - u0)) * (8.0 * pl.nq) * pl.np;  // useful (unpadded) work
             flop += fl;
@@ -1141,20 +1224,25 @@ hipError_t nf2ff_compute(const std::vector<Nf2ffFace>& faces, double freq, const
                               M.mstart[u1]};
                 const dim3 grid((nm + MIRROR_GROUPS - 1) / MIRROR_GROUPS);
                 if (ra.uniform)
-                    hipLaunchKernelGGL(k_nf2ff_reduce_mirror<true>, grid, dim3(MIRROR_THREADS), 0, s, ra, ma);
+                    hipLaunchKernelGGL(k_nf2ff_reduce_mirror<true>, grid, dim3(MIRROR_THREADS), 0, fs, ra, ma);
                 else
-                    hipLaunchKernelGGL(k_nf2ff_reduce_mirror<false>, grid, dim3(MIRROR_THREADS), 0, s, ra, ma);
+                    hipLaunchKernelGGL(k_nf2ff_reduce_mirror<false>, grid, dim3(MIRROR_THREADS), 0, fs, ra, ma);
             } else {
                 const int ng = ra.g1 - ra.g0;
                 hipLaunchKernelGGL(k_nf2ff_reduce, dim3((ng + REDUCE_GROUPS - 1) / REDUCE_GROUPS),
-                                   dim3(REDUCE_THREADS), 0, s, ra);
+                                   dim3(REDUCE_THREADS), 0, fs, ra);
             }
             NF_OK(hipGetLastError());
         }
     }
+    if (L == 2) {
+        NF_OK(hipEventRecord(C.ev_join, C.side));
+        NF_OK(hipStreamWaitEvent(s, C.ev_join, 0));
+    }
     typedef std::complex<double> cd;
     const cd fac = cd(0.0, k) * std::exp(cd(0.0, -k * r)) / (4.0 * M_PI * r);
-    hipLaunchKernelGGL(k_nf2ff_project, dim3((unsigned)pblocks), dim3(256), 0, s, d_nl, d_kap + 3 * n_ang,
+    hipLaunchKernelGGL(k_nf2ff_project, dim3((unsigned)pblocks), dim3(256), 0, s, d_nl, L == 2 ? lb[1].nl : nullptr,
+                       d_kap + 3 * n_ang,
                        d_kap + 3 * n_ang + n_theta, n_phi, n_ang, fac.real(), fac.imag(), r, d_ff, d_umax);
     NF_OK(hipGetLastError());
     NF_OK(hipEventRecord(all->b, s));

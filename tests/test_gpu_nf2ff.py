@@ -156,6 +156,36 @@ def test_mirror_groups_against_per_angle_reduction(monkeypatch):
     assert g.prad == g0.prad
 
 
+def test_side_stream_faces_equal_one_stream(monkeypatch):
+    """The x and y faces on a side stream with their own buffers and N/L
+    accumulator (the default) give the one-stream transform
+    (OE_NF2FF_STREAMS=1) up to the order of the final N/L sum; a second call
+    reuses the side stream and buffers and repeats the first bit for bit."""
+    f = 6e9
+    lam = N.C0 / f
+    faces = N.box_faces([-0.7 * lam, -0.6 * lam, -0.8 * lam], [0.7 * lam, 0.6 * lam, 0.8 * lam], (37, 31, 45),
+                        lambda X_, Y, Z: N.hertzian_dipole_fields(1e-3, f, X_ - 0.002, Y + 0.001, Z))
+    th = np.linspace(0, math.pi, 61)
+    ph = np.linspace(0, 2 * math.pi, 121)[:-1]
+    out = {}
+    for streams in ("1", "2"):
+        monkeypatch.setenv("OE_NF2FF_STREAMS", streams)
+        out[streams] = _gpu_far_field(faces, f, th, ph)
+    g1, g2 = out["1"], out["2"]
+    d = math.sqrt((np.linalg.norm(g2.e_theta - g1.e_theta) ** 2 + np.linalg.norm(g2.e_phi - g1.e_phi) ** 2)
+                  / (np.linalg.norm(g1.e_theta) ** 2 + np.linalg.norm(g1.e_phi) ** 2))
+    Et, Ep, prad, _ = N.far_field(faces, f, th[::6], ph)
+    err = math.sqrt((np.linalg.norm(g2.e_theta[::6] - Et) ** 2 + np.linalg.norm(g2.e_phi[::6] - Ep) ** 2)
+                    / (np.linalg.norm(Et) ** 2 + np.linalg.norm(Ep) ** 2))
+    say(f"[nf2ff] side-stream faces: rel L2 {d:.2e} vs one stream, {err:.2e} vs oracle; "
+        f"device {g2.total_ms:.3f} ms vs {g1.total_ms:.3f} ms")
+    assert d <= 1e-12
+    assert err <= PATTERN_TOL
+    assert g2.prad == g1.prad
+    again = _gpu_far_field(faces, f, th, ph)
+    assert np.array_equal(again.e_theta, g2.e_theta) and np.array_equal(again.e_phi, g2.e_phi)
+
+
 def _fdtd_dipole(precision, n=48, steps=900, stride=3):
     d = 1e-3
     op = Operator.uniform(n, n, n, d)
