@@ -228,11 +228,25 @@ oe_status materialise(oe_engine* e, const oe_operator* op) {
         HIP_OK(launch_fill(e->Cf + (size_t)(6 + d) * N, N, 1.0f, e->stream));
         HIP_OK(launch_fill(e->Cf + (size_t)(9 + d) * N, N, (float)(-(dt / (MU0 * op->d[d]))), e->stream));
     }
+    // The region writes in list order (later ones win where boxes meet),
+    // cut into batches of pairwise-disjoint boxes: a box meeting one of the
+    // current batch starts the next, so each batch is one launch and the
+    // order of writes to any cell is kept.
+    std::vector<BoxFill> fills;
+    auto put = [&](const int bx[6], unsigned mask, const float v[12]) {
+        if (bx[1] <= bx[0] || bx[3] <= bx[2] || bx[5] <= bx[4]) return;
+        BoxFill f{};
+        for (int q = 0; q < 6; ++q) f.box[q] = bx[q];
+        f.mask = mask;
+        for (int q = 0; q < 12; ++q) f.v[q] = v[q];
+        fills.push_back(f);
+    };
     for (size_t oi = 0; oi < op->ops.size(); ++oi) {
         const MatOp& m = op->ops[oi];
         int box[6];
         const size_t lim[6] = {0, (size_t)g.nx, 0, (size_t)g.ny, 0, (size_t)g.nz};
         for (int q = 0; q < 6; ++q) box[q] = (int)std::min(m.box[q], lim[q | 1]);
+        float v[12] = {};
         if (m.kind == 2) {
             // Lumped resistor: per edge, the material in force before this op
             // plus the resistor's conductivity (python/openEMS/ports.py:177-181).
@@ -258,14 +272,14 @@ oe_status materialise(oe_engine* e, const oe_operator* op) {
                         const double eps = eps_r * EPS0;
                         const double loss = (se + sig_add) * dt / (2.0 * eps);
                         const int cell[6] = {i, i + 1, j, j + 1, k, k + 1};
-                        HIP_OK(launch_box_fill(e->Cf + (size_t)d * N, g, cell, (float)((1.0 - loss) / (1.0 + loss)), e->stream));
-                        HIP_OK(launch_box_fill(e->Cf + (size_t)(3 + d) * N, g, cell,
-                                               (float)((dt / eps) / (1.0 + loss) / op->d[d]), e->stream));
+                        v[d] = (float)((1.0 - loss) / (1.0 + loss));
+                        v[3 + d] = (float)((dt / eps) / (1.0 + loss) / op->d[d]);
+                        put(cell, (1u << d) | (1u << (3 + d)), v);
                     }
             continue;
         }
         if (m.kind == 1) {  // src/fdtd/operator.rs:249-263
-            for (int d = 0; d < 6; ++d) HIP_OK(launch_box_fill(e->Cf + (size_t)d * N, g, box, 0.0f, e->stream));
+            put(box, 0x3fu, v);
             continue;
         }
         // src/fdtd/operator.rs:195-241 (computed in double, stored as float)
@@ -278,12 +292,47 @@ oe_status materialise(oe_engine* e, const oe_operator* op) {
         const float da = (float)((1.0 - loss_m) / (1.0 + loss_m));
         const double dbf = -(dt / mu) / (1.0 + loss_m);
         for (int d = 0; d < 3; ++d) {
-            HIP_OK(launch_box_fill(e->Cf + (size_t)d * N, g, box, ca, e->stream));
-            HIP_OK(launch_box_fill(e->Cf + (size_t)(3 + d) * N, g, box, (float)(cbf / op->d[d]), e->stream));
-            HIP_OK(launch_box_fill(e->Cf + (size_t)(6 + d) * N, g, box, da, e->stream));
-            HIP_OK(launch_box_fill(e->Cf + (size_t)(9 + d) * N, g, box, (float)(dbf / op->d[d]), e->stream));
+            v[d] = ca;
+            v[3 + d] = (float)(cbf / op->d[d]);
+            v[6 + d] = da;
+            v[9 + d] = (float)(dbf / op->d[d]);
         }
+        put(box, 0xfffu, v);
     }
+    if (fills.empty()) return OE_OK;
+    auto meets = [](const BoxFill& x, const BoxFill& y) {
+        return x.box[0] < y.box[1] && y.box[0] < x.box[1] && x.box[2] < y.box[3] && y.box[2] < x.box[3] &&
+               x.box[4] < y.box[5] && y.box[4] < x.box[5];
+    };
+    constexpr size_t MAX_BATCH = 1024;  // bounds the host's pairwise test
+    std::vector<size_t> starts{0};
+    for (size_t q = 1; q < fills.size(); ++q) {
+        bool cut = q - starts.back() >= MAX_BATCH;
+        for (size_t r = starts.back(); r < q && !cut; ++r) cut = meets(fills[r], fills[q]);
+        if (cut) starts.push_back(q);
+    }
+    starts.push_back(fills.size());
+    std::vector<int> wg_first(fills.size());
+    std::vector<int> n_wg(starts.size() - 1);
+    for (size_t b = 0; b + 1 < starts.size(); ++b) {
+        int w = 0;
+        for (size_t q = starts[b]; q < starts[b + 1]; ++q) {
+            wg_first[q] = w;
+            w += boxes_fill_blocks(fills[q]);
+        }
+        n_wg[b] = w;
+    }
+    DevBuf dev;
+    const size_t fb = fills.size() * sizeof(BoxFill);
+    HIP_OK(dev.alloc(fb + wg_first.size() * sizeof(int)));
+    HIP_OK(hipMemcpy(dev.p, fills.data(), fb, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(dev.as<char>() + fb, wg_first.data(), wg_first.size() * sizeof(int), hipMemcpyHostToDevice));
+    const BoxFill* d_fills = dev.as<BoxFill>();
+    const int* d_wg = reinterpret_cast<const int*>(dev.as<char>() + fb);
+    for (size_t b = 0; b + 1 < starts.size(); ++b)
+        HIP_OK(launch_boxes_fill(e->Cf, g, d_fills + starts[b], d_wg + starts[b], (int)(starts[b + 1] - starts[b]),
+                                 n_wg[b], e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));  // the launches read `dev`
     return OE_OK;
 }
 

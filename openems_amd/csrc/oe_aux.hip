@@ -30,6 +30,36 @@ __global__ void k_box_fill(float* __restrict__ base, int ny, int nz, int i0, int
     }
 }
 
+// Many region writes in one launch: boxes that are pairwise disjoint, each
+// writing the coefficient arrays of its mask (bit a: array a of the 12).
+// Work-group w fills cells [(w - wg_first[o]) BF_CELLS, ...) of box o, the
+// last box whose wg_first is at most w.
+constexpr int BF_CELLS = 2048;
+__global__ __launch_bounds__(256) void k_boxes_fill(float* __restrict__ Cf, size_t N, int ny, int nz,
+                                                    const BoxFill* __restrict__ ops,
+                                                    const int* __restrict__ wg_first, int n_ops) {
+    int lo = 0, hi = n_ops - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (wg_first[mid] <= (int)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const BoxFill& o = ops[lo];
+    const int ni = o.box[1] - o.box[0], nj = o.box[3] - o.box[2], nk = o.box[5] - o.box[4];
+    const size_t cnt = (size_t)ni * nj * nk;
+    const size_t c0 = (size_t)(blockIdx.x - wg_first[lo]) * BF_CELLS;
+    const size_t c1 = c0 + BF_CELLS < cnt ? c0 + BF_CELLS : cnt;
+    for (size_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+        const int k = (int)(c % nk);
+        const size_t r = c / nk;
+        const int j = (int)(r % nj);
+        const int i = (int)(r / nj);
+        const size_t p = ((size_t)(o.box[0] + i) * ny + (o.box[2] + j)) * nz + (o.box[4] + k);
+        for (int a = 0; a < 12; ++a)
+            if ((o.mask >> a) & 1u) Cf[(size_t)a * N + p] = o.v[a];
+    }
+}
+
 // Excitations (BasicEngine::apply_excitations, src/fdtd/basic_engine.rs:313-395)
 // run in list order on one lane, as repeated sources on one edge must add up
 // in that order; then each probe lane records its node (FieldProbe::sample,
@@ -528,6 +558,19 @@ hipError_t launch_box_fill(float* base, const Grid& g, const int box[6], float v
     const size_t cnt = (size_t)ni * nj * nk;
     hipLaunchKernelGGL(k_box_fill, dim3(grid_for(cnt, 256, 8192)), dim3(256), 0, s, base, g.ny, g.nz, box[0],
                        box[2], box[4], ni, nj, nk, v);
+    return hipGetLastError();
+}
+
+int boxes_fill_blocks(const BoxFill& o) {
+    const size_t cnt = (size_t)(o.box[1] - o.box[0]) * (o.box[3] - o.box[2]) * (o.box[5] - o.box[4]);
+    return (int)((cnt + BF_CELLS - 1) / BF_CELLS);
+}
+
+hipError_t launch_boxes_fill(float* Cf, const Grid& g, const BoxFill* ops, const int* wg_first, int n_ops, int n_wg,
+                             hipStream_t s) {
+    if (n_ops <= 0 || n_wg <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_boxes_fill, dim3((unsigned)n_wg), dim3(256), 0, s, Cf, g.N, g.ny, g.nz, ops, wg_first,
+                       n_ops);
     return hipGetLastError();
 }
 

@@ -227,6 +227,19 @@ hipError_t launch_sweep(bool h, const SweepArgs& a, int n_wg, int axes, int vec,
 // Auxiliary kernels (oe_aux.hip).
 hipError_t launch_fill(float* p, size_t n, float v, hipStream_t s);
 hipError_t launch_box_fill(float* base, const Grid& g, const int box[6], float v, hipStream_t s);
+// One region write of the coefficient block: cells [box0, box1) x [box2,
+// box3) x [box4, box5) of array a get v[a] for every bit a of mask.
+struct BoxFill {
+    int box[6];
+    unsigned mask;
+    float v[12];
+};
+// Work-groups of one box in launch_boxes_fill.
+int boxes_fill_blocks(const BoxFill& o);
+// Pairwise-disjoint boxes in one launch; wg_first[o]: the first of box o's
+// work-groups (the running sum of boxes_fill_blocks), n_wg their total.
+hipError_t launch_boxes_fill(float* Cf, const Grid& g, const BoxFill* ops, const int* wg_first, int n_ops, int n_wg,
+                             hipStream_t s);
 
 struct PointExc {
     uint32_t idx;   // linear cell index
