@@ -15,22 +15,8 @@ __global__ void k_fill(float* __restrict__ p, size_t n, float v) {
     for (; i < n; i += stride) p[i] = v;
 }
 
-// Operator::set_material / set_pec region writes (src/fdtd/operator.rs:222-262).
-__global__ void k_box_fill(float* __restrict__ base, int ny, int nz, int i0, int j0, int k0,
-                           int ni, int nj, int nk, float v) {
-    const size_t cnt = (size_t)ni * nj * nk;
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; t < cnt; t += stride) {
-        const int k = (int)(t % nk);
-        const size_t r = t / nk;
-        const int j = (int)(r % nj);
-        const int i = (int)(r / nj);
-        base[((size_t)(i0 + i) * ny + (j0 + j)) * nz + (k0 + k)] = v;
-    }
-}
-
-// Many region writes in one launch: boxes that are pairwise disjoint, each
+// Operator::set_material / set_pec region writes (src/fdtd/operator.rs:222-262),
+// many in one launch: boxes that are pairwise disjoint, each
 // writing the coefficient arrays of its mask (bit a: array a of the 12).
 // Work-group w fills cells [(w - wg_first[o]) BF_CELLS, ...) of box o, the
 // last box whose wg_first is at most w.
@@ -549,15 +535,6 @@ int grid_for(size_t n, int threads, int cap) {
 
 hipError_t launch_fill(float* p, size_t n, float v, hipStream_t s) {
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, p, n, v);
-    return hipGetLastError();
-}
-
-hipError_t launch_box_fill(float* base, const Grid& g, const int box[6], float v, hipStream_t s) {
-    const int ni = box[1] - box[0], nj = box[3] - box[2], nk = box[5] - box[4];
-    if (ni <= 0 || nj <= 0 || nk <= 0) return hipSuccess;
-    const size_t cnt = (size_t)ni * nj * nk;
-    hipLaunchKernelGGL(k_box_fill, dim3(grid_for(cnt, 256, 8192)), dim3(256), 0, s, base, g.ny, g.nz, box[0],
-                       box[2], box[4], ni, nj, nk, v);
     return hipGetLastError();
 }
 

@@ -226,7 +226,6 @@ hipError_t launch_sweep(bool h, const SweepArgs& a, int n_wg, int axes, int vec,
 
 // Auxiliary kernels (oe_aux.hip).
 hipError_t launch_fill(float* p, size_t n, float v, hipStream_t s);
-hipError_t launch_box_fill(float* base, const Grid& g, const int box[6], float v, hipStream_t s);
 // One region write of the coefficient block: cells [box0, box1) x [box2,
 // box3) x [box4, box5) of array a get v[a] for every bit a of mask.
 struct BoxFill {
