@@ -778,7 +778,7 @@ def yee_point(wl, mode, steps, warmup, period, tiles="", coef_opt="auto", fused=
         "bound": "hbm",
         "kernel": {"update_h:interior": "k_update_h<4,AX_Z,phys> (interior box, z-face CPML)",
                    "update_e:interior": "k_update_e<4,AX_Z,phys> (interior box, z-face CPML)",
-                   "fused:interior": "k_fused<AX_Z,phys> (H and E of the interior box in one pass)"}.get(dom, dom),
+                   "fused:interior": "k_fused_rows<AX_Z,phys> (H and E of the interior box in one pass)"}.get(dom, dom),
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",

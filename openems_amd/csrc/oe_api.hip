@@ -1756,6 +1756,9 @@ oe_status oe_engine_run_batch(oe_engine* e, const oe_batch_desc* b, oe_batch_res
         fa.ci = std::max(1, std::min(std::max(ci, 4), 32));
         fa.n_ib = inner ? (ni + fa.ci - 1) / fa.ci : 0;
         fa.xcd_remap = e->xcd_remap;
+        fa.kind = 2;
+        if (const char* v = getenv("OE_FUSED_KIND")) fa.kind = atoi(v) == 1 ? 1 : 2;
+        if (fa.kind == 2 && e->mat8 && e->n_mat[0] >= 2 && e->n_mat[0] <= 256) fa.mat8_e = e->mat8;
     }
     double peak = b->persistent_peak ? e->peak_persistent : 0.0;
     std::vector<uint64_t> sample_steps;  // timesteps whose energy is pending in energy_dev

@@ -175,6 +175,8 @@ struct FusedArgs {
     int ci;                // planes per work-group
     int n_jb, n_ib;
     int xcd_remap;
+    const uint8_t* mat8_e;  // byte-wide E indices (2 to 256 rows) or null
+    int kind;               // 1: rows exchanged through LDS; 2: recomputed per wave (k_fused_rows)
 };
 // Whether the fused step handles this grid (V = 4 lanes, every k-row inside
 // one wave).  Below nz = 128 most lanes of a row idle, so by default the

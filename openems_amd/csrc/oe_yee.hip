@@ -1092,6 +1092,287 @@ __global__ __launch_bounds__((FUSED_ROWS + 1) * 64) void k_fused(FusedArgs a, Cp
     }
 }
 
+// The fused step without a barrier: FUSED_ROWS waves per work-group, each
+// the owner of one k-row j, marching along i.  Its E update needs the new Hx
+// and Hz of row j - 1; the wave recomputes them from the old fields instead
+// of taking them from the wave below (that row's owner computes the same
+// values with the same arithmetic), so no wave waits for another.  The
+// recomputation reads the old z-face H psi (c.ph_*, the owners write
+// c.phw_*) and stores nothing.  Rows and planes just outside the box (CPML
+// slab rows and planes, whose new H the slab launch wrote to Fn before this
+// one) are read from Fn.  The E update's row indices are loaded one plane
+// ahead.
+template <int AX, bool PHYS, bool TAB>
+__global__ __launch_bounds__(FUSED_ROWS * 64) void k_fused_rows(FusedArgs a, CpmlArgs c) {
+    constexpr int V = 4;
+    constexpr bool ZC = (AX & AX_Z) != 0;
+    extern __shared__ float2 ftab[];
+    float2* tabh = ftab;
+    float2* tabe = ftab + 3 * a.n_mat_h;
+    const int tid = threadIdx.x, w = tid >> 6, kx = tid & 63;
+    if constexpr (TAB) {
+        const float2* sh = reinterpret_cast<const float2*>(a.tab_h);
+        const float2* se = reinterpret_cast<const float2*>(a.tab_e);
+        for (int q = tid; q < 3 * a.n_mat_h; q += blockDim.x) tabh[q] = sh[q];
+        for (int q = tid; q < 3 * a.n_mat_e; q += blockDim.x) tabe[q] = se[q];
+        __syncthreads();
+    }
+    int bid = blockIdx.x;
+    if (a.xcd_remap) {
+        const int per = gridDim.x >> 3;
+        if (bid < (per << 3)) bid = (bid & 7) * per + (bid >> 3);
+    }
+    const int jb = bid % a.n_jb;
+    const int ia = a.i0 + (bid / a.n_jb) * a.ci;
+    const int ie = min(ia + a.ci, a.i1);
+    const int j = a.j0 + jb * FUSED_ROWS + w;
+    const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
+    const int k0 = kx * V;
+    // no barrier follows: a lane past the row or the box leaves (the k - 1
+    // shuffles read lower lanes only)
+    if (j >= a.j1 || k0 >= nz) return;
+    const size_t N = a.g.N;
+    const size_t plane = (size_t)ny * nz;
+    const float* __restrict__ Ex = a.Fo;
+    const float* __restrict__ Ey = a.Fo + N;
+    const float* __restrict__ Ez = a.Fo + 2 * N;
+    const float* __restrict__ Hxo = a.Fo + 3 * N;
+    const float* __restrict__ Hyo = a.Fo + 4 * N;
+    const float* __restrict__ Hzo = a.Fo + 5 * N;
+    float* __restrict__ Exn = a.Fn;
+    float* __restrict__ Eyn = a.Fn + N;
+    float* __restrict__ Ezn = a.Fn + 2 * N;
+    float* __restrict__ Hxn = a.Fn + 3 * N;
+    float* __restrict__ Hyn = a.Fn + 4 * N;
+    float* __restrict__ Hzn = a.Fn + 5 * N;
+    const MatIndex mh{a.n_mat_h == 1 ? nullptr : a.mat_h, nullptr};
+    const MatIndex me{a.n_mat_e == 1 ? nullptr : a.mat_e, a.mat8_e};
+
+    int lz = 0, fz = -1;
+    if constexpr (ZC) fz = face_of(c, 2, k0, lz);
+    const int Lz = fz == 4 ? c.L[4] : (fz == 5 ? c.L[5] : 0);
+    const bool gj = j > 0;
+    const bool hk_last = k0 + V < nz;
+    const bool gk_first = k0 > 0;
+    const bool m_read = gj && j - 1 < a.j0;  // the row below is a slab row
+    const bool m_comp = gj && !m_read;
+    bool upd_all[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) upd_all[e] = true;
+
+    auto h_coef = [&](size_t p, Vec<V>& a0, Vec<V>& a1, Vec<V>& a2, Vec<V>& b0, Vec<V>& b1, Vec<V>& b2) {
+        if constexpr (TAB) {
+            coef_rows<V>(mh, p, tabh, a0, a1, a2, b0, b1, b2);
+        } else {
+            a0 = ldn<V, OE_NT_COEF>(a.Cf + 6 * N + p); a1 = ldn<V, OE_NT_COEF>(a.Cf + 7 * N + p);
+            a2 = ldn<V, OE_NT_COEF>(a.Cf + 8 * N + p); b0 = ldn<V, OE_NT_COEF>(a.Cf + 9 * N + p);
+            b1 = ldn<V, OE_NT_COEF>(a.Cf + 10 * N + p); b2 = ldn<V, OE_NT_COEF>(a.Cf + 11 * N + p);
+        }
+    };
+    // H^{n+1/2} of (i, j, k0..k0+3) from the old fields (h_body's
+    // arithmetic); ey_c/ez_c hold the old Ey/Ez of plane i, ey_n/ez_n those
+    // of plane i+1; ex_c returns the old Ex of (i, j).
+    auto h_full = [&](int i, size_t p, const Vec<V>& ey_c, const Vec<V>& ez_c, const Vec<V>& ey_n,
+                      const Vec<V>& ez_n, Vec<V>& ex_c, Vec<V>& hx, Vec<V>& hy, Vec<V>& hz, Psi<V>& pz) {
+        const bool hi = i + 1 < nx;
+        const bool hj = j + 1 < ny;
+        ex_c = ldv<V>(Ex + p);
+        Vec<V> ez_j = ez_c, ex_j = ex_c;
+        if (hj) {
+            ez_j = ldv<V>(Ez + p + nz);
+            ex_j = ldv<V>(Ex + p + nz);
+        }
+        float ey_kl = 0.f, ex_kl = 0.f;
+        if (hk_last) {
+            ey_kl = Ey[p + V];
+            ex_kl = Ex[p + V];
+        }
+        hx = ldn<V, OE_NT_OWN>(Hxo + p);
+        hy = ldn<V, OE_NT_OWN>(Hyo + p);
+        hz = ldn<V, OE_NT_OWN>(Hzo + p);
+        Vec<V> a0, a1, a2, b0, b1, b2;
+        h_coef(p, a0, a1, a2, b0, b1, b2);
+        if constexpr (ZC)
+            psi_load<V, true>(pz, c, 2, fz, lz, ((size_t)i * ny + j) * Lz + lz, c.ph_pos, c.ph_neg, c.phw_pos,
+                              c.phw_neg, TAB_BH, TAB_AH, TAB_KH);
+        float dEy_z[V], dEx_z[V];
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            const bool hk = (e < V - 1) || hk_last;
+            const float eyk = (e < V - 1) ? ey_c.v[(e < V - 1) ? e + 1 : e] : ey_kl;
+            const float exk = (e < V - 1) ? ex_c.v[(e < V - 1) ? e + 1 : e] : ex_kl;
+            const float dez_dy = hj ? ez_j.v[e] - ez_c.v[e] : 0.f;
+            const float dey_dz = hk ? eyk - ey_c.v[e] : 0.f;
+            const float cx = dez_dy - dey_dz;
+            hx.v[e] = a0.v[e] * hx.v[e] + b0.v[e] * cx;
+            const float dex_dz = hk ? exk - ex_c.v[e] : 0.f;
+            const float dez_dx = hi ? ez_n.v[e] - ez_c.v[e] : 0.f;
+            const float cy = dex_dz - dez_dx;
+            hy.v[e] = a1.v[e] * hy.v[e] + b1.v[e] * cy;
+            const float dey_dx = hi ? ey_n.v[e] - ey_c.v[e] : 0.f;
+            const float dex_dy = hj ? ex_j.v[e] - ex_c.v[e] : 0.f;
+            const float cz = dey_dx - dex_dy;
+            hz.v[e] = a2.v[e] * hz.v[e] + b2.v[e] * cz;
+            dEy_z[e] = dey_dz;
+            dEx_z[e] = dex_dz;
+        }
+        // z: psi_pos <- dEy/dz (Hx), psi_neg <- dEx/dz (Hy)
+        if constexpr (ZC) psi_apply<V, PHYS>(pz, dEy_z, dEx_z, hx, b0, hy, b1, upd_all);
+    };
+    // The new Hx, Hz of row j - 1 at plane i (the same arithmetic as its
+    // owner's h_full): eym_c/eym_n the old Ey of that row at planes i, i+1;
+    // ez_j/ex_j the old Ez/Ex of row j at plane i.
+    auto h_below = [&](int i, size_t pm, const Vec<V>& eym_c, const Vec<V>& eym_n, const Vec<V>& ez_j,
+                       const Vec<V>& ex_j, Vec<V>& hx, Vec<V>& hz) {
+        const bool hi = i + 1 < nx;
+        const Vec<V> exm = ldv<V>(Ex + pm);
+        const Vec<V> ezm = ldv<V>(Ez + pm);
+        const float ey_kl = hk_last ? Ey[pm + V] : 0.f;
+        hx = ldn<V, OE_NT_OWN>(Hxo + pm);
+        hz = ldn<V, OE_NT_OWN>(Hzo + pm);
+        Vec<V> a0, a1, a2, b0, b1, b2;
+        h_coef(pm, a0, a1, a2, b0, b1, b2);
+        Psi<V> pz;
+        if constexpr (ZC)
+            psi_load<V, true>(pz, c, 2, fz, lz, ((size_t)i * ny + (j - 1)) * Lz + lz, c.ph_pos, c.ph_neg,
+                              c.phw_pos, c.phw_neg, TAB_BH, TAB_AH, TAB_KH);
+        float dEy_z[V];
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            const bool hk = (e < V - 1) || hk_last;
+            const float eyk = (e < V - 1) ? eym_c.v[(e < V - 1) ? e + 1 : e] : ey_kl;
+            const float dez_dy = ez_j.v[e] - ezm.v[e];  // row j - 1 + 1 = j < ny
+            const float dey_dz = hk ? eyk - eym_c.v[e] : 0.f;
+            const float cx = dez_dy - dey_dz;
+            hx.v[e] = a0.v[e] * hx.v[e] + b0.v[e] * cx;
+            const float dey_dx = hi ? eym_n.v[e] - eym_c.v[e] : 0.f;
+            const float dex_dy = ex_j.v[e] - exm.v[e];
+            const float cz = dey_dx - dex_dy;
+            hz.v[e] = a2.v[e] * hz.v[e] + b2.v[e] * cz;
+            dEy_z[e] = dey_dz;
+        }
+        if constexpr (ZC) {
+            // the Hx correction only (the Hy of this row is not needed)
+            Vec<V> hy_unused = hx;
+            psi_apply<V, PHYS>(pz, dEy_z, dEy_z, hx, b0, hy_unused, b1, upd_all);
+        }
+    };
+
+    const size_t p0 = (size_t)ia * plane + (size_t)j * nz + k0;
+    Vec<V> ey_c = ldv<V>(Ey + p0), ez_c = ldv<V>(Ez + p0);
+    Vec<V> eym_c = zerov<V>();
+    if (m_comp) eym_c = ldv<V>(Ey + p0 - nz);
+    // H^{n+1/2} of plane ia - 1 (its Hy, Hz enter the x-differences of plane ia)
+    Vec<V> hy_p = zerov<V>(), hz_p = zerov<V>();
+    if (ia > 0) {
+        const size_t pp = p0 - plane;
+        if (ia - 1 < a.i0) {
+            hy_p = ldv<V>(Hyn + pp);
+            hz_p = ldv<V>(Hzn + pp);
+        } else {
+            Vec<V> ex_t, hx_t;
+            Psi<V> pz_t;
+            h_full(ia - 1, pp, ldv<V>(Ey + pp), ldv<V>(Ez + pp), ey_c, ez_c, ex_t, hx_t, hy_p, hz_p, pz_t);
+        }
+    }
+    MatIdx mi_c;
+    if constexpr (TAB) {
+        if (gj && ia > 0) mi_c = coef_index<V>(me, p0);
+    }
+
+    for (int i = ia; i < ie; ++i) {
+        const size_t p = p0 + (size_t)(i - ia) * plane;
+        const bool hi = i + 1 < nx;
+        Vec<V> ey_n = ey_c, ez_n = ez_c, eym_n = eym_c;
+        if (hi) {
+            ey_n = ldv<V>(Ey + p + plane);
+            ez_n = ldv<V>(Ez + p + plane);
+            if (m_comp) eym_n = ldv<V>(Ey + p + plane - nz);
+        }
+        MatIdx mi_n;
+        if constexpr (TAB) {
+            if (gj && i + 1 < ie) mi_n = coef_index<V>(me, p + plane);
+        }
+        Vec<V> ex_c, hx, hy, hz;
+        Psi<V> pz;
+        h_full(i, p, ey_c, ez_c, ey_n, ez_n, ex_c, hx, hy, hz, pz);
+        stn<V, OE_NT_STORE>(Hxn + p, hx);
+        stn<V, OE_NT_STORE>(Hyn + p, hy);
+        stn<V, OE_NT_STORE>(Hzn + p, hz);
+        if constexpr (ZC) psi_store<V>(pz);
+        Vec<V> hx_j = hx, hz_j = hz;  // the new Hx, Hz of row j - 1
+        if (m_comp) {
+            h_below(i, p - nz, eym_c, eym_n, ez_c, ex_c, hx_j, hz_j);
+        } else if (m_read) {
+            hx_j = ldv<V>(Hxn + p - nz);
+            hz_j = ldv<V>(Hzn + p - nz);
+        }
+        const float hy_kf = __shfl_up(hy.v[V - 1], 1, 64);
+        const float hx_kf = __shfl_up(hx.v[V - 1], 1, 64);
+
+        // E^{n+1} (e_body's arithmetic)
+        const bool gi = i > 0;
+        const bool row = gi && gj;
+        Vec<V> s0, s1, s2, c0, c1, c2;
+        if constexpr (TAB) {
+            coef_from<V>(me, mi_c, tabe, s0, s1, s2, c0, c1, c2);
+        } else {
+            s0 = ldn<V, OE_NT_COEF>(a.Cf + p); s1 = ldn<V, OE_NT_COEF>(a.Cf + N + p);
+            s2 = ldn<V, OE_NT_COEF>(a.Cf + 2 * N + p); c0 = ldn<V, OE_NT_COEF>(a.Cf + 3 * N + p);
+            c1 = ldn<V, OE_NT_COEF>(a.Cf + 4 * N + p); c2 = ldn<V, OE_NT_COEF>(a.Cf + 5 * N + p);
+        }
+        Psi<V> pe;
+        if constexpr (ZC)
+            psi_load<V, true>(pe, c, 2, fz, lz, ((size_t)i * ny + j) * Lz + lz, c.pe_pos, c.pe_neg, c.pe_pos, c.pe_neg,
+                              TAB_BE, TAB_AE, TAB_KE);
+        Vec<V> ex = ex_c, ey = ey_c, ez = ez_c;
+        float dHy_z[V], dHx_z[V];
+        bool upd[V];
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            const bool gk = (e > 0) || gk_first;
+            upd[e] = row && gk;
+            const float hyk = (e > 0) ? hy.v[(e > 0) ? e - 1 : e] : hy_kf;
+            const float hxk = (e > 0) ? hx.v[(e > 0) ? e - 1 : e] : hx_kf;
+            const float dHz_y = gj ? hz.v[e] - hz_j.v[e] : 0.f;
+            dHy_z[e] = gk ? hy.v[e] - hyk : 0.f;
+            dHx_z[e] = gk ? hx.v[e] - hxk : 0.f;
+            const float dHz_x = gi ? hz.v[e] - hz_p.v[e] : 0.f;
+            const float dHy_x = gi ? hy.v[e] - hy_p.v[e] : 0.f;
+            const float dHx_y = gj ? hx.v[e] - hx_j.v[e] : 0.f;
+            if (upd[e]) {
+                const float cx = dHz_y - dHy_z[e];
+                ex.v[e] = s0.v[e] * ex.v[e] + c0.v[e] * cx;
+                const float cy = dHx_z[e] - dHz_x;
+                ey.v[e] = s1.v[e] * ey.v[e] + c1.v[e] * cy;
+                const float cz = dHy_x - dHx_y;
+                ez.v[e] = s2.v[e] * ez.v[e] + c2.v[e] * cz;
+            }
+        }
+        // z: psi_pos <- dHy/dz (Ex), psi_neg <- dHx/dz (Ey)
+        if constexpr (ZC) {
+            psi_apply<V, PHYS>(pe, dHy_z, dHx_z, ex, c0, ey, c1, upd);
+            psi_store<V>(pe);
+        }
+        stn<V, OE_NT_STORE>(Exn + p, ex);
+        stn<V, OE_NT_STORE>(Eyn + p, ey);
+        stn<V, OE_NT_STORE>(Ezn + p, ez);
+        hy_p = hy;
+        hz_p = hz;
+        ey_c = ey_n;
+        ez_c = ez_n;
+        eym_c = eym_n;
+        mi_c = mi_n;
+    }
+}
+
+template <int AX, bool PHYS, bool TAB>
+hipError_t launch_fused_rows_t(const FusedArgs& a, const CpmlArgs& c, hipStream_t s) {
+    const size_t lds = TAB ? 3 * (size_t)(a.n_mat_h + a.n_mat_e) * sizeof(float2) : 0;
+    hipLaunchKernelGGL((k_fused_rows<AX, PHYS, TAB>), dim3(a.n_jb * a.n_ib), dim3(FUSED_ROWS * 64), lds, s, a, c);
+    return hipGetLastError();
+}
+
 template <int AX, bool PHYS, bool TAB>
 hipError_t launch_fused_t(const FusedArgs& a, const CpmlArgs& c, hipStream_t s) {
     const size_t lds = 2 * FUSED_ROWS * 2 * 64 * sizeof(float4) +
@@ -1116,6 +1397,13 @@ hipError_t launch_fused(const FusedArgs& a, const CpmlArgs* c, hipStream_t s) {
     if (a.n_jb * a.n_ib <= 0) return hipSuccess;
     const bool tab = a.n_mat_h > 0 && a.n_mat_e > 0;
     const CpmlArgs none{};
+    if (a.kind == 2) {
+        if (!c)
+            return tab ? launch_fused_rows_t<0, false, true>(a, none, s) : launch_fused_rows_t<0, false, false>(a, none, s);
+        if (c->physical)
+            return tab ? launch_fused_rows_t<AX_Z, true, true>(a, *c, s) : launch_fused_rows_t<AX_Z, true, false>(a, *c, s);
+        return tab ? launch_fused_rows_t<AX_Z, false, true>(a, *c, s) : launch_fused_rows_t<AX_Z, false, false>(a, *c, s);
+    }
     if (!c) return tab ? launch_fused_t<0, false, true>(a, none, s) : launch_fused_t<0, false, false>(a, none, s);
     if (c->physical)
         return tab ? launch_fused_t<AX_Z, true, true>(a, *c, s) : launch_fused_t<AX_Z, true, false>(a, *c, s);

@@ -16,6 +16,15 @@ from tests.parity_util import Scenario, assert_parity, gauss_ref, run_pair, say
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(autouse=True, params=["1", "2"], ids=["lds", "rows"])
+def fused_kind(request, monkeypatch):
+    """Both fused kernels (OE_FUSED_KIND): rows exchanged through LDS with a
+    barrier per plane (1), and each wave recomputing the H of the row below
+    (2, the default)."""
+    monkeypatch.setenv("OE_FUSED_KIND", request.param)
+    return request.param
+
 BOXES = [(3.0, 1.0, 0.01, 0.0, (10, 30, 12, 20, 15, 60)), (6.0, 2.0, 0.0, 0.0, (5, 15, 3, 30, 40, 100))]
 PEC = [(20, 22, 8, 28, 70, 72)]
 
