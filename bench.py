@@ -859,7 +859,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--prof-period", type=int, default=0,
                     help="HIP events around every launch of every n-th timed step (1 = all; 0: every step up to "
-                         "64 timed steps, then at most every 8th)")
+                         "64 timed steps, then at most every 64th, so that the steps between replay captured graphs)")
     ap.add_argument("--no-subrecords", action="store_true",
                     help="skip the 512^3 perf point and the sweep that the default line carries beside the headline")
     ap.add_argument("--perf-point-steps", type=int, default=200)
@@ -931,7 +931,9 @@ def main():
 
     wl = W.plain_cavity(steps=args.steps) if args.cavity else W.cpml_boxes(n=args.n, steps=args.steps)
     mode = MODE_PHYSICAL if args.mode == "physical" else MODE_COMPAT
-    period = args.prof_period if args.prof_period > 0 else max(1, min(8, args.steps // 64))
+    # launches of every period-th step are timed with events; the steps in
+    # between run from the engine's captured graphs (16 steps each)
+    period = args.prof_period if args.prof_period > 0 else max(1, min(64, args.steps // 64))
     pt = yee_point(wl, mode, args.steps, args.warmup, period, args.tiles, args.coef, args.fused, args.mode)
 
     cpu = None
