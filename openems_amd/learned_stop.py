@@ -26,8 +26,11 @@ are synthesised for S11.
 """
 from __future__ import annotations
 
+import contextlib
 import math
+import os
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field, replace
 from typing import List, Optional
 
@@ -115,7 +118,19 @@ def refit_residues(x: np.ndarray, s: np.ndarray, k0: int, k1: int, ref: int, dec
     sum_k (x[k] - Re sum_m a_m exp(s_m (k - ref) / decimate))^2 over
     k = k0, k0 + stride, ... < k1 (s per decimated sample).  Columns are
     scaled to unit norm before the solve (poles of very different damping
-    make them span many decades); returns (a, relative residual)."""
+    make them span many decades); returns (a, relative residual).  x may
+    also stack several signals [c, n] that share the poles (port V and I):
+    one factorisation then serves all of them, and a is [c, M], the
+    residual [c]."""
+    x = np.asarray(x)
+    if x.ndim == 2:
+        a, r = _refit(x, s, k0, k1, ref, decimate, stride)
+        return a, r
+    a, r = _refit(x[None], s, k0, k1, ref, decimate, stride)
+    return a[0], float(r[0])
+
+
+def _refit(x: np.ndarray, s: np.ndarray, k0: int, k1: int, ref: int, decimate: int, stride: int):
     k = np.arange(k0, k1, stride)
     t = (k - ref) / decimate
     # each column divided by its largest modulus first (exp of the largest
@@ -125,16 +140,16 @@ def refit_residues(x: np.ndarray, s: np.ndarray, k0: int, k1: int, ref: int, dec
     D = np.concatenate([A.real, -A.imag], axis=1)               # x = D [Re a; Im a]
     sc = np.linalg.norm(D, axis=0)
     sc[sc == 0] = 1.0
-    y = x[k0:k1:stride].astype(np.float64)
+    y = x[:, k0:k1:stride].astype(np.float64).T                  # [K, c]
     sol, *_ = np.linalg.lstsq(D / sc, y, rcond=1e-12)
-    sol = sol / sc
+    sol = sol / sc[:, None]
     fit = D @ sol
     M = len(s)
     with np.errstate(over="ignore", under="ignore"):
         g = np.exp(-shift)
-    a = (sol[:M] + 1j * sol[M:]) * g
+    a = (sol[:M].T + 1j * sol[M:].T) * g                        # [c, M]
     a[~np.isfinite(a)] = 0.0
-    r = float(np.sum((fit - y) ** 2) / max(np.sum(y * y), 1e-300))
+    r = np.sum((fit - y) ** 2, axis=0) / np.maximum(np.sum(y * y, axis=0), 1e-300)
     return a, r
 
 
@@ -219,11 +234,23 @@ def _passes(value: float, bar: float) -> bool:
     return bar == np.inf or (np.isfinite(value) and value <= bar)
 
 
+def _one_thread_blas():
+    """BLAS limited to one thread per caller while host threads each solve
+    small least-squares problems (threads inside a 512 x 64 solve only slow
+    it down)."""
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:
+        return contextlib.nullcontext()
+    return threadpool_limits(1)
+
+
 def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, decimate: int, check: int,
                   max_residual: float = 1e-6, freqs=None, decay_db: float = 50.0, stride: Optional[int] = None,
                   decay_check: int = 100, max_tail_energy: float = 1e-4, horizon_window: int = 300,
                   max_horizon_change: float = 3e-4, poles: str = "network", pencil_order: int = 24,
-                  min_decay_db: float = 20.0, fit_stride: int = 4, log=None) -> LearnedResult:
+                  min_decay_db: float = 20.0, fit_stride: int = 4, workers: Optional[int] = None,
+                  log=None) -> LearnedResult:
     """All structures run together from checkpoint to checkpoint (the first
     at window + check steps, then every `stride`).  At a checkpoint P the
     network maps the decimated V and I of [P - check - window, P - check) to
@@ -248,8 +275,10 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
     structures whose energy has already fallen min_decay_db below its peak
     are evaluated at a checkpoint (a tail of at most max_tail_energy of the
     signal energy needs most of the decay behind it), and the least-squares
-    fits use every fit_stride-th step."""
+    fits use every fit_stride-th step.  The structures of a checkpoint are
+    judged on `workers` host threads (default min(8, cpus))."""
     d = model.dims
+    workers = workers or min(8, os.cpu_count() or 1)
     early = window + check
     stride = stride or window
     fr = np.linspace(1e9, 20e9, 64) if freqs is None else np.asarray(freqs)
@@ -318,58 +347,68 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
                 m, _ = model.forward(X[b0:b0 + d.batch], tails=False)
                 modes[b0:b0 + d.batch] = m
             s_all, _, _ = mode_params(modes, d.modes)
-        rejected = []
         a0, a1 = pos - early, pos - check
-        for k, q in enumerate(still):
+
+        def judge(k: int):
+            """(accepted, S11, residual, tail energy fraction) of structure
+            still[k] at this checkpoint: gates 1-3 of the docstring."""
+            q = still[k]
             if poles == "network":
                 s = s_all[k]
             else:
                 try:
                     s = pencil_poles(sers[k][:, 0], sers[k][:, 1], a0, a1, decimate, pencil_order)
                 except np.linalg.LinAlgError:
-                    rejected.append(q)
-                    continue
+                    return False, None, np.nan, np.nan
                 if len(s) == 0:
-                    rejected.append(q)
-                    continue
+                    return False, None, np.nan, np.nan
             u, i = sers[k][:, 0], sers[k][:, 1]
-            aU, _ = refit_residues(u, s, a0, a1, a1, decimate, fit_stride)
-            aI, _ = refit_residues(i, s, a0, a1, a1, decimate, fit_stride)
+            aUI, _ = refit_residues(np.stack([u, i]), s, a0, a1, a1, decimate, fit_stride)
             kk = np.arange(a1, pos)
             eu = float(np.sum(u[a0:a1] ** 2))
             ei = float(np.sum(i[a0:a1] ** 2))
-            ru = float(np.sum((modes_eval(s, aU, kk, a1, decimate) - u[a1:pos]) ** 2)) / max(eu, 1e-300)
-            ri = float(np.sum((modes_eval(s, aI, kk, a1, decimate) - i[a1:pos]) ** 2)) / max(ei, 1e-300)
-            resid[q] = max(ru, ri)
-            ok = _passes(resid[q], max_residual)
+            ru = float(np.sum((modes_eval(s, aUI[0], kk, a1, decimate) - u[a1:pos]) ** 2)) / max(eu, 1e-300)
+            ri = float(np.sum((modes_eval(s, aUI[1], kk, a1, decimate) - i[a1:pos]) ** 2)) / max(ei, 1e-300)
+            res_q = max(ru, ri)
+            if not _passes(res_q, max_residual):
+                return False, None, res_q, np.nan
+            aUI, _ = refit_residues(np.stack([u, i]), s, pos - window, pos, pos, decimate, fit_stride)
+            aU, aI = aUI[0], aUI[1]
+            stop = predict_stop(np.concatenate(hist_t[q]), np.concatenate(hist_e[q]), pos, max_steps, decay_db,
+                                decay_check)
+            sv = max(float(np.abs(u[a0:pos]).max()), 1e-300)
+            si = max(float(np.abs(i[a0:pos]).max()), 1e-300)
+            kt = np.arange(pos, stop, decimate)
+            e_tail = float(np.sum((modes_eval(s, aU, kt, pos, decimate) / sv) ** 2)
+                           + np.sum((modes_eval(s, aI, kt, pos, decimate) / si) ** 2))
+            e_run = float(np.sum((u[:pos:decimate] / sv) ** 2) + np.sum((i[:pos:decimate] / si) ** 2))
+            frac = e_tail / max(e_run, 1e-300)
+            if not _passes(frac, max_tail_energy):
+                return False, None, res_q, frac
+
+            def s11_to(end):
+                U = dfts[k][:, 0] + tail_dft(s, aU, pos, pos, end, decimate, dt, 1.0, fr)
+                I = dfts[k][:, 1] + tail_dft(s, aI, pos, pos, end, decimate, dt, 0.5, fr)
+                return calc_s11(U, I)
+
+            s_pred = s11_to(stop)
+            change = max(float(np.abs(s11_to(max(pos, stop - horizon_window)) - s_pred).max()),
+                         float(np.abs(s11_to(min(max_steps, stop + horizon_window)) - s_pred).max()))
+            if not _passes(change, max_horizon_change):
+                return False, None, res_q, frac
+            return True, s_pred, res_q, frac
+
+        # the structures are independent: judged on host threads (LAPACK
+        # releases the interpreter lock), each with a one-thread BLAS
+        with _one_thread_blas(), ThreadPoolExecutor(max_workers=workers) as ex:
+            verdicts = list(ex.map(judge, range(len(still))))
+        rejected = []
+        for k, q in enumerate(still):
+            ok, s_q, resid[q], tail_frac[q] = verdicts[k]
             if ok:
-                aU, _ = refit_residues(u, s, pos - window, pos, pos, decimate, fit_stride)
-                aI, _ = refit_residues(i, s, pos - window, pos, pos, decimate, fit_stride)
-                stop = predict_stop(np.concatenate(hist_t[q]), np.concatenate(hist_e[q]), pos, max_steps, decay_db,
-                                    decay_check)
-                sv = max(float(np.abs(u[a0:pos]).max()), 1e-300)
-                si = max(float(np.abs(i[a0:pos]).max()), 1e-300)
-                kt = np.arange(pos, stop, decimate)
-                e_tail = float(np.sum((modes_eval(s, aU, kt, pos, decimate) / sv) ** 2)
-                               + np.sum((modes_eval(s, aI, kt, pos, decimate) / si) ** 2))
-                e_run = float(np.sum((u[:pos:decimate] / sv) ** 2) + np.sum((i[:pos:decimate] / si) ** 2))
-                tail_frac[q] = e_tail / max(e_run, 1e-300)
-                ok = _passes(tail_frac[q], max_tail_energy)
-
-                def s11_to(end):
-                    U = dfts[k][:, 0] + tail_dft(s, aU, pos, pos, end, decimate, dt, 1.0, fr)
-                    I = dfts[k][:, 1] + tail_dft(s, aI, pos, pos, end, decimate, dt, 0.5, fr)
-                    return calc_s11(U, I)
-
-                if ok:
-                    s_pred = s11_to(stop)
-                    change = max(float(np.abs(s11_to(max(pos, stop - horizon_window)) - s_pred).max()),
-                                 float(np.abs(s11_to(min(max_steps, stop + horizon_window)) - s_pred).max()))
-                    ok = _passes(change, max_horizon_change)
-                if ok:
-                    accepted[q] = True
-                    s11[q] = s_pred
-            if not ok:
+                accepted[q] = True
+                s11[q] = s_q
+            else:
                 rejected.append(q)
         t_inf += time.perf_counter() - t1
         live = rejected + waiting

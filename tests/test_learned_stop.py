@@ -73,6 +73,25 @@ def test_refit_residues_recovers_modes_and_ignores_huge_damping():
     assert np.abs(modes_eval(s, got[:3], k[2000:], 2000, 8) - x[2000:]).max() < 1e-10
 
 
+def test_refit_of_stacked_signals_equals_one_at_a_time():
+    """V and I share the poles, so one factorisation serves both: the stacked
+    refit gives each signal's residues and residual as its own refit does."""
+    from openems_amd.learned_stop import modes_eval, refit_residues
+    rng = np.random.default_rng(2)
+    s = np.array([-0.01 + 2j * math.pi * 0.05, -0.003 + 2j * math.pi * 0.11, -0.02 + 2j * math.pi * 0.3])
+    k = np.arange(4000)
+    sig = [modes_eval(s, rng.standard_normal(3) + 1j * rng.standard_normal(3), k, 3000, 8)
+           + 1e-3 * rng.standard_normal(len(k)) for _ in range(2)]
+    a2, r2 = refit_residues(np.stack(sig), s, 1000, 3000, 3000, 8, 4)
+    assert a2.shape == (2, 3) and r2.shape == (2,)
+    for c in range(2):
+        a1, r1 = refit_residues(sig[c], s, 1000, 3000, 3000, 8, 4)
+        assert isinstance(r1, float)
+        assert np.abs(a2[c] - a1).max() <= 1e-12 * np.abs(a1).max()
+        assert abs(r2[c] - r1) <= 1e-12 * r1
+        assert 0 < r1 < 1e-4  # the noise floor, not a failed fit
+
+
 def test_predict_stop_on_exponential_decay():
     from openems_amd.learned_stop import predict_stop
     # energy exp(-t / 400) from a peak of 1: -50 dB at t = 400 * 5 ln 10 = 4605.2,
