@@ -270,12 +270,16 @@ __global__ void k_ade_post_multi(float* __restrict__ F, const float* __restrict_
     if (c < a.n) ade_post_cell(F, Cf, N, a, c);
 }
 
-// One thread per signal: gather the float64 signal, record it, and update
-// the running sums of every frequency.  FP32_KAHAN keeps float32 sums with
-// Kahan compensation; the phase is evaluated in float64 in both modes.
-// Each work-group first tabulates exp(-j 2 pi f t) for the E and H sample
-// times of a chunk of frequencies in LDS, so a thread's inner loop over
-// frequencies is two loads and the accumulator update.
+// A work-group owns 256 signals: one thread per signal gathers the float64
+// signal value and records it; then the work-group's threads share the
+// (frequency, signal) pairs of its signals, so a few signals with many
+// frequencies (a port's V and I) update their sums in parallel, and many
+// signals (a recording surface) still update each frequency's sums in
+// coalesced rows.  Every sum receives the same one update as with a thread
+// per signal.  FP32_KAHAN keeps float32 sums with Kahan compensation; the
+// phase is evaluated in float64 in both modes.  The work-group tabulates
+// exp(-j 2 pi f t) for the E and H sample times of a chunk of frequencies in
+// LDS first.
 constexpr int DFT_FCHUNK = 128;
 
 // With step_dev the step comes from the device counter (a captured graph):
@@ -285,6 +289,8 @@ __global__ __launch_bounds__(256) void k_dft(const float* __restrict__ F, size_t
                                             double tE, double tH, const uint64_t* __restrict__ step_dev, double t0,
                                             double dt) {
     __shared__ double ph[2][DFT_FCHUNK][2];  // [E, H][f][cos, -sin]
+    __shared__ double sv[256];
+    __shared__ uint8_t hv[256];
     if (step_dev) {
         const uint64_t st = *step_dev;
         sample += st;
@@ -292,19 +298,21 @@ __global__ __launch_bounds__(256) void k_dft(const float* __restrict__ F, size_t
         tE = (cur + 1.0) * dt;
         tH = (cur + 0.5) * dt;
     }
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = m < d.n_sig;
-    double s = 0.0;
-    int h = 0;
-    if (act) {
-        for (uint64_t t = d.off[m]; t < d.off[m + 1]; ++t) s += d.w[t] * (double)F[(size_t)d.comp[t] * N + d.cell[t]];
+    const int m0 = blockIdx.x * 256;
+    const int nm = min(256, d.n_sig - m0);
+    const int t = threadIdx.x;
+    if (t < nm) {
+        const int m = m0 + t;
+        double s = 0.0;
+        for (uint64_t k = d.off[m]; k < d.off[m + 1]; ++k) s += d.w[k] * (double)F[(size_t)d.comp[k] * N + d.cell[k]];
         if (d.series && sample < d.series_cap) d.series[sample * d.n_sig + m] = s;
-        h = d.is_h[m] ? 1 : 0;
+        sv[t] = s;
+        hv[t] = d.is_h[m] ? 1 : 0;
     }
     for (int f0 = 0; f0 < d.n_f; f0 += DFT_FCHUNK) {
         const int nf = min(DFT_FCHUNK, d.n_f - f0);
-        __syncthreads();
-        for (int q = threadIdx.x; q < 2 * nf; q += blockDim.x) {
+        __syncthreads();  // sv / hv written; the previous chunk's phasors read
+        for (int q = t; q < 2 * nf; q += blockDim.x) {
             const int hq = q / nf, f = q - hq * nf;
             double sn, cs;
             sincos(2.0 * M_PI * d.freqs[f0 + f] * (hq ? tH : tE), &sn, &cs);
@@ -312,24 +320,25 @@ __global__ __launch_bounds__(256) void k_dft(const float* __restrict__ F, size_t
             ph[hq][f][1] = -sn;
         }
         __syncthreads();
-        if (!act) continue;
-        for (int f = 0; f < nf; ++f) {
-            const double cr = ph[h][f][0], ci = ph[h][f][1];
-            const size_t q = (size_t)(f0 + f) * d.n_sig + m;
+        for (int q = t; q < nf * nm; q += blockDim.x) {
+            const int f = q / nm, g = q - f * nm;
+            const double s = sv[g];
+            const double cr = ph[hv[g]][f][0], ci = ph[hv[g]][f][1];
+            const size_t a = (size_t)(f0 + f) * d.n_sig + m0 + g;
             if (d.prec == 0) {
-                d.acc64[2 * q] += s * cr;
-                d.acc64[2 * q + 1] += s * ci;
+                d.acc64[2 * a] += s * cr;
+                d.acc64[2 * a + 1] += s * ci;
             } else {
-                const float sv = (float)s;
-                float* a = d.acc32 + 4 * q;
-                const float yr = sv * (float)cr - a[2];
-                const float tr = a[0] + yr;
-                a[2] = (tr - a[0]) - yr;
-                a[0] = tr;
-                const float yi = sv * (float)ci - a[3];
-                const float ti = a[1] + yi;
-                a[3] = (ti - a[1]) - yi;
-                a[1] = ti;
+                const float svf = (float)s;
+                float* acc = d.acc32 + 4 * a;
+                const float yr = svf * (float)cr - acc[2];
+                const float tr = acc[0] + yr;
+                acc[2] = (tr - acc[0]) - yr;
+                acc[0] = tr;
+                const float yi = svf * (float)ci - acc[3];
+                const float ti = acc[1] + yi;
+                acc[3] = (ti - acc[1]) - yi;
+                acc[1] = ti;
             }
         }
     }
@@ -624,6 +633,7 @@ hipError_t launch_dft(const float* F, size_t N, const DftDev& d, uint64_t sample
 
 hipError_t launch_dft_dev(const float* F, size_t N, const DftDev& d, const uint64_t* step_dev, uint64_t sample0,
                           double t0, double dt, hipStream_t s) {
+    if (d.n_sig <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_dft, dim3((d.n_sig + 255) / 256), dim3(256), 0, s, F, N, d, sample0, 0.0, 0.0, step_dev, t0,
                        dt);
     return hipGetLastError();
