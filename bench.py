@@ -165,6 +165,7 @@ def summarize_rocprof(paths):
                 out.append("")
         if "kernels" in views:
             out.append(busy_summary(con.execute("select start, end from kernels order by start").fetchall()))
+            out.append(grid_split(con))
         if "counters_collection" in views:
             rows = con.execute("select kernel_name, counter_name, count(*), avg(value), min(value), max(value) "
                                "from counters_collection group by kernel_name, counter_name "
@@ -178,6 +179,27 @@ def summarize_rocprof(paths):
             if {"SQ_VALU_MFMA_BUSY_CYCLES", "GRBM_GUI_ACTIVE"} <= names:
                 out.append(mfma_summary(con))
     return "\n".join(out)
+
+
+def grid_split(con) -> str:
+    """Kernels launched with more than one grid size (one kernel serving
+    several extensions, e.g. a port's DFT and a recording surface's), one
+    row per grid size (total work-items along x, y, z)."""
+    cols = [r[1] for r in con.execute("pragma table_info(kernels)")]
+    grid = [c for c in cols if c.startswith("grid")][:3]
+    if "name" not in cols or not grid:
+        return ""
+    rows = con.execute(f"select name, {', '.join(grid)}, count(*), sum(end - start), avg(end - start) from kernels "
+                       f"group by name, {', '.join(grid)}").fetchall()
+    names = [r[0] for r in rows]
+    rows = [r for r in rows if names.count(r[0]) > 1]
+    if not rows:
+        return ""
+    out = ["| kernel | grid | calls | total us | mean us |", "|---|---|---|---|---|"]
+    for r in sorted(rows, key=lambda r: (r[0], -r[-2])):
+        g = "x".join(str(v) for v in r[1:1 + len(grid)])
+        out.append(f"| `{r[0][:90]}` | {g} | {r[-3]} | {r[-2] / 1e3:.1f} | {r[-1] / 1e3:.2f} |")
+    return "\n".join(out) + "\n"
 
 
 def busy_summary(intervals, host_gap_ns: float = 1e6):
