@@ -114,11 +114,23 @@ def kernel_bytes(shape, layers, coef="arrays"):
 
 
 FUSED_CELL_BYTES = {
-    # per cell and fused step: E and H read and written, and the coefficients
-    # of both halves
+    # per cell and fused step: E and H read and written (48 B), and the
+    # coefficients of both halves
     "arrays": 96,   # 12 coefficient floats
-    "table": 52,    # two 2-byte row indices
+    "table": 48,    # plus the row indices (fused_index_bytes)
 }
+
+
+def fused_index_bytes(rows):
+    """Row-index bytes per cell of the fused pass: none for a half of one
+    row; the E index as a byte for 2 to 256 rows when the default kernel
+    (OE_FUSED_KIND 2) reads the byte-wide copy (OE_BYTE_INDEX), else 2 B;
+    the H index always 2 B."""
+    e_rows, h_rows = rows
+    byte_e = (os.environ.get("OE_BYTE_INDEX", "1") != "0" and os.environ.get("OE_FUSED_KIND", "2") != "1"
+              and 2 <= e_rows <= 256)
+    e = 0 if e_rows == 1 else (1 if byte_e else 2)
+    return e + (0 if h_rows == 1 else 2)
 
 
 def launch_bytes(name, shape, layers, coef="arrays", rows=(0, 0)):
@@ -134,7 +146,7 @@ def launch_bytes(name, shape, layers, coef="arrays", rows=(0, 0)):
     if kind == "fused":
         ix, jy = (nx - 2 * layers, ny - 2 * layers) if part == "interior" else (nx, ny)
         zl = 2 * layers if layers and 2 * layers < nz else 0
-        fb = FUSED_CELL_BYTES[coef] - (2 * sum(r == 1 for r in rows) if coef == "table" else 0)
+        fb = FUSED_CELL_BYTES[coef] + (fused_index_bytes(rows) if coef == "table" else 0)
         return fb * ix * jy * nz + 32 * ix * jy * zl
     coef = half_coef(coef, rows, kind == "update_h")
     if part == "grid" or not layers:

@@ -47,9 +47,50 @@ def test_interior_plus_slabs_is_the_whole_half_step(rows, half):
     assert whole == per_cell * 256 ** 3 + 16 * 3 * 16 * 256 * 256
 
 
-def test_fused_interior_bytes():
-    # 48 B of fields per cell and step, the E index (2 B: the fused pass reads
-    # the 16-bit form), no H index, 32 B of psi per z-slab cell; 766,771,200
-    # is what bench.py --fused reported for this launch
+def test_fused_interior_bytes(monkeypatch):
+    # 48 B of fields per cell and step, the E index as a byte (the default
+    # fused kernel reads the byte-wide copy for 2 to 256 rows), no H index,
+    # 32 B of psi per z-slab cell
+    monkeypatch.delenv("OE_FUSED_KIND", raising=False)
+    monkeypatch.delenv("OE_BYTE_INDEX", raising=False)
+    assert bench.launch_bytes("fused:interior", SHAPE, L, "table", (201, 1)) == 49 * INTERIOR + 32 * ZSLAB
+    assert 49 * INTERIOR + 32 * ZSLAB == 752_025_600
+    # the LDS-exchange kernel reads the 16-bit index (766,771,200 B, what
+    # bench.py --fused reported while it was the fused default)
+    monkeypatch.setenv("OE_FUSED_KIND", "1")
+    assert bench.launch_bytes("fused:interior", SHAPE, L, "table", (201, 1)) == 50 * INTERIOR + 32 * ZSLAB == 766_771_200
+    monkeypatch.setenv("OE_FUSED_KIND", "2")
+    monkeypatch.setenv("OE_BYTE_INDEX", "0")
     assert bench.launch_bytes("fused:interior", SHAPE, L, "table", (201, 1)) == 50 * INTERIOR + 32 * ZSLAB
-    assert 50 * INTERIOR + 32 * ZSLAB == 766_771_200
+    # more than 256 rows: 2 B whatever the kernel; an H table of several rows
+    # adds its 2-byte index
+    monkeypatch.delenv("OE_BYTE_INDEX")
+    assert bench.launch_bytes("fused:interior", SHAPE, L, "table", (300, 1)) == 50 * INTERIOR + 32 * ZSLAB
+    assert bench.launch_bytes("fused:interior", SHAPE, L, "table", (201, 5)) == 51 * INTERIOR + 32 * ZSLAB
+    assert bench.launch_bytes("fused:interior", SHAPE, L, "table", (1, 1)) == 48 * INTERIOR + 32 * ZSLAB
+    assert bench.launch_bytes("fused:interior", SHAPE, L, "arrays", (0, 0)) == 96 * INTERIOR + 32 * ZSLAB
+
+
+def test_rocprof_summary_of_a_trace():
+    """busy_summary and grid_split on a synthetic kernel trace: the union of
+    the dispatch intervals, the gaps between them, and one row per grid size
+    for a kernel launched with two."""
+    import sqlite3
+    con = sqlite3.connect(":memory:")
+    con.execute("create table kernels(id, name, grid_size_x, grid_size_y, grid_size_z, start, end)")
+    con.executemany("insert into kernels values (?,?,?,?,?,?,?)", [
+        (1, "k_dft", 256, 1, 1, 0, 5_000),
+        (2, "k_dft", 2_208_512, 1, 1, 10_000, 152_000),       # 5 us gap before
+        (3, "k_update", 4_096, 1, 1, 100_000, 160_000),       # overlaps the last
+        (4, "k_dft", 256, 1, 1, 2_160_000, 2_165_000),        # a 2 ms host phase before
+    ])
+    busy = bench.busy_summary(con.execute("select start, end from kernels order by start").fetchall())
+    # busy: 5 + 150 + 5 us; span 2165 us; one 2 ms gap, one 5 us gap
+    assert "busy 0.000 s" in busy and "Dispatches 4" in busy
+    assert "gaps over 1 ms 0.002 s" in busy and "1 shorter gaps, 0.000 s" in busy
+    assert "busy / (span - those gaps) 0.970" in busy  # 160 / (2165 - 2000)
+    table = bench.grid_split(con)
+    rows = [r for r in table.splitlines() if r.startswith("| `")]
+    assert len(rows) == 2  # k_update has one grid size and is left out
+    assert "| 2208512x1x1 | 1 | 142.0 | 142.00 |" in rows[0]
+    assert "| 256x1x1 | 2 | 10.0 | 5.00 |" in rows[1]
