@@ -1345,10 +1345,18 @@ oe_status oe_ext_nf2ff_create(oe_engine* e, const size_t box[6], int directions,
     std::vector<Nf2ffFace> faces;
     std::vector<size_t> sig0;
     size_t n_sig = 0;
+    DftSurface surf{};
+    surf.nx = n[0]; surf.ny = n[1]; surf.nz = n[2];
     for (int ax = 0; ax < 3; ++ax)
         for (int side = 0; side < 2; ++side) {
             if (!(directions & (1 << (2 * ax + side)))) continue;
             const int a1 = (ax + 1) % 3, a2 = (ax + 2) % 3;
+            SurfFace& sf = surf.f[surf.n++];
+            sf.ax = ax;
+            sf.q = (int)(side ? hi[ax] : lo[ax]);
+            sf.lo1 = (int)lo[a1]; sf.n1 = (int)(hi[a1] - lo[a1] + 1);
+            sf.lo2 = (int)lo[a2]; sf.n2 = (int)(hi[a2] - lo[a2] + 1);
+            sf.sig0 = (int)n_sig;
             Nf2ffFace f{};
             f.axis = ax;
             f.sign = side ? 1 : -1;
@@ -1397,7 +1405,12 @@ oe_status oe_ext_nf2ff_create(oe_engine* e, const size_t box[6], int directions,
     c.precision = precision;
     c.record_steps = 0;
     c.time_stride = time_stride;
+    if (n_sig >= (size_t)INT32_MAX) return fail(OE_ERR_GRID, "NF2FF box of %zu signals exceeds 2^31", n_sig);
     OE_TRY(oe_ext_dft_create(e, &c, out));
+    // the kernel forms the node values from the faces (the same terms, in
+    // the same order, as the tables above); OE_SURFACE_DFT=0 reads the tables
+    const char* sv = getenv("OE_SURFACE_DFT");
+    if (!(sv && atoi(sv) == 0)) (*out)->dft.surf = surf;
     (*out)->is_nf2ff = true;
     (*out)->nf_faces = std::move(faces);
     (*out)->nf_sig0 = std::move(sig0);

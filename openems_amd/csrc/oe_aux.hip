@@ -282,9 +282,55 @@ __global__ void k_ade_post_multi(float* __restrict__ F, const float* __restrict_
 // LDS first.
 constexpr int DFT_FCHUNK = 128;
 
+// Signal m of a recording box (DftSurface): the node value of E_a1 / E_a2
+// (mean of the two edges meeting at the node, the lower one first) or H_a1
+// / H_a2 (mean of the four faces around it: the lower normal offset first,
+// within it the lower tangential offset), summed in float64 in the order
+// oe_ext_nf2ff_create writes the same terms into the tables.  The face is
+// picked with selects over the unrolled face list (no dynamic indexing of
+// the argument array).
+__device__ __forceinline__ double surface_signal(const float* __restrict__ F, size_t N, const DftSurface& s, int m,
+                                                 int& h) {
+    int ax = 0, q = 0, lo1 = 0, lo2 = 0, n2 = 1, sig0 = 0;
+#pragma unroll
+    for (int f = 0; f < MAX_SURF_FACES; ++f)
+        if (f < s.n && m >= s.f[f].sig0) {
+            ax = s.f[f].ax; q = s.f[f].q; lo1 = s.f[f].lo1; lo2 = s.f[f].lo2; n2 = s.f[f].n2; sig0 = s.f[f].sig0;
+        }
+    const int r = m - sig0, node = r >> 2, c = r & 3;
+    const int i1 = lo1 + node / n2, i2 = lo2 + node % n2;
+    const long long sx = (long long)s.ny * s.nz, sy = s.nz;
+    // strides of ax, a1 = (ax + 1) % 3, a2 = (ax + 2) % 3, and the field
+    // components a1, a2
+    const long long S0 = ax == 0 ? sx : (ax == 1 ? sy : 1);
+    const long long S1 = ax == 0 ? sy : (ax == 1 ? 1 : sx);
+    const long long S2 = ax == 0 ? 1 : (ax == 1 ? sx : sy);
+    const int a1 = ax == 2 ? 0 : ax + 1, a2 = ax == 0 ? 2 : ax - 1;
+    const long long base = q * S0 + i1 * S1 + i2 * S2;
+    double v = 0.0;
+    if (c < 2) {  // E_a1 / E_a2: the edges at -1 and 0 along that axis
+        const long long St = c == 0 ? S1 : S2;
+        const float* Fe = F + (size_t)(c == 0 ? a1 : a2) * N;
+        v += 0.5 * (double)Fe[base - St];
+        v += 0.5 * (double)Fe[base];
+        h = 0;
+    } else {  // H_a1 (other tangential axis a2) / H_a2 (other a1)
+        const long long So = c == 2 ? S2 : S1;
+        const float* Fh = F + (size_t)(3 + (c == 2 ? a1 : a2)) * N;
+        v += 0.25 * (double)Fh[base - S0 - So];
+        v += 0.25 * (double)Fh[base - S0];
+        v += 0.25 * (double)Fh[base - So];
+        v += 0.25 * (double)Fh[base];
+        h = 1;
+    }
+    return v;
+}
+
 // With step_dev the step comes from the device counter (a captured graph):
 // sample = sample + st, tE = (t0 + st + 1) dt, tH = (t0 + st + 1/2) dt, the
-// same doubles the host forms from the absolute step.
+// same doubles the host forms from the absolute step.  SURF: the signals of
+// a recording box from its face description (d.surf) instead of the tables.
+template <bool SURF>
 __global__ __launch_bounds__(256) void k_dft(const float* __restrict__ F, size_t N, DftDev d, uint64_t sample,
                                             double tE, double tH, const uint64_t* __restrict__ step_dev, double t0,
                                             double dt) {
@@ -304,10 +350,16 @@ __global__ __launch_bounds__(256) void k_dft(const float* __restrict__ F, size_t
     if (t < nm) {
         const int m = m0 + t;
         double s = 0.0;
-        for (uint64_t k = d.off[m]; k < d.off[m + 1]; ++k) s += d.w[k] * (double)F[(size_t)d.comp[k] * N + d.cell[k]];
+        int h = 0;
+        if constexpr (SURF) {
+            s = surface_signal(F, N, d.surf, m, h);
+        } else {
+            for (uint64_t k = d.off[m]; k < d.off[m + 1]; ++k) s += d.w[k] * (double)F[(size_t)d.comp[k] * N + d.cell[k]];
+            h = d.is_h[m] ? 1 : 0;
+        }
         if (d.series && sample < d.series_cap) d.series[sample * d.n_sig + m] = s;
         sv[t] = s;
-        hv[t] = d.is_h[m] ? 1 : 0;
+        hv[t] = (uint8_t)h;
     }
     for (int f0 = 0; f0 < d.n_f; f0 += DFT_FCHUNK) {
         const int nf = min(DFT_FCHUNK, d.n_f - f0);
@@ -626,16 +678,24 @@ hipError_t launch_upml_post(float* F, const Grid& g, const UpmlDev& u, bool e, h
 hipError_t launch_dft(const float* F, size_t N, const DftDev& d, uint64_t sample, double tE, double tH,
                       hipStream_t s) {
     if (d.n_sig <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_dft, dim3((d.n_sig + 255) / 256), dim3(256), 0, s, F, N, d, sample, tE, tH,
-                       (const uint64_t*)nullptr, 0.0, 0.0);
+    if (d.surf.n > 0)
+        hipLaunchKernelGGL(k_dft<true>, dim3((d.n_sig + 255) / 256), dim3(256), 0, s, F, N, d, sample, tE, tH,
+                           (const uint64_t*)nullptr, 0.0, 0.0);
+    else
+        hipLaunchKernelGGL(k_dft<false>, dim3((d.n_sig + 255) / 256), dim3(256), 0, s, F, N, d, sample, tE, tH,
+                           (const uint64_t*)nullptr, 0.0, 0.0);
     return hipGetLastError();
 }
 
 hipError_t launch_dft_dev(const float* F, size_t N, const DftDev& d, const uint64_t* step_dev, uint64_t sample0,
                           double t0, double dt, hipStream_t s) {
     if (d.n_sig <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_dft, dim3((d.n_sig + 255) / 256), dim3(256), 0, s, F, N, d, sample0, 0.0, 0.0, step_dev, t0,
-                       dt);
+    if (d.surf.n > 0)
+        hipLaunchKernelGGL(k_dft<true>, dim3((d.n_sig + 255) / 256), dim3(256), 0, s, F, N, d, sample0, 0.0, 0.0,
+                           step_dev, t0, dt);
+    else
+        hipLaunchKernelGGL(k_dft<false>, dim3((d.n_sig + 255) / 256), dim3(256), 0, s, F, N, d, sample0, 0.0, 0.0,
+                           step_dev, t0, dt);
     return hipGetLastError();
 }
 

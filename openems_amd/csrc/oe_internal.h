@@ -317,6 +317,21 @@ struct UpmlDev {
 hipError_t launch_upml_pre(float* F, const Grid& g, const UpmlDev& u, bool e, hipStream_t s);
 hipError_t launch_upml_post(float* F, const Grid& g, const UpmlDev& u, bool e, hipStream_t s);
 
+// A recording box's faces for the running DFT without term tables: face f
+// holds the nodes (i1, i2) of the plane q of axis ax, i1 along
+// a1 = (ax + 1) % 3 from lo1 (n1 nodes), i2 along a2 = (ax + 2) % 3 from lo2
+// (n2 nodes), and per node the 4 signals E_a1, E_a2, H_a1, H_a2 from
+// signal sig0 + 4 (node), node = (i1 - lo1) n2 + (i2 - lo2).
+struct SurfFace {
+    int ax, q, lo1, n1, lo2, n2, sig0;
+};
+constexpr int MAX_SURF_FACES = 6;
+struct DftSurface {
+    int n;  // faces (0: the signals come from the term tables)
+    int nx, ny, nz;
+    SurfFace f[MAX_SURF_FACES];
+};
+
 // Running DFT of weighted field signals (one thread per signal).
 struct DftDev {
     int n_sig, n_f, prec;
@@ -330,6 +345,7 @@ struct DftDev {
     float* acc32;    // [n_f][n_sig][4]: sum re, sum im, compensation re, im
     double* series;  // [cap][n_sig] or null
     uint64_t series_cap;
+    DftSurface surf;  // n > 0: signals formed from the box's indices, not the tables
 };
 hipError_t launch_dft(const float* F, size_t N, const DftDev& d, uint64_t sample, double tE, double tH,
                       hipStream_t s);

@@ -349,3 +349,37 @@ def test_fp32_contraction_after_non_finite_phasors(monkeypatch):
     assert np.array_equal(nf.dft_values(), nf2.dft_values())
     assert np.array_equal(again.e_theta, fresh.e_theta) and np.array_equal(again.e_phi, fresh.e_phi)
     assert again.prad == fresh.prad and np.abs(fresh.e_theta).max() > 0
+
+
+def _box_phasors(shape, box, directions, precision, stride, steps=240):
+    d = 1e-3
+    op = Operator.uniform(*shape, d)
+    eng = HipEngine(op)
+    cp = eng.cpml(CpmlBoundaries(layers=(6,) * 6, mode=MODE_PHYSICAL))
+    nf = eng.nf2ff_box(box, [7e9, 11e9, 13e9], directions=directions, precision=precision, time_stride=stride)
+    t0, tau = X.openems_gaussian_params(5e9)
+    smp = X.schedule(lambda t: X.gaussian_modulated(t, 10e9, t0, tau), 0, steps, op.dt)
+    src = tuple(n // 2 for n in shape)
+    eng.run_batch(EngineBatch(num_steps=steps, excitations=[ScheduledExcitation(src, 0, smp),
+                                                            ScheduledExcitation(src, 2, smp)],
+                              extensions=[cp, nf]))
+    return nf.dft_values(), np.concatenate(eng.read_fields())
+
+
+@pytest.mark.parametrize("directions", [0, 0b100110])
+@pytest.mark.parametrize("precision,stride", [(0, 1), (1, 1), (0, 3)])
+def test_surface_dft_equals_term_tables(monkeypatch, directions, precision, stride):
+    """The recording box's DFT formed from the faces' own indices equals the
+    table-driven one bit for bit (same terms, same order, same float64
+    sums), on a box of unequal sides in a grid of unequal sides, with every
+    face or a subset (x-hi, y-lo, z-hi), in FP64 and FP32-Kahan, sampled
+    every step (graph replay) or every 3rd."""
+    shape, box = (40, 36, 44), (9, 31, 8, 27, 10, 34)
+    say(f"[nf2ff] surface DFT vs tables: directions {directions:#b}, precision {precision}, stride {stride}")
+    monkeypatch.setenv("OE_SURFACE_DFT", "0")
+    ref, f_ref = _box_phasors(shape, box, directions, precision, stride)
+    monkeypatch.setenv("OE_SURFACE_DFT", "1")
+    got, f_got = _box_phasors(shape, box, directions, precision, stride)
+    assert np.array_equal(f_got, f_ref)
+    assert got.shape == ref.shape and np.abs(ref).max() > 0
+    assert np.array_equal(got, ref)
