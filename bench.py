@@ -1005,8 +1005,8 @@ def main():
         "cpu_baseline": cpu,
     }
     if not args.no_subrecords and not args.cavity and args.n == 256:
-        # the north-star figures beside the headline: the 512^3 perf point of
-        # the same workload, and the 256-structure sweep
+        # the other figures of the metric beside the headline: the 512^3
+        # perf point of the same workload, and the 256-structure sweep
         big = W.cpml_boxes(n=512, steps=args.perf_point_steps)
         p5 = yee_point(big, mode, args.perf_point_steps, 20, 4, "", "auto", False, args.mode)
         out["perf_point_512"] = {k: p5[k] for k in ("config", "steps", "ms_per_step", "mcells", "hbm_frac_step",

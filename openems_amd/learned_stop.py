@@ -276,7 +276,7 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
     are evaluated at a checkpoint (a tail of at most max_tail_energy of the
     signal energy needs most of the decay behind it), and the least-squares
     fits use every fit_stride-th step.  The structures of a checkpoint are
-    judged on `workers` host threads (default min(8, cpus))."""
+    assessed on `workers` host threads (default min(8, cpus))."""
     d = model.dims
     workers = workers or min(8, os.cpu_count() or 1)
     early = window + check
@@ -349,7 +349,7 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
             s_all, _, _ = mode_params(modes, d.modes)
         a0, a1 = pos - early, pos - check
 
-        def judge(k: int):
+        def assess(k: int):
             """(accepted, S11, residual, tail energy fraction) of structure
             still[k] at this checkpoint: gates 1-3 of the docstring."""
             q = still[k]
@@ -398,13 +398,13 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
                 return False, None, res_q, frac
             return True, s_pred, res_q, frac
 
-        # the structures are independent: judged on host threads (LAPACK
+        # the structures are independent: assessed on host threads (LAPACK
         # releases the interpreter lock), each with a one-thread BLAS
         with _one_thread_blas(), ThreadPoolExecutor(max_workers=workers) as ex:
-            verdicts = list(ex.map(judge, range(len(still))))
+            outcomes = list(ex.map(assess, range(len(still))))
         rejected = []
         for k, q in enumerate(still):
-            ok, s_q, resid[q], tail_frac[q] = verdicts[k]
+            ok, s_q, resid[q], tail_frac[q] = outcomes[k]
             if ok:
                 accepted[q] = True
                 s11[q] = s_q
