@@ -668,7 +668,7 @@ def bench_learned(args):
         f"steps plain {int(plain.steps.sum())} learned {int(learned.steps_learned.sum())}")
     qs = (lambda v: [float(np.median(v)), float(np.percentile(v, 90)), float(v.max())] if len(v) else None)
     # the same gates and horizon with classical matrix-pencil poles: what an
-    # exact-structure extrapolator reaches against the -50 dB judge
+    # exact-structure extrapolator reaches against the -50 dB plain run
     pen = LS.learned_sweep(model, test_s, T, window, dec, check, stride=args.learned_stride, poles="pencil",
                            log=lambda m: log(f"[bench] learned (pencil): {m}"))
     dp = np.abs(pen.s11 - plain.s11).max(axis=1)

@@ -116,11 +116,11 @@ def test_pencil_poles_recover_damped_modes():
         assert np.min(np.abs(got - sm)) < 1e-8
 
 
-def test_learned_s11_equals_judge_for_exact_modes():
+def test_learned_s11_equals_plain_for_exact_modes():
     """For port signals that are exact damped modes, the learned S11 (the
     simulated DFT to the checkpoint, modes refitted from the window, their
-    closed-form DFT from the checkpoint to the judge's stop) equals the
-    judge's own S11 over [0, S): the horizon rule leaves nothing but the
+    closed-form DFT from the checkpoint to the plain run's stop) equals the
+    plain run's own S11 over [0, S): the horizon rule leaves nothing but the
     extrapolation itself."""
     from openems_amd.learned_stop import calc_s11, modes_eval, refit_residues, tail_dft
     dt, D, P, W, S = 9.5e-13, 8, 2304, 2048, 5001
@@ -134,14 +134,14 @@ def test_learned_s11_equals_judge_for_exact_modes():
         kk = np.arange(k0, k1)
         return 2 * dt * np.exp(-2j * math.pi * np.outer(fr, (kk + off) * dt)) @ x[k0:k1]
 
-    judge = calc_s11(dft(u, 0, S, 1.0), dft(i, 0, S, 0.5))
+    plain = calc_s11(dft(u, 0, S, 1.0), dft(i, 0, S, 0.5))
     aU, _ = refit_residues(u, s, P - W, P, P, D, 4)
     aI, _ = refit_residues(i, s, P - W, P, P, D, 4)
     U = dft(u, 0, P, 1.0) + tail_dft(s, aU, P, P, S, D, dt, 1.0, fr)
     I = dft(i, 0, P, 0.5) + tail_dft(s, aI, P, P, S, D, dt, 0.5, fr)
-    assert np.abs(calc_s11(U, I) - judge).max() < 1e-9
-    # stopping the tail at the cap instead of the judge's stop does not
-    # reproduce the judge (the bias the horizon rule removes)
+    assert np.abs(calc_s11(U, I) - plain).max() < 1e-9
+    # stopping the tail at the cap instead of the plain run's stop does not
+    # reproduce the plain run (the bias the horizon rule removes)
     U2 = dft(u, 0, P, 1.0) + tail_dft(s, aU, P, P, 20000, D, dt, 1.0, fr)
     I2 = dft(i, 0, P, 0.5) + tail_dft(s, aI, P, P, 20000, D, dt, 0.5, fr)
-    assert np.abs(calc_s11(U2, I2) - judge).max() > 1e-4
+    assert np.abs(calc_s11(U2, I2) - plain).max() > 1e-4
