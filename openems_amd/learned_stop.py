@@ -29,6 +29,7 @@ from __future__ import annotations
 import contextlib
 import math
 import os
+import threading
 import time
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field, replace
@@ -245,12 +246,98 @@ def _one_thread_blas():
     return threadpool_limits(1)
 
 
+@dataclass
+class _Checkpoint:
+    """What the assessment of the structures at checkpoint pos reads: the
+    port series and DFT sums as they were at pos, the energy history up to
+    pos, the poles (network) and the gate settings."""
+    pos: int
+    still: List[int]
+    sers: list
+    dfts: list
+    hist: list                     # (timesteps, energies) per structure of still
+    s_all: Optional[np.ndarray]    # network poles per structure, or None (pencil)
+    window: int
+    check: int
+    decimate: int
+    max_steps: int
+    dt: float
+    freqs: np.ndarray
+    max_residual: float
+    max_tail_energy: float
+    horizon_window: int
+    max_horizon_change: float
+    decay_db: float
+    decay_check: int
+    pencil_order: int
+    fit_stride: int
+
+
+def _assess(c: _Checkpoint, k: int):
+    """(accepted, S11, residual, tail energy fraction) of structure
+    c.still[k] at checkpoint c.pos: the gates of learned_sweep."""
+    pos, dec = c.pos, c.decimate
+    a0, a1 = pos - c.window - c.check, pos - c.check
+    u, i = c.sers[k][:, 0], c.sers[k][:, 1]
+    if c.s_all is not None:
+        s = c.s_all[k]
+    else:
+        try:
+            s = pencil_poles(u, i, a0, a1, dec, c.pencil_order)
+        except np.linalg.LinAlgError:
+            return False, None, np.nan, np.nan
+        if len(s) == 0:
+            return False, None, np.nan, np.nan
+    aUI, _ = refit_residues(np.stack([u, i]), s, a0, a1, a1, dec, c.fit_stride)
+    kk = np.arange(a1, pos)
+    eu = float(np.sum(u[a0:a1] ** 2))
+    ei = float(np.sum(i[a0:a1] ** 2))
+    ru = float(np.sum((modes_eval(s, aUI[0], kk, a1, dec) - u[a1:pos]) ** 2)) / max(eu, 1e-300)
+    ri = float(np.sum((modes_eval(s, aUI[1], kk, a1, dec) - i[a1:pos]) ** 2)) / max(ei, 1e-300)
+    res_q = max(ru, ri)
+    if not _passes(res_q, c.max_residual):
+        return False, None, res_q, np.nan
+    aUI, _ = refit_residues(np.stack([u, i]), s, pos - c.window, pos, pos, dec, c.fit_stride)
+    aU, aI = aUI[0], aUI[1]
+    ts, en = c.hist[k]
+    stop = predict_stop(ts, en, pos, c.max_steps, c.decay_db, c.decay_check)
+    sv = max(float(np.abs(u[a0:pos]).max()), 1e-300)
+    si = max(float(np.abs(i[a0:pos]).max()), 1e-300)
+    kt = np.arange(pos, stop, dec)
+    e_tail = float(np.sum((modes_eval(s, aU, kt, pos, dec) / sv) ** 2)
+                   + np.sum((modes_eval(s, aI, kt, pos, dec) / si) ** 2))
+    e_run = float(np.sum((u[:pos:dec] / sv) ** 2) + np.sum((i[:pos:dec] / si) ** 2))
+    frac = e_tail / max(e_run, 1e-300)
+    if not _passes(frac, c.max_tail_energy):
+        return False, None, res_q, frac
+
+    def s11_to(end):
+        U = c.dfts[k][:, 0] + tail_dft(s, aU, pos, pos, end, dec, c.dt, 1.0, c.freqs)
+        I = c.dfts[k][:, 1] + tail_dft(s, aI, pos, pos, end, dec, c.dt, 0.5, c.freqs)
+        return calc_s11(U, I)
+
+    s_pred = s11_to(stop)
+    change = max(float(np.abs(s11_to(max(pos, stop - c.horizon_window)) - s_pred).max()),
+                 float(np.abs(s11_to(min(c.max_steps, stop + c.horizon_window)) - s_pred).max()))
+    if not _passes(change, c.max_horizon_change):
+        return False, None, res_q, frac
+    return True, s_pred, res_q, frac
+
+
+def _assess_all(c: _Checkpoint, workers: int):
+    """_assess over the checkpoint's structures, which are independent, on
+    host threads (LAPACK releases the interpreter lock), each with a
+    one-thread BLAS."""
+    with _one_thread_blas(), ThreadPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(lambda k: _assess(c, k), range(len(c.still))))
+
+
 def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, decimate: int, check: int,
                   max_residual: float = 1e-6, freqs=None, decay_db: float = 50.0, stride: Optional[int] = None,
                   decay_check: int = 100, max_tail_energy: float = 1e-4, horizon_window: int = 300,
                   max_horizon_change: float = 3e-4, poles: str = "network", pencil_order: int = 24,
                   min_decay_db: float = 20.0, fit_stride: int = 4, workers: Optional[int] = None,
-                  log=None) -> LearnedResult:
+                  overlap: bool = True, log=None) -> LearnedResult:
     """All structures run together from checkpoint to checkpoint (the first
     at window + check steps, then every `stride`).  At a checkpoint P the
     network maps the decimated V and I of [P - check - window, P - check) to
@@ -276,7 +363,11 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
     are evaluated at a checkpoint (a tail of at most max_tail_energy of the
     signal energy needs most of the decay behind it), and the least-squares
     fits use every fit_stride-th step.  The structures of a checkpoint are
-    assessed on `workers` host threads (default min(8, cpus))."""
+    assessed on `workers` host threads (default min(8, cpus)).  With
+    overlap, that assessment runs while the GPU steps the next segment, in
+    which the candidates take part: one accepted at P keeps the S11 formed
+    from its state at P, and the segment it ran beyond P counts in its steps
+    (overlap=False: it stops at P)."""
     d = model.dims
     workers = workers or min(8, os.cpu_count() or 1)
     early = window + check
@@ -298,6 +389,21 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
     checkpoints = []
     wall = 0.0
     t_inf = 0.0
+    pending = None  # the last checkpoint, assessed while the next segment runs
+
+    def apply(c: _Checkpoint, outcomes):
+        rejected = []
+        for k, q in enumerate(c.still):
+            ok, s_q, resid[q], tail_frac[q] = outcomes[k]
+            if ok:
+                accepted[q] = True
+                s11[q] = s_q
+            else:
+                rejected.append(q)
+        if log:
+            log(f"checkpoint {c.pos}: {int(accepted.sum())} accepted")
+        return rejected
+
     while live:
         target = min(max_steps, early if pos < early else pos + stride)
         sw = Sweep()
@@ -306,11 +412,37 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
             sw.add(eng, cp, port, shifted(exc, pos))
         if ci is not None:
             sw.set_ci(ci)
+        seg = {}
+
+        def run_segment(sw=sw, n_steps=target - pos, tune=ci is None):
+            s0 = time.perf_counter()
+            try:
+                seg["res"] = sw.run(n_steps, energy_decay_db=decay_db, check_interval=decay_check, autotune=tune)
+            except BaseException as err:  # re-raised on the calling thread
+                seg["err"] = err
+            seg["t"] = time.perf_counter() - s0
+
         t0 = time.perf_counter()
-        res = sw.run(target - pos, energy_decay_db=decay_db, check_interval=decay_check, autotune=ci is None)
-        wall += time.perf_counter() - t0
+        outcomes = None
+        if pending is not None:
+            th = threading.Thread(target=run_segment)
+            th.start()
+            try:
+                outcomes = _assess_all(pending, workers)
+            finally:
+                th.join()
+        else:
+            run_segment()
+        if "err" in seg:
+            raise seg["err"]
+        wall += seg["t"]
+        t_inf += time.perf_counter() - t0 - seg["t"]  # host work the segment did not hide
+        res = seg["res"]
         if ci is None:
             ci = sw.tuning()["ci"]
+        if pending is not None:
+            apply(pending, outcomes)
+            pending = None
         pos = target
         still, waiting = [], []
         for k, q in enumerate(live):
@@ -318,6 +450,8 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
             ts, en = sw.energy_history(k)
             hist_t[q].append(ts)
             hist_e[q].append(en)
+            if accepted[q]:
+                continue  # stopped at the last checkpoint; this segment's state is not used
             if res[k].termination_reason != 0 or pos >= max_steps:
                 s11[q] = built[q][3].s11(50.0)
                 continue
@@ -341,79 +475,24 @@ def learned_sweep(model: Extrapolator, structs, max_steps: int, window: int, dec
             X[k], _, _ = _window(ser[:, 0], ser[:, 1], pos - early, window, decimate, 0)
             sers.append(ser)
             dfts.append(port.dft_values())
+        s_all = None
         if poles == "network":
             modes = np.zeros((len(still), 6 * d.modes), np.float32)
             for b0 in range(0, len(still), d.batch):
                 m, _ = model.forward(X[b0:b0 + d.batch], tails=False)
                 modes[b0:b0 + d.batch] = m
             s_all, _, _ = mode_params(modes, d.modes)
-        a0, a1 = pos - early, pos - check
-
-        def assess(k: int):
-            """(accepted, S11, residual, tail energy fraction) of structure
-            still[k] at this checkpoint: gates 1-3 of the docstring."""
-            q = still[k]
-            if poles == "network":
-                s = s_all[k]
-            else:
-                try:
-                    s = pencil_poles(sers[k][:, 0], sers[k][:, 1], a0, a1, decimate, pencil_order)
-                except np.linalg.LinAlgError:
-                    return False, None, np.nan, np.nan
-                if len(s) == 0:
-                    return False, None, np.nan, np.nan
-            u, i = sers[k][:, 0], sers[k][:, 1]
-            aUI, _ = refit_residues(np.stack([u, i]), s, a0, a1, a1, decimate, fit_stride)
-            kk = np.arange(a1, pos)
-            eu = float(np.sum(u[a0:a1] ** 2))
-            ei = float(np.sum(i[a0:a1] ** 2))
-            ru = float(np.sum((modes_eval(s, aUI[0], kk, a1, decimate) - u[a1:pos]) ** 2)) / max(eu, 1e-300)
-            ri = float(np.sum((modes_eval(s, aUI[1], kk, a1, decimate) - i[a1:pos]) ** 2)) / max(ei, 1e-300)
-            res_q = max(ru, ri)
-            if not _passes(res_q, max_residual):
-                return False, None, res_q, np.nan
-            aUI, _ = refit_residues(np.stack([u, i]), s, pos - window, pos, pos, decimate, fit_stride)
-            aU, aI = aUI[0], aUI[1]
-            stop = predict_stop(np.concatenate(hist_t[q]), np.concatenate(hist_e[q]), pos, max_steps, decay_db,
-                                decay_check)
-            sv = max(float(np.abs(u[a0:pos]).max()), 1e-300)
-            si = max(float(np.abs(i[a0:pos]).max()), 1e-300)
-            kt = np.arange(pos, stop, decimate)
-            e_tail = float(np.sum((modes_eval(s, aU, kt, pos, decimate) / sv) ** 2)
-                           + np.sum((modes_eval(s, aI, kt, pos, decimate) / si) ** 2))
-            e_run = float(np.sum((u[:pos:decimate] / sv) ** 2) + np.sum((i[:pos:decimate] / si) ** 2))
-            frac = e_tail / max(e_run, 1e-300)
-            if not _passes(frac, max_tail_energy):
-                return False, None, res_q, frac
-
-            def s11_to(end):
-                U = dfts[k][:, 0] + tail_dft(s, aU, pos, pos, end, decimate, dt, 1.0, fr)
-                I = dfts[k][:, 1] + tail_dft(s, aI, pos, pos, end, decimate, dt, 0.5, fr)
-                return calc_s11(U, I)
-
-            s_pred = s11_to(stop)
-            change = max(float(np.abs(s11_to(max(pos, stop - horizon_window)) - s_pred).max()),
-                         float(np.abs(s11_to(min(max_steps, stop + horizon_window)) - s_pred).max()))
-            if not _passes(change, max_horizon_change):
-                return False, None, res_q, frac
-            return True, s_pred, res_q, frac
-
-        # the structures are independent: assessed on host threads (LAPACK
-        # releases the interpreter lock), each with a one-thread BLAS
-        with _one_thread_blas(), ThreadPoolExecutor(max_workers=workers) as ex:
-            outcomes = list(ex.map(assess, range(len(still))))
-        rejected = []
-        for k, q in enumerate(still):
-            ok, s_q, resid[q], tail_frac[q] = outcomes[k]
-            if ok:
-                accepted[q] = True
-                s11[q] = s_q
-            else:
-                rejected.append(q)
+        ctx = _Checkpoint(pos, list(still), sers, dfts,
+                          [(np.concatenate(hist_t[q]), np.concatenate(hist_e[q])) for q in still], s_all,
+                          window, check, decimate, max_steps, dt, fr, max_residual, max_tail_energy,
+                          horizon_window, max_horizon_change, decay_db, decay_check, pencil_order, fit_stride)
+        if overlap:
+            pending = ctx
+            live = still + waiting  # the candidates run on while they are assessed
+        else:
+            rejected = apply(ctx, _assess_all(ctx, workers))
+            live = rejected + waiting
         t_inf += time.perf_counter() - t1
-        live = rejected + waiting
-        if log:
-            log(f"checkpoint {pos}: {int(accepted.sum())} accepted, {len(live)} continue")
     return LearnedResult(s11, accepted, resid, wall + t_inf, t_inf, steps, checkpoints, tail_frac)
 
 

@@ -48,8 +48,8 @@ def test_rejected_structures_continue_to_the_plain_result():
 
 def test_open_gates_stop_at_the_first_checkpoint():
     structs = _structs()
-    res = LS.learned_sweep(_model(), structs, MAX_STEPS, WINDOW, DEC, CHECK, max_residual=np.inf,
-                           max_tail_energy=np.inf, max_horizon_change=np.inf, min_decay_db=0.0, stride=448)
+    kw = dict(max_residual=np.inf, max_tail_energy=np.inf, max_horizon_change=np.inf, min_decay_db=0.0, stride=448)
+    res = LS.learned_sweep(_model(), structs, MAX_STEPS, WINDOW, DEC, CHECK, overlap=False, **kw)
     say(f"[learned] open gates: steps {res.steps_learned.tolist()}, checkpoints {res.checkpoints}")
     early = WINDOW + CHECK
     # a structure stops at the first checkpoint unless it decayed before it
@@ -58,3 +58,13 @@ def test_open_gates_stop_at_the_first_checkpoint():
         assert res.steps_learned[q] <= early
     assert res.accepted.any()
     assert np.isfinite(res.s11).all()
+    # assessed while the next segment runs: the same structures accepted with
+    # the same S11 (formed from the state at the checkpoint); each accepted
+    # one ran that segment too
+    ov = LS.learned_sweep(_model(), structs, MAX_STEPS, WINDOW, DEC, CHECK, overlap=True, **kw)
+    say(f"[learned] open gates, overlapped: steps {ov.steps_learned.tolist()}")
+    assert np.array_equal(ov.accepted, res.accepted)
+    assert np.array_equal(ov.s11, res.s11)
+    acc = res.accepted
+    assert np.all(ov.steps_learned[acc] > early)
+    assert np.array_equal(ov.steps_learned[~acc], res.steps_learned[~acc])
