@@ -366,15 +366,17 @@ def _box_phasors(shape, box, directions, precision, stride, steps=240):
     return nf.dft_values(), np.concatenate(eng.read_fields())
 
 
+@pytest.mark.parametrize("box", [(9, 31, 8, 27, 10, 34), (1, 39, 5, 30, 1, 43)], ids=["inner", "to-the-edges"])
 @pytest.mark.parametrize("directions", [0, 0b100110])
 @pytest.mark.parametrize("precision,stride", [(0, 1), (1, 1), (0, 3)])
-def test_surface_dft_equals_term_tables(monkeypatch, directions, precision, stride):
+def test_surface_dft_equals_term_tables(monkeypatch, directions, precision, stride, box):
     """The recording box's DFT formed from the faces' own indices equals the
     table-driven one bit for bit (same terms, same order, same float64
-    sums), on a box of unequal sides in a grid of unequal sides, with every
-    face or a subset (x-hi, y-lo, z-hi), in FP64 and FP32-Kahan, sampled
-    every step (graph replay) or every 3rd."""
-    shape, box = (40, 36, 44), (9, 31, 8, 27, 10, 34)
+    sums), on a box of unequal sides in a grid of unequal sides (also one
+    reaching the first and last nodes the box may hold, 1 and n - 1 on x
+    and z), with every face or a subset (x-hi, y-lo, z-hi), in FP64 and
+    FP32-Kahan, sampled every step (graph replay) or every 3rd."""
+    shape = (40, 36, 44)
     say(f"[nf2ff] surface DFT vs tables: directions {directions:#b}, precision {precision}, stride {stride}")
     monkeypatch.setenv("OE_SURFACE_DFT", "0")
     ref, f_ref = _box_phasors(shape, box, directions, precision, stride)
