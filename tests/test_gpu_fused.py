@@ -75,6 +75,34 @@ def test_fused_cpml_all_faces(mode):
     _check_psi(res)
 
 
+def _many_boxes(n, shape, seed=7):
+    """n small dielectric boxes of distinct permittivity: a material table
+    of n + 1 E rows (more than a byte indexes once n > 255)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(n):
+        x0 = int(rng.integers(0, shape[0] - 4))
+        y0 = int(rng.integers(0, shape[1] - 4))
+        z0 = int(rng.integers(0, shape[2] - 8))
+        out.append((1.5 + 0.01 * q, 1.0, 0.0, 0.0, (x0, x0 + 4, y0, y0 + 4, z0, z0 + 8)))
+    return out
+
+
+def test_fused_many_materials_two_byte_index():
+    """More than 256 E table rows: the fused pass reads the 16-bit row index
+    (the byte-wide copy exists only up to 256 rows)."""
+    shape = (44, 40, 128)
+    sc = Scenario("fused_many_rows", shape, 0.5e-3, 120, cpml=dict(layers=(8,) * 6, mode=MODE_PHYSICAL),
+                  materials=_many_boxes(300, shape), excitations=_exc(), probes=[(30, 18, 64)], fused=1)
+    res = run_pair(sc)
+    rows = res["eng"].material_table(-1)
+    say(f"[fused] table rows (E, H): {rows}")
+    assert rows[0] > 256 and rows[1] == 1
+    _assert_fused(res["eng"])
+    assert_parity(res, sc.name)
+    _check_psi(res)
+
+
 def test_fused_cpml_z_faces_only_and_kappa():
     sc = Scenario("fused_cpml_z", (36, 30, 136), 0.5e-3, 180,
                   cpml=dict(layers=(0, 0, 0, 0, 8, 8), kappa_max=4.0, alpha_max=0.03, mode=MODE_PHYSICAL),
